@@ -1,0 +1,171 @@
+/*
+ * lcr.h -- C ABI of liblcr, the MI355X fine-registration library.
+ *
+ * Drop-in boundary for the reference's fine-registration hot path:
+ *   pcl::IterativeClosestPoint<PointXYZRGBNormal, PointXYZRGBNormal>
+ *     (pcl_tools/fine_registration.cpp:105-126), and
+ *   pcl::NormalEstimation<PointXYZRGB, Normal>
+ *     (pcl_tools/normal_estimation.cpp:84-118).
+ * Plain pointers and sizes only.  Every call runs on the GPU through
+ * hand-written HIP kernels; there is no CPU fallback.  Errors: functions return
+ * 0 (or a documented non-negative value) on success and a negative LCR_E_* code on
+ * failure; lcr_last_error() describes the last failure on the calling thread.
+ * Handles are not thread-safe (like a pcl::Registration object): use one per
+ * thread.  Input arrays are copied to the device; the caller keeps ownership.
+ */
+#ifndef LCR_H
+#define LCR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define LCR_OK 0
+#define LCR_E_ARG (-1)      /* invalid argument */
+#define LCR_E_HIP (-2)      /* HIP runtime error (no device, launch failure, ...) */
+#define LCR_E_STATE (-3)    /* call out of order (e.g. no source set) */
+#define LCR_E_NORMALS (-4)  /* point-to-plane without target normals */
+#define LCR_E_LIMIT (-5)    /* size beyond a documented limit */
+
+/* pcl::registration::DefaultConvergenceCriteria::ConvergenceState */
+#define LCR_CONV_NOT_CONVERGED 0
+#define LCR_CONV_ITERATIONS 1
+#define LCR_CONV_TRANSFORM 2
+#define LCR_CONV_ABS_MSE 3
+#define LCR_CONV_REL_MSE 4
+#define LCR_CONV_NO_CORRESPONDENCES 5
+
+/* correspondence search arithmetic */
+#define LCR_PREC_FP32 0 /* exact FP32 scan (FLANN distance form) */
+
+/* ICP parameters.  Mirrors the setters called at fine_registration.cpp:112-118:
+ * setMaxCorrespondenceDistance, setMaximumIterations, setTransformationEpsilon,
+ * setEuclideanFitnessEpsilon; point_to_plane selects
+ * TransformationEstimationPointToPlaneLLS instead of TransformationEstimationSVD. */
+typedef struct {
+  double max_corr_dist;
+  int32_t max_iterations;
+  double transformation_epsilon;
+  double euclidean_fitness_epsilon;
+  int32_t point_to_plane;
+  int32_t precision_mode; /* LCR_PREC_* */
+  double cell_factor;     /* grid cell edge / estimated point spacing; <= 0 = default */
+  int32_t tile;           /* source points per work tile: 64, 128 or 256; 0 = default */
+} lcr_icp_params;
+
+typedef struct lcr_icp lcr_icp;
+
+/* PCL's defaults: max_iterations 10, max_corr_dist sqrt(DBL_MAX), epsilons 0 and
+ * -DBL_MAX (pcl/registration/registration.h constructor). */
+void lcr_icp_params_default(lcr_icp_params *p);
+
+/* Selects the HIP device used by handles created afterwards on this thread. */
+int lcr_init(int device);
+
+/* `IterativeClosestPoint icp;` (fine_registration.cpp:105) */
+int lcr_icp_create(const lcr_icp_params *p, lcr_icp **out);
+void lcr_icp_destroy(lcr_icp *h);
+/* setMaxCorrespondenceDistance / setMaximumIterations / setTransformationEpsilon /
+ * setEuclideanFitnessEpsilon (fine_registration.cpp:112-118) */
+int lcr_icp_set_params(lcr_icp *h, const lcr_icp_params *p);
+
+/* setInputSource (fine_registration.cpp:108).  xyz: n points, x,y,z floats at the
+ * start of each record of `stride_bytes` (12 for packed xyz, 48 for PCL's
+ * PointXYZRGBNormal); nrm: nullable, same stride. */
+int lcr_icp_set_source(lcr_icp *h, const float *xyz, const float *nrm, int64_t n,
+                       int64_t stride_bytes);
+/* setInputTarget (fine_registration.cpp:109).  The target's search grid is built
+ * once per new target, on the next align. */
+int lcr_icp_set_target(lcr_icp *h, const float *xyz, const float *nrm, int64_t n,
+                       int64_t stride_bytes);
+
+/* align(output) / align(output, guess) (fine_registration.cpp:121).  guess:
+ * row-major 4x4 or NULL (identity).  out_xyz / out_nrm (nullable): the source
+ * transformed by the final transform, 3 floats per source point in input order
+ * (IterativeClosestPoint::transformCloud; non-finite points copied). */
+int lcr_icp_align(lcr_icp *h, const double *guess, float *out_xyz, float *out_nrm);
+
+/* getFinalTransformation (fine_registration.cpp:124): row-major 4x4. */
+int lcr_icp_final_transform(const lcr_icp *h, double T[16]);
+/* hasConverged (fine_registration.cpp:125): 1, 0, or < 0 on error. */
+int lcr_icp_has_converged(const lcr_icp *h);
+/* getFitnessScore(max_range) (fine_registration.cpp:126): mean squared distance
+ * from each source point, transformed by the final transform, to its nearest
+ * target point, over points whose squared distance is <= max_range (PCL compares
+ * the squared distance with max_range as given); DBL_MAX if none. */
+int lcr_icp_fitness(lcr_icp *h, double max_range, double *out);
+/* nr_iterations_ and DefaultConvergenceCriteria::getConvergenceState */
+int lcr_icp_iterations(const lcr_icp *h);
+int lcr_icp_convergence_state(const lcr_icp *h);
+/* CorrespondenceEstimation::determineCorrespondences for the source transformed by
+ * T (row-major, NULL = the current final transform): per source point the target
+ * index of its nearest neighbour, or -1 when d2 > max_corr_dist^2 or the point is
+ * not finite, and that nearest d2 (FLANN float form; +inf if none was searched). */
+int lcr_icp_correspondences(lcr_icp *h, const double *T, int32_t *idx, float *d2);
+
+/* pcl::NormalEstimation with setKSearch(k) and a viewpoint, then the CLI's flip
+ * of all normals when flip_all != 0 (normal_estimation.cpp:84-118).  xyz: n
+ * records of stride_bytes.  nrm: 3 floats per point, curv: 1 float per point
+ * (nullable); NaN where fewer than 3 neighbours exist or the point is not finite. */
+int lcr_normals(const float *xyz, int64_t n, int64_t stride_bytes, int32_t k, const float vp[3],
+                int32_t flip_all, float *nrm, float *curv);
+
+/* ---- batched alignment: every adjacent pair of a capture cycle (and optional
+ * pose hypotheses per pair) in one launch per ICP iteration.  Replaces the
+ * per-pair process loop of scripts/alignment.sh:106-113. */
+typedef struct {
+  int32_t n_clouds;
+  const float *const *xyz;      /* per cloud: packed x,y,z floats */
+  const float *const *nrm;      /* per cloud normals (packed) or NULL: computed on
+                                   the device when point_to_plane is set */
+  const int64_t *n_points;
+  int32_t n_pairs;
+  const int32_t *pair_src;      /* source cloud of each pair (capture i) */
+  const int32_t *pair_tgt;      /* target cloud of each pair (capture i-1) */
+  int32_t n_hyp;                /* initial-pose hypotheses per pair, >= 1 */
+  const double *guesses;        /* n_pairs * n_hyp row-major 4x4, or NULL = identity */
+  lcr_icp_params params;
+  int32_t normals_k;            /* k of the on-device normal estimation (default 30) */
+  const float *viewpoints;      /* per cloud 3 floats, or NULL = origin */
+} lcr_cycle_desc;
+
+typedef struct {
+  double *T;            /* n_pairs * 16: best hypothesis's final transform */
+  int32_t *iterations;  /* n_pairs */
+  int32_t *converged;   /* n_pairs */
+  int32_t *state;       /* n_pairs */
+  double *fitness;      /* n_pairs: getFitnessScore() of the best hypothesis */
+  int32_t *best_hyp;    /* n_pairs: index of the chosen hypothesis (nullable) */
+} lcr_cycle_result;
+
+typedef struct lcr_cycle lcr_cycle;
+/* Uploads the clouds once and prepares the batched iteration. */
+int lcr_cycle_create(const lcr_cycle_desc *d, lcr_cycle **out);
+/* normals (when needed) -> grids -> ICP iterations (one hipGraph) -> fitness ->
+ * hypothesis selection.  Synchronous. */
+int lcr_cycle_run(lcr_cycle *c);
+int lcr_cycle_results(lcr_cycle *c, lcr_cycle_result *r);
+/* per-instance (pair x hypothesis) results before selection; arrays of
+ * n_pairs * n_hyp entries (each pointer nullable) */
+int lcr_cycle_instances(lcr_cycle *c, double *T, int32_t *iterations, int32_t *state,
+                        double *fitness);
+void lcr_cycle_destroy(lcr_cycle *c);
+/* One-shot: create, run, results, destroy. */
+int lcr_icp_batch_align(const lcr_cycle_desc *d, lcr_cycle_result *r);
+
+/* ---- profiling: per-kernel GPU time measured with HIP events on the stream the
+ * kernel runs on.  names: "icp_iteration", "fitness", "grid_build", "normals". */
+int lcr_prof_enable(int32_t on);
+int lcr_prof_reset(void);
+int lcr_prof_read(const char *name, int64_t *launches, double *total_ms);
+
+const char *lcr_last_error(void);
+const char *lcr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

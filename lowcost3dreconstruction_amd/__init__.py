@@ -1,0 +1,397 @@
+"""lowcost3dreconstruction_amd -- MI355X fine registration (ICP + kNN/PCA normals).
+
+Python mirror of the PCL interface the reference drives
+(pcl_tools/fine_registration.cpp:105-126, pcl_tools/normal_estimation.cpp:84-118),
+bound with ctypes to the C ABI of ``liblcr.so`` (include/lcr.h).  Every call runs
+the hand-written HIP kernels in ``csrc/``; there is no CPU fallback: importing
+this package fails if ``liblcr.so`` has not been built, and calls fail with
+``LcrError`` when no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblcr.so")
+
+DBL_MAX = sys.float_info.max
+
+# pcl::registration::DefaultConvergenceCriteria::ConvergenceState
+CONV_NOT_CONVERGED = 0
+CONV_ITERATIONS = 1
+CONV_TRANSFORM = 2
+CONV_ABS_MSE = 3
+CONV_REL_MSE = 4
+CONV_NO_CORRESPONDENCES = 5
+
+PREC_FP32 = 0
+
+# symbols declared in include/lcr.h
+EXPORTS = (
+    "lcr_icp_params_default", "lcr_init", "lcr_icp_create", "lcr_icp_destroy",
+    "lcr_icp_set_params", "lcr_icp_set_source", "lcr_icp_set_target", "lcr_icp_align",
+    "lcr_icp_final_transform", "lcr_icp_has_converged", "lcr_icp_fitness",
+    "lcr_icp_iterations", "lcr_icp_convergence_state", "lcr_icp_correspondences",
+    "lcr_normals", "lcr_cycle_create", "lcr_cycle_run", "lcr_cycle_results",
+    "lcr_cycle_instances", "lcr_cycle_destroy", "lcr_icp_batch_align", "lcr_prof_enable",
+    "lcr_prof_reset", "lcr_prof_read", "lcr_last_error", "lcr_version",
+)
+
+
+class LcrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"liblcr error {code}: {msg}")
+        self.code = code
+
+
+class IcpParams(C.Structure):
+    _fields_ = [
+        ("max_corr_dist", C.c_double),
+        ("max_iterations", C.c_int32),
+        ("transformation_epsilon", C.c_double),
+        ("euclidean_fitness_epsilon", C.c_double),
+        ("point_to_plane", C.c_int32),
+        ("precision_mode", C.c_int32),
+        ("cell_factor", C.c_double),
+        ("tile", C.c_int32),
+    ]
+
+
+class CycleDesc(C.Structure):
+    _fields_ = [
+        ("n_clouds", C.c_int32),
+        ("xyz", C.POINTER(C.POINTER(C.c_float))),
+        ("nrm", C.POINTER(C.POINTER(C.c_float))),
+        ("n_points", C.POINTER(C.c_int64)),
+        ("n_pairs", C.c_int32),
+        ("pair_src", C.POINTER(C.c_int32)),
+        ("pair_tgt", C.POINTER(C.c_int32)),
+        ("n_hyp", C.c_int32),
+        ("guesses", C.POINTER(C.c_double)),
+        ("params", IcpParams),
+        ("normals_k", C.c_int32),
+        ("viewpoints", C.POINTER(C.c_float)),
+    ]
+
+
+class CycleResult(C.Structure):
+    _fields_ = [
+        ("T", C.POINTER(C.c_double)),
+        ("iterations", C.POINTER(C.c_int32)),
+        ("converged", C.POINTER(C.c_int32)),
+        ("state", C.POINTER(C.c_int32)),
+        ("fitness", C.POINTER(C.c_double)),
+        ("best_hyp", C.POINTER(C.c_int32)),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C lowcost3dreconstruction_amd` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    fp, dp, ip = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    L.lcr_icp_params_default.argtypes = [C.POINTER(IcpParams)]
+    L.lcr_icp_params_default.restype = None
+    L.lcr_init.argtypes = [C.c_int]
+    L.lcr_icp_create.argtypes = [C.POINTER(IcpParams), C.POINTER(vp)]
+    L.lcr_icp_destroy.argtypes = [vp]
+    L.lcr_icp_destroy.restype = None
+    L.lcr_icp_set_params.argtypes = [vp, C.POINTER(IcpParams)]
+    L.lcr_icp_set_source.argtypes = [vp, fp, fp, C.c_int64, C.c_int64]
+    L.lcr_icp_set_target.argtypes = [vp, fp, fp, C.c_int64, C.c_int64]
+    L.lcr_icp_align.argtypes = [vp, dp, fp, fp]
+    L.lcr_icp_final_transform.argtypes = [vp, dp]
+    L.lcr_icp_has_converged.argtypes = [vp]
+    L.lcr_icp_fitness.argtypes = [vp, C.c_double, dp]
+    L.lcr_icp_iterations.argtypes = [vp]
+    L.lcr_icp_convergence_state.argtypes = [vp]
+    L.lcr_icp_correspondences.argtypes = [vp, dp, ip, fp]
+    L.lcr_normals.argtypes = [fp, C.c_int64, C.c_int64, C.c_int32, fp, C.c_int32, fp, fp]
+    L.lcr_cycle_create.argtypes = [C.POINTER(CycleDesc), C.POINTER(vp)]
+    L.lcr_cycle_run.argtypes = [vp]
+    L.lcr_cycle_results.argtypes = [vp, C.POINTER(CycleResult)]
+    L.lcr_cycle_instances.argtypes = [vp, dp, ip, ip, dp]
+    L.lcr_cycle_destroy.argtypes = [vp]
+    L.lcr_cycle_destroy.restype = None
+    L.lcr_icp_batch_align.argtypes = [C.POINTER(CycleDesc), C.POINTER(CycleResult)]
+    L.lcr_prof_enable.argtypes = [C.c_int32]
+    L.lcr_prof_reset.argtypes = []
+    L.lcr_prof_read.argtypes = [C.c_char_p, C.POINTER(C.c_int64), dp]
+    L.lcr_last_error.restype = C.c_char_p
+    L.lcr_version.restype = C.c_char_p
+    return L
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc < 0:
+        raise LcrError(rc, lib.lcr_last_error().decode(errors="replace"))
+    return rc
+
+
+def _f32(a, cols=3):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1, cols))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def version():
+    return lib.lcr_version().decode()
+
+
+def init(device=0):
+    _check(lib.lcr_init(device))
+
+
+def default_params():
+    p = IcpParams()
+    lib.lcr_icp_params_default(C.byref(p))
+    return p
+
+
+class IterativeClosestPoint:
+    """pcl::IterativeClosestPoint (point_to_plane=False) or
+    IterativeClosestPointWithNormals-style point-to-plane ICP (point_to_plane=True),
+    with PCL's defaults (max_iterations 10, no distance limit, epsilons off)."""
+
+    def __init__(self, point_to_plane=False, cell_factor=0.0, tile=0):
+        self._p = default_params()
+        self._p.point_to_plane = int(bool(point_to_plane))
+        self._p.cell_factor = float(cell_factor)
+        self._p.tile = int(tile)
+        h = C.c_void_p()
+        _check(lib.lcr_icp_create(C.byref(self._p), C.byref(h)))
+        self._h = h
+        self._src = None
+        self._src_nrm = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.lcr_icp_destroy(h)
+            self._h = None
+
+    def _push(self):
+        _check(lib.lcr_icp_set_params(self._h, C.byref(self._p)))
+
+    # ---- pcl::Registration setters (fine_registration.cpp:108-118)
+    def setInputSource(self, cloud, normals=None):
+        self._src = _f32(cloud)
+        self._src_nrm = _f32(normals) if normals is not None else None
+        _check(lib.lcr_icp_set_source(self._h, _fp(self._src), _fp(self._src_nrm),
+                                      len(self._src), 12))
+
+    def setInputTarget(self, cloud, normals=None):
+        t = _f32(cloud)
+        n = _f32(normals) if normals is not None else None
+        _check(lib.lcr_icp_set_target(self._h, _fp(t), _fp(n), len(t), 12))
+
+    def setMaxCorrespondenceDistance(self, d):
+        self._p.max_corr_dist = float(d)
+        self._push()
+
+    def setMaximumIterations(self, n):
+        self._p.max_iterations = int(n)
+        self._push()
+
+    def setTransformationEpsilon(self, e):
+        self._p.transformation_epsilon = float(e)
+        self._push()
+
+    def setEuclideanFitnessEpsilon(self, e):
+        self._p.euclidean_fitness_epsilon = float(e)
+        self._push()
+
+    # ---- align and getters (fine_registration.cpp:121-126)
+    def align(self, guess=None):
+        """Returns the registered source cloud (N, 3) (and its normals when the source
+        has normals, as a second value)."""
+        n = len(self._src) if self._src is not None else 0
+        out = np.empty((n, 3), np.float32)
+        outn = np.empty((n, 3), np.float32) if self._src_nrm is not None else None
+        g = np.ascontiguousarray(np.asarray(guess, np.float64).reshape(16)) \
+            if guess is not None else None
+        _check(lib.lcr_icp_align(self._h, _dp(g), _fp(out), _fp(outn)))
+        return (out, outn) if outn is not None else out
+
+    def getFinalTransformation(self):
+        T = np.empty(16, np.float64)
+        _check(lib.lcr_icp_final_transform(self._h, _dp(T)))
+        return T.reshape(4, 4)
+
+    def hasConverged(self):
+        return bool(_check(lib.lcr_icp_has_converged(self._h)))
+
+    def getFitnessScore(self, max_range=DBL_MAX):
+        out = C.c_double()
+        _check(lib.lcr_icp_fitness(self._h, float(max_range), C.byref(out)))
+        return out.value
+
+    def nr_iterations(self):
+        return _check(lib.lcr_icp_iterations(self._h))
+
+    def getConvergenceState(self):
+        return _check(lib.lcr_icp_convergence_state(self._h))
+
+    def correspondences(self, T=None):
+        """CorrespondenceEstimation::determineCorrespondences for the source under T
+        (default: the final transform): (target index or -1, d2) per source point."""
+        n = len(self._src)
+        idx = np.empty(n, np.int32)
+        d2 = np.empty(n, np.float32)
+        TT = np.ascontiguousarray(np.asarray(T, np.float64).reshape(16)) if T is not None else None
+        _check(lib.lcr_icp_correspondences(self._h, _dp(TT),
+                                           idx.ctypes.data_as(C.POINTER(C.c_int32)), _fp(d2)))
+        return idx, d2
+
+
+def normals(xyz, k, viewpoint=(0.0, 0.0, 0.0), flip_all=False):
+    """pcl::NormalEstimation with setKSearch(k) and a viewpoint, then the CLI's
+    negation of all normals when flip_all (normal_estimation.cpp:84-118).
+    Returns (normals (N, 3) float32, curvature (N,) float32)."""
+    x = _f32(xyz)
+    vp = np.asarray(viewpoint, np.float32)
+    nrm = np.empty_like(x)
+    curv = np.empty(len(x), np.float32)
+    _check(lib.lcr_normals(_fp(x), len(x), 12, int(k), _fp(vp), int(bool(flip_all)), _fp(nrm),
+                           _fp(curv)))
+    return nrm, curv
+
+
+class NormalEstimation:
+    """pcl::NormalEstimation<PointXYZRGB, Normal> as used at normal_estimation.cpp:84-108."""
+
+    def __init__(self):
+        self._cloud = None
+        self._k = 50
+        self._vp = (0.0, 0.0, 0.0)
+
+    def setInputCloud(self, cloud):
+        self._cloud = _f32(cloud)
+
+    def setKSearch(self, k):
+        self._k = int(k)
+
+    def setViewPoint(self, x, y, z):
+        self._vp = (float(x), float(y), float(z))
+
+    def useSensorOriginAsViewPoint(self):
+        self._vp = (0.0, 0.0, 0.0)  # PLY clouds carry the sensor origin (0, 0, 0)
+
+    def compute(self):
+        return normals(self._cloud, self._k, self._vp)
+
+
+class Cycle:
+    """All pairs (and pose hypotheses) of a capture cycle aligned in one batched
+    ICP: one launch per iteration for every instance, replayed from a hipGraph.
+    Replaces the per-pair loop of scripts/alignment.sh:106-113."""
+
+    def __init__(self, clouds, pairs, normals=None, guesses=None, n_hyp=1, max_corr_dist=0.05,
+                 max_iterations=60, transformation_epsilon=1e-9, euclidean_fitness_epsilon=1e-3,
+                 point_to_plane=True, normals_k=30, viewpoints=None, cell_factor=0.0, tile=0):
+        self.clouds = [_f32(c) for c in clouds]
+        self.normals = [_f32(n) for n in normals] if normals is not None else None
+        pr = np.asarray(pairs, np.int32).reshape(-1, 2)
+        self.n_pairs = len(pr)
+        self.n_hyp = int(n_hyp)
+        self._src = np.ascontiguousarray(pr[:, 0])
+        self._tgt = np.ascontiguousarray(pr[:, 1])
+        self._guess = np.ascontiguousarray(np.asarray(guesses, np.float64).reshape(-1)) \
+            if guesses is not None else None
+        if self._guess is not None and len(self._guess) != 16 * self.n_pairs * self.n_hyp:
+            raise ValueError("guesses must hold n_pairs * n_hyp 4x4 matrices")
+        fpp = C.POINTER(C.c_float)
+        self._xyz_arr = (fpp * len(self.clouds))(*[_fp(c) for c in self.clouds])
+        self._nrm_arr = (fpp * len(self.clouds))(*[_fp(n) for n in self.normals]) \
+            if self.normals is not None else None
+        self._n = np.asarray([len(c) for c in self.clouds], np.int64)
+        self._vps = _f32(viewpoints).reshape(-1) if viewpoints is not None else None
+        p = default_params()
+        p.max_corr_dist = float(max_corr_dist)
+        p.max_iterations = int(max_iterations)
+        p.transformation_epsilon = float(transformation_epsilon)
+        p.euclidean_fitness_epsilon = float(euclidean_fitness_epsilon)
+        p.point_to_plane = int(bool(point_to_plane))
+        p.cell_factor = float(cell_factor)
+        p.tile = int(tile)
+        d = CycleDesc()
+        d.n_clouds = len(self.clouds)
+        d.xyz = self._xyz_arr
+        d.nrm = self._nrm_arr
+        d.n_points = self._n.ctypes.data_as(C.POINTER(C.c_int64))
+        d.n_pairs = self.n_pairs
+        d.pair_src = self._src.ctypes.data_as(C.POINTER(C.c_int32))
+        d.pair_tgt = self._tgt.ctypes.data_as(C.POINTER(C.c_int32))
+        d.n_hyp = self.n_hyp
+        d.guesses = _dp(self._guess)
+        d.params = p
+        d.normals_k = int(normals_k)
+        d.viewpoints = _fp(self._vps) if self._vps is not None else None
+        self._desc = d
+        h = C.c_void_p()
+        _check(lib.lcr_cycle_create(C.byref(d), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.lcr_cycle_destroy(h)
+            self._h = None
+
+    def run(self):
+        _check(lib.lcr_cycle_run(self._h))
+
+    def results(self):
+        P = self.n_pairs
+        T = np.empty((P, 4, 4), np.float64)
+        it = np.empty(P, np.int32)
+        cv = np.empty(P, np.int32)
+        st = np.empty(P, np.int32)
+        fit = np.empty(P, np.float64)
+        best = np.empty(P, np.int32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        r = CycleResult(_dp(T), ip(it), ip(cv), ip(st), _dp(fit), ip(best))
+        _check(lib.lcr_cycle_results(self._h, C.byref(r)))
+        return dict(T=T, iterations=it, converged=cv.astype(bool), state=st, fitness=fit,
+                    best_hyp=best)
+
+    def instances(self):
+        n = self.n_pairs * self.n_hyp
+        T = np.empty((n, 4, 4), np.float64)
+        it = np.empty(n, np.int32)
+        st = np.empty(n, np.int32)
+        fit = np.empty(n, np.float64)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        _check(lib.lcr_cycle_instances(self._h, _dp(T), ip(it), ip(st), _dp(fit)))
+        return dict(T=T, iterations=it, state=st, fitness=fit)
+
+
+def prof_enable(on=True):
+    _check(lib.lcr_prof_enable(int(bool(on))))
+
+
+def prof_reset():
+    _check(lib.lcr_prof_reset())
+
+
+def prof_read(name):
+    n = C.c_int64()
+    ms = C.c_double()
+    _check(lib.lcr_prof_read(name.encode(), C.byref(n), C.byref(ms)))
+    return n.value, ms.value

@@ -1,0 +1,613 @@
+// api.cpp -- the C ABI of liblcr (include/lcr.h).  Every entry point catches
+// exceptions and turns them into negative return codes plus lcr_last_error().
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/lcr.h"
+#include "runtime.hpp"
+
+using namespace lcr;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_device = 0;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    return fail(e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(LCR_E_HIP, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(LCR_E_ARG, e.what());
+  }
+}
+
+bool params_ok(const lcr_icp_params* p, std::string* why) {
+  if (!p) {
+    *why = "params is NULL";
+    return false;
+  }
+  if (!(p->max_corr_dist > 0.0) || isnan(p->max_corr_dist)) {
+    *why = "max_corr_dist must be > 0";
+    return false;
+  }
+  if (p->max_iterations <= 0) {
+    *why = "max_iterations needs to be greater than zero";
+    return false;
+  }
+  if (p->max_iterations > 100000) {
+    *why = "max_iterations above 100000";
+    return false;
+  }
+  if (p->precision_mode != LCR_PREC_FP32) {
+    *why = "unsupported precision_mode";
+    return false;
+  }
+  if (p->tile != 0 && p->tile != 64 && p->tile != 128 && p->tile != 256) {
+    *why = "tile must be 0, 64, 128 or 256";
+    return false;
+  }
+  return true;
+}
+
+IcpConst make_const(const lcr_icp_params& p) {
+  IcpConst c;
+  c.max_iterations = p.max_iterations;
+  c.point_to_plane = p.point_to_plane ? 1 : 0;
+  c.rotation_threshold = 1.0 - p.transformation_epsilon;
+  c.translation_threshold = p.transformation_epsilon;
+  c.mse_rel = p.euclidean_fitness_epsilon;
+  c.mse_abs = 1e-12;
+  return c;
+}
+
+struct Stream {
+  hipStream_t s = nullptr;
+  int device = 0;
+  Stream() {
+    device = g_device;
+    LCR_HIP(hipSetDevice(device));
+    LCR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  ~Stream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void bind() const { LCR_HIP(hipSetDevice(device)); }
+};
+
+struct HostCloud {
+  std::vector<float> xyz, nrm;  // packed
+  int64_t n = 0;
+  bool has_nrm = false;
+  void set(const float* p, const float* q, int64_t count, int64_t stride) {
+    n = count;
+    xyz.resize((size_t)std::max<int64_t>(count, 0) * 3);
+    has_nrm = q != nullptr;
+    nrm.resize(has_nrm ? xyz.size() : 0);
+    const char* pb = reinterpret_cast<const char*>(p);
+    const char* qb = reinterpret_cast<const char*>(q);
+    for (int64_t i = 0; i < count; ++i) {
+      const float* a = reinterpret_cast<const float*>(pb + i * stride);
+      xyz[3 * i] = a[0];
+      xyz[3 * i + 1] = a[1];
+      xyz[3 * i + 2] = a[2];
+      if (has_nrm) {
+        const float* b = reinterpret_cast<const float*>(qb + i * stride);
+        nrm[3 * i] = b[0];
+        nrm[3 * i + 1] = b[1];
+        nrm[3 * i + 2] = b[2];
+      }
+    }
+  }
+};
+
+constexpr int64_t kMaxCloudPoints = (int64_t)1 << 24;
+
+}  // namespace
+
+struct lcr_icp {
+  lcr_icp_params params;
+  Stream stream;
+  HostCloud src, tgt;
+  bool have_src = false, have_tgt = false, dirty = true;
+  CloudSet cs;  // cloud 0 = source, cloud 1 = target
+  Engine eng;
+  bool aligned = false;
+  PairState result{};
+  // scratch for correspondences
+  DevBuf d_idx, d_d2;
+  HostBuf h_idx;
+};
+
+struct lcr_cycle {
+  lcr_cycle_desc desc;
+  Stream stream;
+  std::vector<std::vector<float>> xyz, nrm;
+  std::vector<int32_t> src, tgt;
+  std::vector<double> guesses;
+  std::vector<float> vps;
+  CloudSet cs;
+  Engine eng;
+  bool ran = false;
+  bool need_normals = false;
+  std::vector<PairState> states;
+};
+
+extern "C" {
+
+void lcr_icp_params_default(lcr_icp_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->max_corr_dist = sqrt(DBL_MAX);
+  p->max_iterations = 10;
+  p->transformation_epsilon = 0.0;
+  p->euclidean_fitness_epsilon = -DBL_MAX;
+  p->point_to_plane = 0;
+  p->precision_mode = LCR_PREC_FP32;
+  p->cell_factor = 0.0;
+  p->tile = 0;
+}
+
+int lcr_init(int device) {
+  return guarded([&] {
+    int n = 0;
+    LCR_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LCR_E_ARG, "no such HIP device");
+    LCR_HIP(hipSetDevice(device));
+    g_device = device;
+    return LCR_OK;
+  });
+}
+
+int lcr_icp_create(const lcr_icp_params* p, lcr_icp** out) {
+  return guarded([&] {
+    if (!out) return fail(LCR_E_ARG, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    if (!params_ok(p, &why)) return fail(LCR_E_ARG, why);
+    std::unique_ptr<lcr_icp> h(new lcr_icp());
+    h->params = *p;
+    *out = h.release();
+    return LCR_OK;
+  });
+}
+
+void lcr_icp_destroy(lcr_icp* h) {
+  if (!h) return;
+  try {
+    h->stream.bind();
+  } catch (...) {
+  }
+  delete h;
+}
+
+int lcr_icp_set_params(lcr_icp* h, const lcr_icp_params* p) {
+  return guarded([&] {
+    if (!h) return fail(LCR_E_ARG, "handle is NULL");
+    std::string why;
+    if (!params_ok(p, &why)) return fail(LCR_E_ARG, why);
+    const bool regrid = p->cell_factor != h->params.cell_factor || p->tile != h->params.tile;
+    h->params = *p;
+    if (regrid) h->dirty = true;
+    return LCR_OK;
+  });
+}
+
+static int set_cloud(lcr_icp* h, HostCloud& hc, bool& have, const float* xyz, const float* nrm,
+                     int64_t n, int64_t stride) {
+  if (!h) return fail(LCR_E_ARG, "handle is NULL");
+  if (n < 0 || (n > 0 && !xyz)) return fail(LCR_E_ARG, "invalid cloud");
+  if (n > kMaxCloudPoints) return fail(LCR_E_LIMIT, "cloud larger than 2^24 points");
+  if (stride == 0) stride = 3 * sizeof(float);
+  if (stride < (int64_t)(3 * sizeof(float)) || stride % 4 != 0)
+    return fail(LCR_E_ARG, "stride_bytes must be a multiple of 4 and >= 12");
+  hc.set(xyz, nrm, n, stride);
+  have = true;
+  h->dirty = true;
+  h->aligned = false;
+  return LCR_OK;
+}
+
+int lcr_icp_set_source(lcr_icp* h, const float* xyz, const float* nrm, int64_t n,
+                       int64_t stride_bytes) {
+  return guarded([&] { return set_cloud(h, h->src, h->have_src, xyz, nrm, n, stride_bytes); });
+}
+
+int lcr_icp_set_target(lcr_icp* h, const float* xyz, const float* nrm, int64_t n,
+                       int64_t stride_bytes) {
+  return guarded([&] { return set_cloud(h, h->tgt, h->have_tgt, xyz, nrm, n, stride_bytes); });
+}
+
+static void prepare(lcr_icp* h) {
+  h->stream.bind();
+  h->eng.c = make_const(h->params);
+  if (h->dirty) {
+    std::vector<const float*> xyz = {h->src.xyz.data(), h->tgt.xyz.data()};
+    std::vector<const float*> nrm = {h->src.has_nrm ? h->src.nrm.data() : nullptr,
+                                     h->tgt.has_nrm ? h->tgt.nrm.data() : nullptr};
+    // normals travel with the target (point-to-plane); the source's are only
+    // transformed for the output, from the host copy
+    const bool tn = h->tgt.has_nrm;
+    std::vector<const float*> nrm_up = {tn ? (h->src.has_nrm ? h->src.nrm.data() : nullptr)
+                                           : nullptr,
+                                        tn ? h->tgt.nrm.data() : nullptr};
+    (void)nrm;
+    std::vector<int64_t> cnt = {h->src.n, h->tgt.n};
+    // a cloud set has normals only when every cloud has them; give the source
+    // zeros if only the target has normals
+    std::vector<float> zeros;
+    if (tn && !h->src.has_nrm) {
+      zeros.assign(h->src.xyz.size(), 0.0f);
+      nrm_up[0] = zeros.data();
+    }
+    h->cs.upload(h->stream.s, xyz, nrm_up, cnt, 3 * sizeof(float));
+    h->cs.build(h->stream.s, h->params.cell_factor, nullptr);
+    const double r = h->params.max_corr_dist;
+    h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 256);
+    h->dirty = false;
+  } else {
+    // parameters may have changed: refresh the instance radius
+    const double r = h->params.max_corr_dist;
+    if (h->eng.hp.size() == 1 && h->eng.hp[0].max_corr_d2 != r * r) {
+      h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 256);
+    }
+  }
+}
+
+int lcr_icp_align(lcr_icp* h, const double* guess, float* out_xyz, float* out_nrm) {
+  return guarded([&] {
+    if (!h) return fail(LCR_E_ARG, "handle is NULL");
+    if (!h->have_src || !h->have_tgt) return fail(LCR_E_STATE, "source and target must be set");
+    if (h->params.point_to_plane && !h->tgt.has_nrm)
+      return fail(LCR_E_NORMALS, "point-to-plane ICP needs target normals");
+    prepare(h);
+    h->eng.run(h->stream.s, h->cs, guess, true);
+    h->result = h->eng.states()[0];
+    h->aligned = true;
+    if (out_xyz || out_nrm) {
+      // K8: transform the raw source by the final transform
+      DevBuf out;
+      const int64_t n = h->src.n;
+      out.reserve((size_t)std::max<int64_t>(n, 1) * 6 * sizeof(float));
+      float* dx = out.as<float>();
+      float* dn = dx + 3 * std::max<int64_t>(n, 1);
+      DevBuf rawn;
+      const float4* raw_nrm = nullptr;
+      if (out_nrm && h->src.has_nrm) {
+        rawn.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(float4));
+        std::vector<float4> tmp((size_t)n);
+        for (int64_t i = 0; i < n; ++i)
+          tmp[i] = make_float4(h->src.nrm[3 * i], h->src.nrm[3 * i + 1], h->src.nrm[3 * i + 2], 0.f);
+        LCR_HIP(hipMemcpyAsync(rawn.p, tmp.data(), n * sizeof(float4), hipMemcpyHostToDevice,
+                               h->stream.s));
+        LCR_HIP(hipStreamSynchronize(h->stream.s));
+        raw_nrm = rawn.as<float4>();
+      }
+      launch_transform_output(h->cs.d_raw.as<float4>(), raw_nrm, n, h->eng.d_state.as<PairState>(),
+                              0, dx, raw_nrm ? dn : nullptr, h->stream.s);
+      if (out_xyz && n > 0)
+        LCR_HIP(hipMemcpyAsync(out_xyz, dx, n * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                               h->stream.s));
+      if (out_nrm && n > 0) {
+        if (raw_nrm) {
+          LCR_HIP(hipMemcpyAsync(out_nrm, dn, n * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                 h->stream.s));
+        } else {
+          for (int64_t i = 0; i < 3 * n; ++i) out_nrm[i] = NAN;
+        }
+      }
+      LCR_HIP(hipStreamSynchronize(h->stream.s));
+    }
+    return LCR_OK;
+  });
+}
+
+int lcr_icp_final_transform(const lcr_icp* h, double T[16]) {
+  if (!h || !T) return fail(LCR_E_ARG, "NULL argument");
+  if (!h->aligned) {
+    for (int i = 0; i < 16; ++i) T[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    return LCR_OK;
+  }
+  memcpy(T, h->result.T, 16 * sizeof(double));
+  return LCR_OK;
+}
+
+int lcr_icp_has_converged(const lcr_icp* h) {
+  if (!h) return fail(LCR_E_ARG, "handle is NULL");
+  return h->aligned && h->result.converged ? 1 : 0;
+}
+
+int lcr_icp_iterations(const lcr_icp* h) {
+  if (!h) return fail(LCR_E_ARG, "handle is NULL");
+  return h->aligned ? h->result.iterations : 0;
+}
+
+int lcr_icp_convergence_state(const lcr_icp* h) {
+  if (!h) return fail(LCR_E_ARG, "handle is NULL");
+  return h->aligned ? h->result.state : LCR_CONV_NOT_CONVERGED;
+}
+
+int lcr_icp_fitness(lcr_icp* h, double max_range, double* out) {
+  return guarded([&] {
+    if (!h || !out) return fail(LCR_E_ARG, "NULL argument");
+    if (!h->aligned) return fail(LCR_E_STATE, "align has not run");
+    if (isnan(max_range)) return fail(LCR_E_ARG, "max_range is NaN");
+    h->stream.bind();
+    if (max_range == DBL_MAX || isinf(max_range)) {
+      *out = h->result.fitness;  // computed by align with max_range = DBL_MAX
+      return LCR_OK;
+    }
+    h->eng.run_fitness(h->stream.s, h->cs, max_range);
+    *out = h->eng.states()[0].fitness;
+    return LCR_OK;
+  });
+}
+
+int lcr_icp_correspondences(lcr_icp* h, const double* T, int32_t* idx, float* d2) {
+  return guarded([&] {
+    if (!h || !idx || !d2) return fail(LCR_E_ARG, "NULL argument");
+    if (!h->have_src || !h->have_tgt) return fail(LCR_E_STATE, "source and target must be set");
+    prepare(h);
+    const int64_t n = h->src.n;
+    for (int64_t i = 0; i < n; ++i) {
+      idx[i] = -1;
+      d2[i] = INFINITY;
+    }
+    // probe slot: state[1] carries T (or the current final transform)
+    PairState probe{};
+    if (T)
+      memcpy(probe.T, T, sizeof probe.T);
+    else if (h->aligned)
+      memcpy(probe.T, h->result.T, sizeof probe.T);
+    else
+      for (int i = 0; i < 16; ++i) probe.T[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    h->h_idx.reserve(sizeof(PairState));
+    memcpy(h->h_idx.p, &probe, sizeof probe);
+    PairState* slot = h->eng.d_state.as<PairState>() + 1;
+    LCR_HIP(hipMemcpyAsync(slot, h->h_idx.p, sizeof probe, hipMemcpyHostToDevice, h->stream.s));
+    IcpBuffers b = h->eng.buffers(h->cs);
+    b.state = slot;
+    h->d_idx.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t));
+    h->d_d2.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(float));
+    if (n > 0) {
+      LCR_HIP(hipMemcpyAsync(h->d_idx.p, idx, n * sizeof(int32_t), hipMemcpyHostToDevice,
+                             h->stream.s));
+      LCR_HIP(hipMemcpyAsync(h->d_d2.p, d2, n * sizeof(float), hipMemcpyHostToDevice, h->stream.s));
+      launch_correspondences(b, 0, h->cs.grids[0].npts, h->d_idx.as<int32_t>(),
+                             h->d_d2.as<float>(), h->stream.s);
+      LCR_HIP(hipMemcpyAsync(idx, h->d_idx.p, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             h->stream.s));
+      LCR_HIP(hipMemcpyAsync(d2, h->d_d2.p, n * sizeof(float), hipMemcpyDeviceToHost, h->stream.s));
+    }
+    LCR_HIP(hipStreamSynchronize(h->stream.s));
+    return LCR_OK;
+  });
+}
+
+int lcr_normals(const float* xyz, int64_t n, int64_t stride_bytes, int32_t k, const float vp[3],
+                int32_t flip_all, float* nrm, float* curv) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!xyz || !nrm))) return fail(LCR_E_ARG, "invalid arguments");
+    if (n > kMaxCloudPoints) return fail(LCR_E_LIMIT, "cloud larger than 2^24 points");
+    if (k < 1) return fail(LCR_E_ARG, "k must be >= 1");
+    if (stride_bytes == 0) stride_bytes = 3 * sizeof(float);
+    if (stride_bytes < 12 || stride_bytes % 4 != 0) return fail(LCR_E_ARG, "bad stride_bytes");
+    if (n == 0) return LCR_OK;
+    Stream st;
+    CloudSet cs;
+    cs.upload(st.s, {xyz}, {nullptr}, {n}, stride_bytes);
+    // cells sized for a k-neighbourhood: ~sqrt(k) point spacings
+    const double factor = std::max(2.0, 0.75 * sqrt((double)k));
+    cs.build(st.s, factor, nullptr);
+    const float v[3] = {vp ? vp[0] : 0.f, vp ? vp[1] : 0.f, vp ? vp[2] : 0.f};
+    cs.compute_normals(st.s, k, v, flip_all, nullptr);
+    std::vector<float4> sorted((size_t)std::max<int64_t>(cs.total_pts, 1));
+    if (cs.total_pts > 0) {
+      LCR_HIP(hipMemcpyAsync(sorted.data(), cs.d_nrm.p, cs.total_pts * sizeof(float4),
+                             hipMemcpyDeviceToHost, st.s));
+    }
+    std::vector<float4> pts((size_t)std::max<int64_t>(cs.total_pts, 1));
+    if (cs.total_pts > 0) {
+      LCR_HIP(hipMemcpyAsync(pts.data(), cs.d_pts.p, cs.total_pts * sizeof(float4),
+                             hipMemcpyDeviceToHost, st.s));
+    }
+    LCR_HIP(hipStreamSynchronize(st.s));
+    for (int64_t i = 0; i < 3 * n; ++i) nrm[i] = NAN;
+    if (curv)
+      for (int64_t i = 0; i < n; ++i) curv[i] = NAN;
+    for (int64_t j = 0; j < cs.total_pts; ++j) {
+      int o;
+      memcpy(&o, &pts[j].w, sizeof o);
+      nrm[3 * (int64_t)o] = sorted[j].x;
+      nrm[3 * (int64_t)o + 1] = sorted[j].y;
+      nrm[3 * (int64_t)o + 2] = sorted[j].z;
+      if (curv) curv[o] = sorted[j].w;
+    }
+    return LCR_OK;
+  });
+}
+
+// ------------------------------------------------------------------ cycles
+
+int lcr_cycle_create(const lcr_cycle_desc* d, lcr_cycle** out) {
+  return guarded([&] {
+    if (!d || !out) return fail(LCR_E_ARG, "NULL argument");
+    *out = nullptr;
+    std::string why;
+    if (!params_ok(&d->params, &why)) return fail(LCR_E_ARG, why);
+    if (d->n_clouds <= 0 || !d->xyz || !d->n_points) return fail(LCR_E_ARG, "no clouds");
+    if (d->n_pairs < 0 || (d->n_pairs > 0 && (!d->pair_src || !d->pair_tgt)))
+      return fail(LCR_E_ARG, "invalid pairs");
+    if (d->n_hyp < 1) return fail(LCR_E_ARG, "n_hyp must be >= 1");
+    std::unique_ptr<lcr_cycle> c(new lcr_cycle());
+    c->desc = *d;
+    c->stream.bind();
+    bool all_nrm = d->nrm != nullptr;
+    for (int i = 0; i < d->n_clouds; ++i) {
+      if (d->n_points[i] < 0 || (d->n_points[i] > 0 && !d->xyz[i]))
+        return fail(LCR_E_ARG, "invalid cloud");
+      if (d->n_points[i] > kMaxCloudPoints) return fail(LCR_E_LIMIT, "cloud larger than 2^24 points");
+      if (all_nrm && !d->nrm[i]) all_nrm = false;
+    }
+    for (int p = 0; p < d->n_pairs; ++p) {
+      if (d->pair_src[p] < 0 || d->pair_src[p] >= d->n_clouds || d->pair_tgt[p] < 0 ||
+          d->pair_tgt[p] >= d->n_clouds)
+        return fail(LCR_E_ARG, "pair refers to a missing cloud");
+    }
+    std::vector<const float*> xyz(d->xyz, d->xyz + d->n_clouds);
+    std::vector<const float*> nrm(d->n_clouds, nullptr);
+    if (all_nrm)
+      for (int i = 0; i < d->n_clouds; ++i) nrm[i] = d->nrm[i];
+    std::vector<int64_t> cnt(d->n_points, d->n_points + d->n_clouds);
+    c->need_normals = d->params.point_to_plane && !all_nrm;
+    c->cs.upload(c->stream.s, xyz, nrm, cnt, 3 * sizeof(float));
+    c->src.assign(d->pair_src, d->pair_src + d->n_pairs);
+    c->tgt.assign(d->pair_tgt, d->pair_tgt + d->n_pairs);
+    const int ninst = d->n_pairs * d->n_hyp;
+    c->guesses.resize((size_t)ninst * 16);
+    for (int i = 0; i < ninst; ++i)
+      for (int k = 0; k < 16; ++k)
+        c->guesses[16 * (size_t)i + k] = d->guesses ? d->guesses[16 * (size_t)i + k]
+                                                    : ((k % 5 == 0) ? 1.0 : 0.0);
+    c->vps.assign((size_t)d->n_clouds * 3, 0.0f);
+    if (d->viewpoints)
+      for (int i = 0; i < 3 * d->n_clouds; ++i) c->vps[i] = d->viewpoints[i];
+    c->desc.xyz = nullptr;
+    c->desc.nrm = nullptr;
+    c->desc.n_points = nullptr;
+    c->desc.pair_src = nullptr;
+    c->desc.pair_tgt = nullptr;
+    c->desc.guesses = nullptr;
+    c->desc.viewpoints = nullptr;
+    *out = c.release();
+    return LCR_OK;
+  });
+}
+
+int lcr_cycle_run(lcr_cycle* c) {
+  return guarded([&] {
+    if (!c) return fail(LCR_E_ARG, "handle is NULL");
+    c->stream.bind();
+    const lcr_cycle_desc& d = c->desc;
+    Profiler& prof = Profiler::get();
+    std::vector<Profiler::Span> spans;
+    const int k = d.normals_k > 0 ? d.normals_k : 30;
+    c->cs.has_nrm = !c->need_normals && c->cs.has_nrm;
+    c->cs.build(c->stream.s, d.params.cell_factor, &spans);
+    if (c->need_normals) c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
+    if (d.params.point_to_plane && !c->cs.has_nrm)
+      return fail(LCR_E_NORMALS, "point-to-plane ICP needs normals");
+    const int ninst = d.n_pairs * d.n_hyp;
+    std::vector<int> src(ninst), tgt(ninst);
+    std::vector<double> r2(ninst, d.params.max_corr_dist * d.params.max_corr_dist);
+    for (int p = 0; p < d.n_pairs; ++p)
+      for (int hy = 0; hy < d.n_hyp; ++hy) {
+        src[p * d.n_hyp + hy] = c->src[p];
+        tgt[p * d.n_hyp + hy] = c->tgt[p];
+      }
+    c->eng.c = make_const(d.params);
+    const int tile = d.params.tile ? d.params.tile : 256;
+    bool same = c->eng.npairs == ninst && c->eng.tile == tile && c->eng.hp.size() == (size_t)ninst;
+    if (same) {
+      for (int i = 0; i < ninst && same; ++i)
+        same = c->eng.hp[i].ntiles == (c->cs.grids[src[i]].npts + tile - 1) / tile;
+    }
+    if (!same) c->eng.setup(c->cs, src, tgt, r2, tile);
+    if (!spans.empty()) prof.collect(spans, false);
+    c->eng.run(c->stream.s, c->cs, c->guesses.data(), true);
+    c->states.assign(c->eng.states(), c->eng.states() + ninst);
+    c->ran = true;
+    return LCR_OK;
+  });
+}
+
+int lcr_cycle_instances(lcr_cycle* c, double* T, int32_t* iterations, int32_t* state,
+                        double* fitness) {
+  if (!c) return fail(LCR_E_ARG, "handle is NULL");
+  if (!c->ran) return fail(LCR_E_STATE, "lcr_cycle_run has not run");
+  for (size_t i = 0; i < c->states.size(); ++i) {
+    const PairState& s = c->states[i];
+    if (T) memcpy(T + 16 * i, s.T, 16 * sizeof(double));
+    if (iterations) iterations[i] = s.iterations;
+    if (state) state[i] = s.state;
+    if (fitness) fitness[i] = s.fitness;
+  }
+  return LCR_OK;
+}
+
+int lcr_cycle_results(lcr_cycle* c, lcr_cycle_result* r) {
+  if (!c || !r) return fail(LCR_E_ARG, "NULL argument");
+  if (!c->ran) return fail(LCR_E_STATE, "lcr_cycle_run has not run");
+  const int H = c->desc.n_hyp;
+  for (int p = 0; p < c->desc.n_pairs; ++p) {
+    int best = 0;
+    for (int h = 1; h < H; ++h)
+      if (c->states[p * H + h].fitness < c->states[p * H + best].fitness) best = h;
+    const PairState& s = c->states[p * H + best];
+    if (r->T) memcpy(r->T + 16 * (size_t)p, s.T, 16 * sizeof(double));
+    if (r->iterations) r->iterations[p] = s.iterations;
+    if (r->converged) r->converged[p] = s.converged;
+    if (r->state) r->state[p] = s.state;
+    if (r->fitness) r->fitness[p] = s.fitness;
+    if (r->best_hyp) r->best_hyp[p] = best;
+  }
+  return LCR_OK;
+}
+
+void lcr_cycle_destroy(lcr_cycle* c) {
+  if (!c) return;
+  try {
+    c->stream.bind();
+  } catch (...) {
+  }
+  delete c;
+}
+
+int lcr_icp_batch_align(const lcr_cycle_desc* d, lcr_cycle_result* r) {
+  lcr_cycle* c = nullptr;
+  int rc = lcr_cycle_create(d, &c);
+  if (rc != LCR_OK) return rc;
+  rc = lcr_cycle_run(c);
+  if (rc == LCR_OK) rc = lcr_cycle_results(c, r);
+  lcr_cycle_destroy(c);
+  return rc;
+}
+
+// ------------------------------------------------------------------ profiling
+
+int lcr_prof_enable(int32_t on) {
+  Profiler::get().enable(on != 0);
+  return LCR_OK;
+}
+
+int lcr_prof_reset(void) {
+  Profiler::get().reset();
+  return LCR_OK;
+}
+
+int lcr_prof_read(const char* name, int64_t* launches, double* total_ms) {
+  if (!name || !launches || !total_ms) return fail(LCR_E_ARG, "NULL argument");
+  Profiler::get().read(name, launches, total_ms);
+  return LCR_OK;
+}
+
+const char* lcr_last_error(void) { return g_err.c_str(); }
+
+const char* lcr_version(void) { return "lcr 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+// grid.hip -- K1: dense voxel grid build for a batch of clouds.
+//
+// Replaces KdTreeFLANN::setInputCloud (run by Registration::initCompute when the
+// target changes; fine_registration.cpp:109,121) and the search-tree build of
+// NormalEstimation (normal_estimation.cpp:89-90).  Per cloud: bounding box ->
+// per-cell counts (with occupancy) -> exclusive scan into global point offsets ->
+// scatter into a cell-sorted float4 array -> sort each cell by original index, so
+// the layout does not depend on atomic ordering.  All kernels are HBM-bound
+// integer work: one pass over the points per step, coalesced.
+#include <hip/hip_runtime.h>
+
+#include "lcr_common.hpp"
+#include "lcr_kernels.hpp"
+
+namespace lcr {
+
+namespace {
+
+constexpr int kBBoxThreads = 256;
+
+__device__ __forceinline__ bool finite3(const float4& p) {
+  return isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
+}
+
+// One block per (cloud, chunk): partial min/max of finite points.
+__global__ void __launch_bounds__(kBBoxThreads)
+k_bbox_partial(const float4* __restrict__ raw, const int64_t* __restrict__ cloud_off, int nclouds,
+               int chunks_per_cloud, float* __restrict__ part) {
+  const int cloud = blockIdx.x / chunks_per_cloud;
+  const int chunk = blockIdx.x % chunks_per_cloud;
+  const int64_t b = cloud_off[cloud], e = cloud_off[cloud + 1];
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float cnt = 0.0f;
+  for (int64_t i = b + (int64_t)chunk * blockDim.x + threadIdx.x; i < e;
+       i += (int64_t)chunks_per_cloud * blockDim.x) {
+    const float4 p = raw[i];
+    if (!finite3(p)) continue;
+    mn[0] = fminf(mn[0], p.x);
+    mn[1] = fminf(mn[1], p.y);
+    mn[2] = fminf(mn[2], p.z);
+    mx[0] = fmaxf(mx[0], p.x);
+    mx[1] = fmaxf(mx[1], p.y);
+    mx[2] = fmaxf(mx[2], p.z);
+    cnt += 1.0f;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, 64));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
+    }
+    cnt += __shfl_xor(cnt, off, 64);
+  }
+  __shared__ float s[kBBoxThreads / 64][7];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s[w][0] = mn[0];
+    s[w][1] = mn[1];
+    s[w][2] = mn[2];
+    s[w][3] = mx[0];
+    s[w][4] = mx[1];
+    s[w][5] = mx[2];
+    s[w][6] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      for (int a = 0; a < 3; ++a) {
+        s[0][a] = fminf(s[0][a], s[k][a]);
+        s[0][3 + a] = fmaxf(s[0][3 + a], s[k][3 + a]);
+      }
+      s[0][6] += s[k][6];
+    }
+    float* o = part + (int64_t)blockIdx.x * 7;
+    for (int a = 0; a < 7; ++a) o[a] = s[0][a];
+  }
+}
+
+// One thread per cloud: fold the chunk partials.  Output per cloud: min xyz,
+// max xyz, finite count.
+__global__ void k_bbox_final(const float* __restrict__ part, int nclouds, int chunks_per_cloud,
+                             float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nclouds) return;
+  float r[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
+  for (int k = 0; k < chunks_per_cloud; ++k) {
+    const float* p = part + ((int64_t)c * chunks_per_cloud + k) * 7;
+    for (int a = 0; a < 3; ++a) {
+      r[a] = fminf(r[a], p[a]);
+      r[3 + a] = fmaxf(r[3 + a], p[3 + a]);
+    }
+    r[6] += p[6];
+  }
+  for (int a = 0; a < 7; ++a) out[c * 7 + a] = r[a];
+}
+
+__device__ __forceinline__ int cell_of(const GridDesc& g, const float4& p) {
+  int x = (int)floorf((p.x - g.ox) * g.inv_h);
+  int y = (int)floorf((p.y - g.oy) * g.inv_h);
+  int z = (int)floorf((p.z - g.oz) * g.inv_h);
+  x = min(max(x, 0), g.nx - 1);
+  y = min(max(y, 0), g.ny - 1);
+  z = min(max(z, 0), g.nz - 1);
+  return (z * g.ny + y) * g.nx + x;
+}
+
+// Per point: global cell index (or -1 for non-finite points) and per-cell counts;
+// occupied[cloud] counts cells that became non-empty.
+__global__ void k_cell_count(const float4* __restrict__ raw, const int* __restrict__ point_cloud,
+                             const GridDesc* __restrict__ grids, int64_t n,
+                             int* __restrict__ cid, int* __restrict__ counts,
+                             int* __restrict__ occupied) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = raw[i];
+  if (!finite3(p)) {
+    cid[i] = -1;
+    return;
+  }
+  const int c = point_cloud[i];
+  const GridDesc g = grids[c];
+  const int gc = (int)g.cell_off + cell_of(g, p);
+  cid[i] = gc;
+  const int old = atomicAdd(&counts[gc], 1);
+  if (old == 0 && occupied) atomicAdd(&occupied[c], 1);
+}
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanPerThread = 4;
+constexpr int kScanTile = kScanThreads * kScanPerThread;
+
+__device__ __forceinline__ int block_excl_scan(int v, int* lds, int& total) {
+  // inclusive wave scan with shuffles, then a scan over wave totals
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) lds[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    const int nw = blockDim.x >> 6;
+    int t = lane < nw ? lds[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(t, off, 64);
+      if (lane >= off) t += y;
+    }
+    if (lane < nw) lds[lane] = t;
+  }
+  __syncthreads();
+  const int before = w > 0 ? lds[w - 1] : 0;
+  total = lds[(blockDim.x >> 6) - 1];
+  __syncthreads();
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int64_t n, int* __restrict__ sums) {
+  __shared__ int lds[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+  int v[kScanPerThread];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int run = block_excl_scan(s, lds, total);
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the tile sums, in chunks with a carry.
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int* __restrict__ sums, int n) {
+  __shared__ int lds[kScanThreads / 64];
+  int carry = 0;
+  for (int b = 0; b < n; b += kScanThreads) {
+    const int i = b + threadIdx.x;
+    const int v = i < n ? sums[i] : 0;
+    int total;
+    const int ex = block_excl_scan(v, lds, total);
+    if (i < n) sums[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ sums) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] += sums[i / kScanTile];
+}
+
+__global__ void k_scatter(const float4* __restrict__ raw, const float4* __restrict__ raw_nrm,
+                          const int* __restrict__ cid, int64_t n, const int* __restrict__ cell_start,
+                          int* __restrict__ fill, float4* __restrict__ pts,
+                          float4* __restrict__ nrm, const int64_t* __restrict__ cloud_off,
+                          const int* __restrict__ point_cloud) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cid[i];
+  if (c < 0) return;
+  const int pos = cell_start[c] + atomicAdd(&fill[c], 1);
+  float4 p = raw[i];
+  const int local = (int)(i - cloud_off[point_cloud[i]]);
+  p.w = __int_as_float(local);
+  pts[pos] = p;
+  if (raw_nrm) nrm[pos] = raw_nrm[i];
+}
+
+// One thread per cell: insertion sort of the cell's points by original index.
+__global__ void k_cell_sort(const int* __restrict__ cell_start, int64_t ncells_total,
+                            float4* __restrict__ pts, float4* __restrict__ nrm) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells_total) return;
+  const int b = cell_start[c], e = cell_start[c + 1];
+  if (e - b < 2) return;
+  for (int i = b + 1; i < e; ++i) {
+    const float4 p = pts[i];
+    const float4 q = nrm ? nrm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int key = __float_as_int(p.w);
+    int j = i - 1;
+    while (j >= b && __float_as_int(pts[j].w) > key) {
+      pts[j + 1] = pts[j];
+      if (nrm) nrm[j + 1] = nrm[j];
+      --j;
+    }
+    pts[j + 1] = p;
+    if (nrm) nrm[j + 1] = q;
+  }
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void launch_bbox(const float4* raw, const int64_t* cloud_off, int nclouds, float* part,
+                 float* out, hipStream_t s) {
+  constexpr int chunks = 64;
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nclouds * chunks), dim3(kBBoxThreads), 0, s, raw,
+                     cloud_off, nclouds, chunks, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(blocks_for(nclouds, 64)), dim3(64), 0, s, part, nclouds,
+                     chunks, out);
+}
+
+void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_cell_count(const float4* raw, const int* point_cloud, const GridDesc* grids,
+                       int64_t n, int* cid, int* counts, int* occupied, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, raw, point_cloud,
+                     grids, n, cid, counts, occupied);
+}
+
+int64_t scan_tmp_ints(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, out, n,
+                     tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, tmp, (int)tiles);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, s, out, n, tmp);
+}
+
+void launch_scatter(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n,
+                    const int* cell_start, int* fill, float4* pts, float4* nrm,
+                    const int64_t* cloud_off, const int* point_cloud, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, raw, raw_nrm, cid, n,
+                     cell_start, fill, pts, nrm, cloud_off, point_cloud);
+}
+
+void launch_cell_sort(const int* cell_start, int64_t ncells_total, float4* pts, float4* nrm,
+                      hipStream_t s) {
+  if (ncells_total <= 0) return;
+  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(ncells_total, 256)), dim3(256), 0, s,
+                     cell_start, ncells_total, pts, nrm);
+}
+
+}  // namespace lcr

@@ -1,0 +1,400 @@
+// icp.hip -- K2..K6 and K8: one ICP iteration for every active instance of a
+// batch in a single launch, the fitness pass, and the output transform.
+//
+// Replaces IterativeClosestPoint::computeTransformation's loop body
+// (pcl/registration/impl/icp.hpp; driven by icp.align at fine_registration.cpp:121):
+//   determineCorrespondences   -> exact grid 1-NN per source point (lcr_search.hpp)
+//   estimateRigidTransformation-> FP64 moments per tile, folded by the instance's
+//                                 last block, Horn (point-to-point) or Cholesky
+//                                 (point-to-plane) in FP64 (lcr_math.hpp)
+//   final = T * final          -> FP64 composition on the device
+//   convergence criteria       -> DefaultConvergenceCriteria on the device
+// and Registration::getFitnessScore (fine_registration.cpp:126).
+//
+// Work decomposition: a tile is `TILE` consecutive cell-sorted source points of
+// one instance (one point per thread).  The launch is persistent: a fixed number
+// of blocks (a multiple of 8) loops over tiles.  Hardware places block b on XCD
+// b % 8, so each XCD is given one contiguous eighth of the tile range: the tiles
+// of an instance, and the target grid they search, then stay in that XCD's L2.
+// Each block adds its tile's FP64 sums into a per-tile partial, then takes a
+// ticket on the instance; the block that takes the last ticket folds the
+// partials in tile order (deterministic, no floating-point atomics) and updates
+// the instance.  Converged instances are skipped, so the iteration can be
+// replayed a fixed number of times inside a hipGraph.
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+#include "lcr_common.hpp"
+#include "lcr_kernels.hpp"
+#include "lcr_math.hpp"
+#include "lcr_search.hpp"
+
+namespace lcr {
+
+#pragma clang fp contract(off)
+
+namespace {
+
+// x' = (float)(((T0*x + T1*y) + T2*z) + T3) in FP64, no FMA: the source is always
+// transformed from its original coordinates by the FP64-composed transform.
+__device__ __forceinline__ void xform(const double* T, const float4& s, float& qx, float& qy,
+                                      float& qz) {
+  const double x = s.x, y = s.y, z = s.z;
+  double v;
+  v = T[0] * x;
+  v = v + T[1] * y;
+  v = v + T[2] * z;
+  v = v + T[3];
+  qx = (float)v;
+  v = T[4] * x;
+  v = v + T[5] * y;
+  v = v + T[6] * z;
+  v = v + T[7];
+  qy = (float)v;
+  v = T[8] * x;
+  v = v + T[9] * y;
+  v = v + T[10] * z;
+  v = v + T[11];
+  qz = (float)v;
+}
+
+struct TileRange {
+  int first, end, step;
+};
+
+// XCD-aware persistent mapping (gridDim.x is a multiple of 8).
+__device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+  const int t0 = (int)(((int64_t)ntiles * xcd) >> 3);
+  const int t1 = (int)(((int64_t)ntiles * (xcd + 1)) >> 3);
+  return TileRange{t0 + slot, t1, per};
+}
+
+template <bool P2PL>
+__device__ __forceinline__ void accumulate(double* v, float qx, float qy, float qz,
+                                           const float4& t, const float4& n, float d2) {
+  v[0] = 1.0;
+  if constexpr (!P2PL) {
+    v[1] = qx;
+    v[2] = qy;
+    v[3] = qz;
+    v[4] = t.x;
+    v[5] = t.y;
+    v[6] = t.z;
+    const double q[3] = {qx, qy, qz}, tt[3] = {t.x, t.y, t.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) v[7 + 3 * a + b] = q[a] * tt[b];
+    v[16] = d2;
+  } else {
+    v[28] = d2;
+    if (isfinite(n.x) && isfinite(n.y) && isfinite(n.z)) {
+      // transformation_estimation_point_to_plane_lls.hpp: row [s x n ; n],
+      // d = n . (dst - src), accumulated in double
+      const double sx = qx, sy = qy, sz = qz, nx = n.x, ny = n.y, nz = n.z;
+      const double row[6] = {nz * sy - ny * sz, nx * sz - nz * sx, ny * sx - nx * sy, nx, ny, nz};
+      const double d = nx * (double)t.x + ny * (double)t.y + nz * (double)t.z - nx * sx -
+                       ny * sy - nz * sz;
+      int k = 1;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = i; j < 6; ++j) v[k++] = row[i] * row[j];
+        v[22 + i] = row[i] * d;
+      }
+    }
+  }
+}
+
+template <bool P2PL, int TILE>
+__global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const IcpConst c) {
+  constexpr int NV = P2PL ? kP2LVals : kP2PVals;
+  __shared__ double lds[NV * (TILE / 64)];
+  __shared__ int s_last;
+  const TileRange tr = xcd_tiles(b.ntiles);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int p = b.tile_pair[tile];
+    PairState* st = b.state + p;
+    if (st->active == 0) continue;  // uniform across the block
+    const PairDesc pd = b.pairs[p];
+    const GridDesc gs = b.grids[pd.src];
+    const GridDesc gt = b.grids[pd.tgt];
+    const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = 0.0;
+    if (local < gs.npts) {
+      const float4 s = b.pts[gs.pt_off + local];
+      float qx, qy, qz;
+      xform(st->T, s, qx, qy, qz);
+      const float r = (float)sqrt(pd.max_corr_d2);
+      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r,
+                                   isfinite(r));
+      if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
+        const float4 t = b.pts[nn.j];
+        const float4 n = P2PL ? b.nrm[nn.j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        accumulate<P2PL>(v, qx, qy, qz, t, n, nn.d2);
+      }
+    }
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0) {
+      double* out = b.partials + (int64_t)tile * kMaxVals;
+#pragma unroll
+      for (int i = 0; i < NV; ++i) out[i] = v[i];
+      __threadfence();
+      const unsigned t = atomicAdd(&st->ticket, 1u);
+      s_last = (t == (unsigned)(pd.ntiles - 1));
+    }
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+#pragma unroll
+      for (int i = 0; i < NV; ++i) v[i] = 0.0;
+      for (int t = threadIdx.x; t < pd.ntiles; t += TILE) {
+        const double* in = b.partials + (int64_t)(pd.tile_begin + t) * kMaxVals;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] += in[i];
+      }
+      block_sum<NV>(v, lds);
+      if (threadIdx.x == 0) {
+        const double ncorr = v[0];
+        st->ncorr = (int)ncorr;
+        if (ncorr < 3.0) {
+          // icp.hpp: fewer than min_number_correspondences_ (3) -> not converged
+          st->state = kNoCorrespondences;
+          st->converged = 0;
+          st->active = 0;
+        } else {
+          double Tinc[16];
+          if constexpr (P2PL) {
+            st->solve_failed = p2pl_increment(v, Tinc) ? 0 : 1;
+          } else {
+            p2p_increment(v, Tinc);
+          }
+          double Tn[16];
+          mat4_mul(Tinc, st->T, Tn);
+#pragma unroll
+          for (int i = 0; i < 16; ++i) st->T[i] = Tn[i];
+          st->iterations += 1;
+          const double mse = v[NV - 1] / ncorr;
+          st->last_mse = mse;
+          if (conv_step(st, c, Tinc, mse)) {
+            st->converged = 1;
+            st->active = 0;
+          }
+        }
+        st->ticket = 0u;
+      }
+    }
+  }
+}
+
+template <int TILE>
+__global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max_range) {
+  __shared__ double lds[2 * (TILE / 64)];
+  __shared__ int s_last;
+  const TileRange tr = xcd_tiles(b.ntiles);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int p = b.tile_pair[tile];
+    PairState* st = b.state + p;
+    const PairDesc pd = b.pairs[p];
+    const GridDesc gs = b.grids[pd.src];
+    const GridDesc gt = b.grids[pd.tgt];
+    const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (local < gs.npts) {
+      const float4 s = b.pts[gs.pt_off + local];
+      float qx, qy, qz;
+      xform(st->T, s, qx, qy, qz);
+      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, INFINITY,
+                                   false);
+      if (nn.j >= 0 && (double)nn.d2 <= max_range) {
+        v[0] = nn.d2;
+        v[1] = 1.0;
+      }
+    }
+    block_sum<2>(v, lds);
+    if (threadIdx.x == 0) {
+      double* out = b.partials + (int64_t)tile * kMaxVals;
+      out[0] = v[0];
+      out[1] = v[1];
+      __threadfence();
+      const unsigned t = atomicAdd(&st->ticket, 1u);
+      s_last = (t == (unsigned)(pd.ntiles - 1));
+    }
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      v[0] = v[1] = 0.0;
+      for (int t = threadIdx.x; t < pd.ntiles; t += TILE) {
+        const double* in = b.partials + (int64_t)(pd.tile_begin + t) * kMaxVals;
+        v[0] += in[0];
+        v[1] += in[1];
+      }
+      block_sum<2>(v, lds);
+      if (threadIdx.x == 0) {
+        st->fit_count = (int)v[1];
+        st->fitness = v[1] > 0.0 ? v[0] / v[1] : DBL_MAX;
+        st->ticket = 0u;
+      }
+    }
+  }
+}
+
+__global__ void k_state_init(PairState* st, const double* guesses, const PairDesc* pairs,
+                             const GridDesc* grids, int npairs) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  PairState s;
+  for (int i = 0; i < 16; ++i)
+    s.T[i] = guesses ? guesses[16 * (int64_t)p + i] : ((i % 5 == 0) ? 1.0 : 0.0);
+  s.prev_mse = DBL_MAX;
+  s.last_mse = 0.0;
+  s.fitness = DBL_MAX;
+  s.iterations = 0;
+  s.state = kNotConverged;
+  // Registration::initCompute fails on an empty source or target: nothing runs
+  const PairDesc pd = pairs[p];
+  s.active = (grids[pd.src].npts > 0 && grids[pd.tgt].npts > 0) ? 1 : 0;
+  s.converged = 0;
+  s.similar = 0;
+  s.ncorr = 0;
+  s.solve_failed = 0;
+  s.ticket = 0u;
+  s.fit_count = 0;
+  s.pad_ = 0;
+  st[p] = s;
+}
+
+__global__ void k_correspondences(const IcpBuffers b, int pair, int32_t* idx, float* d2) {
+  const PairDesc pd = b.pairs[pair];
+  const GridDesc gs = b.grids[pd.src];
+  const GridDesc gt = b.grids[pd.tgt];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= gs.npts) return;
+  const float4 s = b.pts[gs.pt_off + i];
+  float qx, qy, qz;
+  xform(b.state[pair].T, s, qx, qy, qz);
+  const float r = (float)sqrt(pd.max_corr_d2);
+  const NNResult nn =
+      nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r, isfinite(r));
+  const int o = __float_as_int(s.w);
+  const bool keep = nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2;
+  idx[o] = keep ? nn.id : -1;
+  d2[o] = nn.d2;
+}
+
+// raw: the source cloud in original order (x, y, z, *) and its normals (nullable)
+__global__ void k_transform_output(const float4* raw, const float4* raw_nrm, int64_t n,
+                                   const PairState* st, int pair, float* out_xyz,
+                                   float* out_nrm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* T = st[pair].T;
+  const float4 p = raw[i];
+  if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+    float x, y, z;
+    xform(T, p, x, y, z);
+    out_xyz[3 * i] = x;
+    out_xyz[3 * i + 1] = y;
+    out_xyz[3 * i + 2] = z;
+  } else {
+    out_xyz[3 * i] = p.x;
+    out_xyz[3 * i + 1] = p.y;
+    out_xyz[3 * i + 2] = p.z;
+  }
+  if (out_nrm && raw_nrm) {
+    const float4 m = raw_nrm[i];
+    if (isfinite(m.x) && isfinite(m.y) && isfinite(m.z)) {
+      const double nx = m.x, ny = m.y, nz = m.z;
+      out_nrm[3 * i] = (float)((T[0] * nx + T[1] * ny) + T[2] * nz);
+      out_nrm[3 * i + 1] = (float)((T[4] * nx + T[5] * ny) + T[6] * nz);
+      out_nrm[3 * i + 2] = (float)((T[8] * nx + T[9] * ny) + T[10] * nz);
+    } else {
+      out_nrm[3 * i] = m.x;
+      out_nrm[3 * i + 1] = m.y;
+      out_nrm[3 * i + 2] = m.z;
+    }
+  }
+}
+
+template <bool P2PL, int TILE>
+void launch_iter_t(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((k_icp_iter<P2PL, TILE>), dim3(nblocks), dim3(TILE), 0, s, b, c);
+}
+
+}  // namespace
+
+void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+  const bool pl = c.point_to_plane != 0;
+  switch (b.tile) {
+    case 64:
+      pl ? launch_iter_t<true, 64>(b, c, nblocks, s) : launch_iter_t<false, 64>(b, c, nblocks, s);
+      break;
+    case 128:
+      pl ? launch_iter_t<true, 128>(b, c, nblocks, s) : launch_iter_t<false, 128>(b, c, nblocks, s);
+      break;
+    default:
+      pl ? launch_iter_t<true, 256>(b, c, nblocks, s) : launch_iter_t<false, 256>(b, c, nblocks, s);
+      break;
+  }
+}
+
+void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s) {
+  switch (b.tile) {
+    case 64:
+      hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_fitness<128>, dim3(nblocks), dim3(128), 0, s, b, max_range);
+      break;
+    default:
+      hipLaunchKernelGGL(k_fitness<256>, dim3(nblocks), dim3(256), 0, s, b, max_range);
+      break;
+  }
+}
+
+void launch_state_init(PairState* st, const double* guesses, const PairDesc* pairs,
+                       const GridDesc* grids, int npairs, hipStream_t s) {
+  hipLaunchKernelGGL(k_state_init, dim3((npairs + 63) / 64), dim3(64), 0, s, st, guesses, pairs,
+                     grids, npairs);
+}
+
+void launch_correspondences(const IcpBuffers& b, int pair, int npts_src, int32_t* idx, float* d2,
+                            hipStream_t s) {
+  if (npts_src <= 0) return;
+  hipLaunchKernelGGL(k_correspondences, dim3((npts_src + 255) / 256), dim3(256), 0, s, b, pair,
+                     idx, d2);
+}
+
+void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n,
+                             const PairState* st, int pair, float* out_xyz, float* out_nrm,
+                             hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_transform_output, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, raw,
+                     raw_nrm, n, st, pair, out_xyz, out_nrm);
+}
+
+int icp_max_blocks_per_cu(int tile, bool p2pl) {
+  int nb = 0;
+  hipError_t e = hipSuccess;
+  switch (tile) {
+    case 64:
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64>, 64, 0);
+      break;
+    case 128:
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 128>, 128, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 128>, 128, 0);
+      break;
+    default:
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 256>, 256, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 256>, 256, 0);
+      break;
+  }
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+}  // namespace lcr

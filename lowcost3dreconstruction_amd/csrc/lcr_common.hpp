@@ -1,0 +1,84 @@
+// lcr_common.hpp -- device data layout shared by the HIP kernels and the host
+// runtime of liblcr.  Everything lives in HBM as:
+//   * per cloud: float4 points sorted by voxel cell (x, y, z, bit-cast original
+//     index), float4 normals in the same order (nx, ny, nz, curvature), and a
+//     dense cell-start table over the cloud's bounding box (ncells + 1 int32);
+//   * per pair / hypothesis: a PairDesc (which clouds, tile range, radius) and a
+//     PairState (FP64 transform, convergence bookkeeping, reduction ticket);
+//   * per source tile: FP64 partial sums written by the tile's block.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lcr {
+
+// pcl::registration::DefaultConvergenceCriteria::ConvergenceState
+enum ConvState : int {
+  kNotConverged = 0,
+  kIterations = 1,
+  kTransform = 2,
+  kAbsMse = 3,
+  kRelMse = 4,
+  kNoCorrespondences = 5,
+  kFailureAfterMaxIterations = 6,
+};
+
+// Uniform dense voxel grid over one cloud's bounding box.  Cell (x, y, z) has id
+// (z * ny + y) * nx + x; its points are pts[cell_start[id] .. cell_start[id+1]).
+// cell_start holds global offsets into the concatenated point array.
+struct GridDesc {
+  float ox, oy, oz;   // min corner
+  float h, inv_h;     // cell edge and its reciprocal
+  int nx, ny, nz;
+  int64_t cell_off;   // first entry of this grid's cell_start (ncells + 1 entries)
+  int64_t pt_off;     // first point of the cloud in the sorted arrays
+  int npts;           // finite points in the grid
+  int nraw;           // points handed in (including non-finite ones)
+};
+
+// One ICP instance (a pair, or one pose hypothesis of a pair).
+struct PairDesc {
+  int src;            // cloud index of the source
+  int tgt;            // cloud index of the target
+  int tile_begin;     // first work tile of this instance
+  int ntiles;         // tiles = ceil(source points / tile size)
+  double max_corr_d2; // setMaxCorrespondenceDistance^2
+};
+
+// Per-instance state, updated on the device by the last block of each iteration.
+struct PairState {
+  double T[16];        // composed transform (row-major), FP64
+  double prev_mse;     // DefaultConvergenceCriteria::correspondences_prev_mse_
+  double last_mse;     // MSE of the last iteration's correspondences
+  double fitness;      // getFitnessScore result
+  int iterations;      // nr_iterations_
+  int state;           // ConvState
+  int active;          // 1 while the instance still iterates
+  int converged;       // Registration::converged_
+  int similar;         // iterations_similar_transforms_
+  int ncorr;           // correspondences of the last iteration
+  int solve_failed;    // 6x6 system not solvable in the last iteration
+  unsigned int ticket; // blocks of the current launch that finished this instance
+  int fit_count;       // points counted by the fitness pass
+  int pad_;
+};
+
+struct IcpConst {
+  int max_iterations;
+  int point_to_plane;
+  double rotation_threshold;     // 1 - transformation_epsilon
+  double translation_threshold;  // transformation_epsilon
+  double mse_rel;                // euclidean_fitness_epsilon
+  double mse_abs;                // 1e-12
+};
+
+// FP64 partial sums per tile.
+//   point-to-point: [0] n, [1..3] sum s, [4..6] sum t, [7..15] sum s_a t_b (a-major),
+//                   [16] sum d2
+//   point-to-plane: [0] n, [1..21] upper triangle of A^T A (row-major),
+//                   [22..27] A^T b, [28] sum d2
+constexpr int kP2PVals = 17;
+constexpr int kP2LVals = 29;
+constexpr int kMaxVals = 29;
+
+}  // namespace lcr

@@ -1,0 +1,66 @@
+// lcr_kernels.hpp -- host-side launchers of the HIP kernels (all asynchronous on
+// the given stream).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lcr_common.hpp"
+
+namespace lcr {
+
+// ---- K1 grid build (grid.hip)
+void launch_bbox(const float4* raw, const int64_t* cloud_off, int nclouds, float* part,
+                 float* out, hipStream_t s);
+constexpr int kBBoxChunks = 64;  // partial boxes per cloud (part holds 7 floats each)
+void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
+void launch_cell_count(const float4* raw, const int* point_cloud, const GridDesc* grids,
+                       int64_t n, int* cid, int* counts, int* occupied, hipStream_t s);
+int64_t scan_tmp_ints(int64_t n);
+void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
+void launch_scatter(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n,
+                    const int* cell_start, int* fill, float4* pts, float4* nrm,
+                    const int64_t* cloud_off, const int* point_cloud, hipStream_t s);
+void launch_cell_sort(const int* cell_start, int64_t ncells_total, float4* pts, float4* nrm,
+                      hipStream_t s);
+
+// ---- K2-K6 ICP iteration, fitness, output (icp.hip)
+struct IcpBuffers {
+  const GridDesc* grids;
+  const int* cell_start;
+  const float4* pts;      // cell-sorted points of all clouds
+  const float4* nrm;      // cell-sorted normals (nullable for point-to-point)
+  const PairDesc* pairs;
+  const int* tile_pair;   // tile -> instance
+  PairState* state;
+  double* partials;       // ntiles * kMaxVals
+  int ntiles;
+  int npairs;
+  int tile;               // source points per tile (= threads per block)
+};
+
+// One ICP iteration for every active instance (one launch).
+void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s);
+// getFitnessScore(max_range) for every instance with a nonzero iteration count or
+// all instances when `all` is set.
+void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s);
+// Initialise instance states from row-major 4x4 guesses (nullable = identity);
+// instances with an empty source or target start inactive.
+void launch_state_init(PairState* st, const double* guesses, const PairDesc* pairs,
+                       const GridDesc* grids, int npairs, hipStream_t s);
+// Correspondences of each source point of instance `pair` under its current
+// transform: original target index (-1 if d2 > r^2) and d2, in original order.
+void launch_correspondences(const IcpBuffers& b, int pair, int npts_src, int32_t* idx, float* d2,
+                            hipStream_t s);
+// K8: raw source points (original order) transformed by the instance's final
+// transform (xyz, and normals when given); non-finite points are copied.
+void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n,
+                             const PairState* st, int pair, float* out_xyz, float* out_nrm,
+                             hipStream_t s);
+int icp_max_blocks_per_cu(int tile, bool p2pl);
+
+// ---- K7 normals (normals.hip)
+void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
+                    const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
+                    int flip_all, float4* nrm_sorted, hipStream_t s);
+
+}  // namespace lcr

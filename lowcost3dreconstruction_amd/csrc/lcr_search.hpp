@@ -1,0 +1,181 @@
+// lcr_search.hpp -- exact nearest-neighbour search over the dense voxel grid.
+//
+// Replaces KdTreeFLANN::nearestKSearch(k = 1) as used by CorrespondenceEstimation
+// (fine_registration.cpp:121) and Registration::getFitnessScore
+// (fine_registration.cpp:126).  The search visits Chebyshev rings of cells around
+// the query's cell; after ring k every point not yet visited lies outside the
+// (2k+1)^3 block, so its distance is at least the query's distance to the block
+// boundary on the sides where grid cells remain.  The search stops once the best
+// distance is below that bound (the answer is then exact), once the bound exceeds
+// the correspondence radius (nothing unvisited can be kept), or once the block
+// covers the grid.  Distances use FLANN's float form ((0+dx^2)+dy^2)+dz^2 with
+// dx = q - p and no FMA contraction; ties resolve to the smallest original index.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lcr_common.hpp"
+
+namespace lcr {
+
+#pragma clang fp contract(off)
+
+#define LCR_SHD __host__ __device__ __forceinline__
+
+LCR_SHD int f2i(float f) { return __builtin_bit_cast(int, f); }
+
+LCR_SHD float flann_d2(float qx, float qy, float qz, const float4& p) {
+  const float dx = qx - p.x;
+  const float dy = qy - p.y;
+  const float dz = qz - p.z;
+  float r = dx * dx;
+  r = r + dy * dy;
+  r = r + dz * dz;
+  return r;
+}
+
+// Cell coordinates of a query in cell units (clamped far outside the grid so the
+// integer conversion cannot overflow).
+struct CellPos {
+  float fx, fy, fz;
+  int cx, cy, cz;
+  float slack;  // cell-unit rounding allowance for the certification bound
+};
+
+LCR_SHD CellPos cell_pos(const GridDesc& g, float qx, float qy, float qz) {
+  CellPos c;
+  c.fx = fminf(fmaxf((qx - g.ox) * g.inv_h, -4.0e6f), 4.0e6f);
+  c.fy = fminf(fmaxf((qy - g.oy) * g.inv_h, -4.0e6f), 4.0e6f);
+  c.fz = fminf(fmaxf((qz - g.oz) * g.inv_h, -4.0e6f), 4.0e6f);
+  c.cx = (int)floorf(c.fx);
+  c.cy = (int)floorf(c.fy);
+  c.cz = (int)floorf(c.fz);
+  // points are binned with the same (p - o) * inv_h arithmetic; cell coordinates
+  // of magnitude <= 4096 carry < 5e-4 of rounding, larger ones ~1.2e-7 relative
+  c.slack = 1.0e-3f + 1.0e-6f * fmaxf(fabsf(c.fx), fmaxf(fabsf(c.fy), fabsf(c.fz)));
+  return c;
+}
+
+// Distance in cell units from the query to the nearest face of block k behind
+// which unvisited grid cells remain; +inf once the block covers the grid.
+LCR_SHD float block_gap(const GridDesc& g, const CellPos& c, int k) {
+  float L = INFINITY;
+  if (c.cx - k > 0) L = fminf(L, c.fx - (float)(c.cx - k));
+  if (c.cx + k + 1 <= g.nx - 1) L = fminf(L, (float)(c.cx + k + 1) - c.fx);
+  if (c.cy - k > 0) L = fminf(L, c.fy - (float)(c.cy - k));
+  if (c.cy + k + 1 <= g.ny - 1) L = fminf(L, (float)(c.cy + k + 1) - c.fy);
+  if (c.cz - k > 0) L = fminf(L, c.fz - (float)(c.cz - k));
+  if (c.cz + k + 1 <= g.nz - 1) L = fminf(L, (float)(c.cz + k + 1) - c.fz);
+  return L;
+}
+
+// First ring that intersects the grid.
+LCR_SHD int first_ring(const GridDesc& g, const CellPos& c) {
+  const int kx = c.cx < 0 ? -c.cx : (c.cx >= g.nx ? c.cx - g.nx + 1 : 0);
+  const int ky = c.cy < 0 ? -c.cy : (c.cy >= g.ny ? c.cy - g.ny + 1 : 0);
+  const int kz = c.cz < 0 ? -c.cz : (c.cz >= g.nz ? c.cz - g.nz + 1 : 0);
+  return max(kx, max(ky, kz));
+}
+
+// Calls f(first_cell, last_cell) for every run of x-contiguous cells of ring k
+// (cells at Chebyshev distance exactly k), clipped to the grid.
+template <class F>
+LCR_SHD void for_ring_runs(const GridDesc& g, const CellPos& c, int k, F&& f) {
+  const int z0 = max(c.cz - k, 0), z1 = min(c.cz + k, g.nz - 1);
+  const int y0 = max(c.cy - k, 0), y1 = min(c.cy + k, g.ny - 1);
+  const int x0 = max(c.cx - k, 0), x1 = min(c.cx + k, g.nx - 1);
+  if (z0 > z1 || y0 > y1 || x0 > x1) return;
+  for (int z = z0; z <= z1; ++z) {
+    const bool zface = (z == c.cz - k) || (z == c.cz + k);
+    for (int y = y0; y <= y1; ++y) {
+      const int row = (z * g.ny + y) * g.nx;
+      if (zface || y == c.cy - k || y == c.cy + k) {
+        f(row + x0, row + x1);
+      } else {
+        if (c.cx - k >= 0) f(row + c.cx - k, row + c.cx - k);
+        if (k > 0 && c.cx + k <= g.nx - 1) f(row + c.cx + k, row + c.cx + k);
+      }
+    }
+  }
+}
+
+struct NNResult {
+  float d2;  // FLANN-form squared distance (+inf if none)
+  int id;    // original index of the match (INT_MAX if none)
+  int j;     // position of the match in the sorted point array (-1 if none)
+};
+
+// Exact 1-NN of q among the grid's points.  With `bounded`, points farther than
+// `r` may be skipped (the result is then either the true nearest or a point the
+// caller rejects as d2 > r^2).
+LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
+                                             const float4* __restrict__ P, float qx, float qy,
+                                             float qz, float r, bool bounded) {
+  NNResult res{INFINITY, 0x7fffffff, -1};
+  if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
+  const CellPos c = cell_pos(g, qx, qy, qz);
+  const int k0 = first_ring(g, c);
+  const int kmax = k0 + max(g.nx, max(g.ny, g.nz));
+  const float rlim = r * 1.00001f;
+  if (bounded && k0 > 0) {
+    const float L = (block_gap(g, c, k0 - 1) - c.slack) * g.h;
+    if (L > rlim) return res;
+  }
+  float best = INFINITY;
+  int bid = 0x7fffffff, bj = -1;
+  for (int k = k0; k <= kmax; ++k) {
+    for_ring_runs(g, c, k, [&](int c0, int c1) {
+      const int s = cs[c0], e = cs[c1 + 1];
+      for (int j = s; j < e; ++j) {
+        const float4 p = P[j];
+        const float d = flann_d2(qx, qy, qz, p);
+        const int id = f2i(p.w);
+        if (d < best || (d == best && id < bid)) {
+          best = d;
+          bid = id;
+          bj = j;
+        }
+      }
+    });
+    const float Lc = block_gap(g, c, k);
+    if (Lc == INFINITY) break;
+    const float L = (Lc - c.slack) * g.h;
+    if (L > 0.0f && best < L * L * 0.99998f) break;
+    if (bounded && L > rlim) break;
+  }
+  res.d2 = best;
+  res.id = bid;
+  res.j = bj;
+  return res;
+}
+
+// ------------------------------------------------------------------ reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of N doubles per thread; the totals are valid in thread 0.
+// `lds` must hold N * (blockDim.x / 64) doubles.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[wid * N + i] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] += lds[w * N + i];
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace lcr

@@ -1,0 +1,134 @@
+// mathcheck.hip -- host entry points over the FP64 solvers of lcr_math.hpp (the
+// same __host__ __device__ code the ICP and normals kernels run), so the CPU test
+// suite can check them against the oracle without a GPU.  Not part of liblcr's
+// C ABI.
+#include <string.h>
+
+#include "lcr_common.hpp"
+#include "lcr_math.hpp"
+#include "lcr_search.hpp"
+
+#include <math.h>
+#include <vector>
+
+using namespace lcr;
+
+extern "C" {
+
+// v: point-to-point moments (17 doubles, lcr_common.hpp layout) -> row-major T
+void lcrm_p2p_increment(const double* v, double* T) { p2p_increment(v, T); }
+
+// v: point-to-plane moments (29 doubles) -> T; returns 1 when solved
+int lcrm_p2pl_increment(const double* v, double* T) { return p2pl_increment(v, T) ? 1 : 0; }
+
+// S: 3x3 row-major cross-covariance (source first) -> R row-major
+void lcrm_horn(const double* S, double* R) {
+  double s[3][3], r[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) s[i][j] = S[3 * i + j];
+  horn_rotation(s, r);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[3 * i + j] = r[i][j];
+}
+
+// A: symmetric 3x3 -> w ascending, V columns
+void lcrm_eigh3(const double* A, double* w, double* V) {
+  double a[3][3], v[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = A[3 * i + j];
+  eigh3(a, w, v);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V[3 * i + j] = v[i][j];
+}
+
+// One convergence step.  io[0..3] = iterations, state, similar, converged;
+// dio[0] = prev_mse.  cst: max_iterations, then (rot_thr, trans_thr, rel, abs).
+int lcrm_conv_step(int* io, double* dio, int max_iterations, const double* thr,
+                   const double* Tinc, double mse) {
+  PairState st{};
+  st.iterations = io[0];
+  st.state = io[1];
+  st.similar = io[2];
+  st.prev_mse = dio[0];
+  IcpConst c{};
+  c.max_iterations = max_iterations;
+  c.rotation_threshold = thr[0];
+  c.translation_threshold = thr[1];
+  c.mse_rel = thr[2];
+  c.mse_abs = thr[3];
+  const bool done = conv_step(&st, c, Tinc, mse);
+  io[1] = st.state;
+  io[2] = st.similar;
+  io[3] = done ? 1 : 0;
+  dio[0] = st.prev_mse;
+  return done ? 1 : 0;
+}
+
+// Host build of one cloud's grid in the device layout (cell edge h, origin at the
+// bounding-box minimum, points sorted by cell then by index), then nn_query for
+// every query: exercises the ring search and its certification bound on the CPU.
+// Returns 0, or -1 when the grid would be too large.
+int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float h, float r,
+                 int bounded, int32_t* idx, float* d2) {
+  GridDesc g{};
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  int npts = 0;
+  for (int64_t i = 0; i < nt; ++i) {
+    const float* p = tgt + 3 * i;
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))) continue;
+    ++npts;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], p[a]);
+      hi[a] = fmaxf(hi[a], p[a]);
+    }
+  }
+  g.npts = npts;
+  g.h = h;
+  g.inv_h = (float)(1.0 / (double)h);
+  if (npts == 0) {
+    g.nx = g.ny = g.nz = 1;
+  } else {
+    g.ox = lo[0];
+    g.oy = lo[1];
+    g.oz = lo[2];
+    g.nx = (int)(floor(((double)hi[0] - lo[0]) * (double)g.inv_h) + 1);
+    g.ny = (int)(floor(((double)hi[1] - lo[1]) * (double)g.inv_h) + 1);
+    g.nz = (int)(floor(((double)hi[2] - lo[2]) * (double)g.inv_h) + 1);
+  }
+  const int64_t ncells = (int64_t)g.nx * g.ny * g.nz;
+  if (ncells > (int64_t)1 << 26) return -1;
+  std::vector<int> cnt(ncells + 1, 0), cid(nt, -1);
+  for (int64_t i = 0; i < nt; ++i) {
+    const float* p = tgt + 3 * i;
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))) continue;
+    int x = (int)floorf((p[0] - g.ox) * g.inv_h), y = (int)floorf((p[1] - g.oy) * g.inv_h),
+        z = (int)floorf((p[2] - g.oz) * g.inv_h);
+    x = x < 0 ? 0 : (x > g.nx - 1 ? g.nx - 1 : x);
+    y = y < 0 ? 0 : (y > g.ny - 1 ? g.ny - 1 : y);
+    z = z < 0 ? 0 : (z > g.nz - 1 ? g.nz - 1 : z);
+    cid[i] = (z * g.ny + y) * g.nx + x;
+    cnt[cid[i]]++;
+  }
+  std::vector<int> cs(ncells + 1, 0);
+  for (int64_t c = 0; c < ncells; ++c) cs[c + 1] = cs[c] + cnt[c];
+  std::vector<int> fill(cs.begin(), cs.end() - 1);
+  std::vector<float4> P((size_t)(npts > 0 ? npts : 1));
+  for (int64_t i = 0; i < nt; ++i) {
+    if (cid[i] < 0) continue;
+    const float* p = tgt + 3 * i;
+    P[fill[cid[i]]++] = make_float4(p[0], p[1], p[2], __builtin_bit_cast(float, (int)i));
+  }
+  for (int64_t i = 0; i < nq; ++i) {
+    const NNResult res = nn_query(g, cs.data(), P.data(), q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
+                                  bounded != 0);
+    const bool keep = res.j >= 0 && (!bounded || (double)res.d2 <= (double)r * (double)r);
+    idx[i] = keep ? res.id : -1;
+    d2[i] = res.d2;
+  }
+  return 0;
+}
+
+// C = A * B (4x4 row-major)
+void lcrm_mat4_mul(const double* A, const double* B, double* C) { mat4_mul(A, B, C); }
+
+}  // extern "C"

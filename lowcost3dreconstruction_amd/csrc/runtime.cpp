@@ -1,0 +1,462 @@
+// runtime.cpp -- host runtime of liblcr (see runtime.hpp).
+#include "runtime.hpp"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+namespace lcr {
+
+// ------------------------------------------------------------------ profiler
+
+Profiler& Profiler::get() {
+  static Profiler p;
+  return p;
+}
+
+hipEvent_t Profiler::take() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  LCR_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Profiler::reset() {
+  std::lock_guard<std::mutex> g(mu_);
+  totals_.clear();
+}
+
+int Profiler::begin(const char* name, hipStream_t s, std::vector<Span>* into) {
+  if (!on_ || !into) return -1;
+  std::lock_guard<std::mutex> g(mu_);
+  Span sp{name, take(), take()};
+  LCR_HIP(hipEventRecord(sp.a, s));
+  into->push_back(sp);
+  return (int)into->size() - 1;
+}
+
+void Profiler::end(int span, hipStream_t s, std::vector<Span>* into) {
+  if (span < 0 || !into) return;
+  LCR_HIP(hipEventRecord((*into)[span].b, s));
+}
+
+void Profiler::collect(std::vector<Span>& spans, bool keep) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& sp : spans) {
+    LCR_HIP(hipEventSynchronize(sp.b));
+    float ms = 0.f;
+    LCR_HIP(hipEventElapsedTime(&ms, sp.a, sp.b));
+    auto& t = totals_[sp.name];
+    t.first += 1;
+    t.second += ms;
+    if (!keep) {
+      pool_.push_back(sp.a);
+      pool_.push_back(sp.b);
+    }
+  }
+  if (!keep) spans.clear();
+}
+
+bool Profiler::read(const std::string& name, int64_t* launches, double* ms) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = totals_.find(name);
+  if (it == totals_.end()) {
+    *launches = 0;
+    *ms = 0.0;
+    return false;
+  }
+  *launches = it->second.first;
+  *ms = it->second.second;
+  return true;
+}
+
+int device_cu_count() {
+  int dev = 0;
+  LCR_HIP(hipGetDevice(&dev));
+  int cus = 0;
+  LCR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus > 0 ? cus : 1;
+}
+
+// ------------------------------------------------------------------ clouds
+
+namespace {
+
+__global__ void k_segments(const int64_t* __restrict__ off, int n, int64_t total,
+                           int* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {  // last segment whose start is <= i
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  out[i] = lo;
+}
+
+void launch_segments(const int64_t* off, int n, int64_t total, int* out, hipStream_t s) {
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_segments, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, off, n,
+                     total, out);
+}
+
+constexpr int64_t kMaxDim = 4096;
+
+}  // namespace
+
+void CloudSet::upload(hipStream_t s, const std::vector<const float*>& xyz,
+                      const std::vector<const float*>& nrm, const std::vector<int64_t>& counts,
+                      int64_t stride_bytes) {
+  n = (int)xyz.size();
+  nraw = counts;
+  raw_off.assign(n + 1, 0);
+  for (int c = 0; c < n; ++c) raw_off[c + 1] = raw_off[c] + counts[c];
+  total_raw = raw_off[n];
+  has_nrm = true;
+  for (int c = 0; c < n; ++c)
+    if (!nrm[c]) has_nrm = false;
+  const size_t np = (size_t)std::max<int64_t>(total_raw, 1);
+  h_stage.reserve(np * sizeof(float4));
+  float4* st = h_stage.as<float4>();
+  int* pc = nullptr;
+  h_stage_nrm.reserve(np * (sizeof(float4) + sizeof(int)));
+  float4* sn = h_stage_nrm.as<float4>();
+  pc = reinterpret_cast<int*>(sn + np);
+  for (int c = 0; c < n; ++c) {
+    const char* base = reinterpret_cast<const char*>(xyz[c]);
+    const char* nb = reinterpret_cast<const char*>(nrm[c]);
+    for (int64_t i = 0; i < counts[c]; ++i) {
+      const float* p = reinterpret_cast<const float*>(base + i * stride_bytes);
+      st[raw_off[c] + i] = make_float4(p[0], p[1], p[2], 0.0f);
+      pc[raw_off[c] + i] = c;
+      if (has_nrm) {
+        const float* m = reinterpret_cast<const float*>(nb + i * stride_bytes);
+        sn[raw_off[c] + i] = make_float4(m[0], m[1], m[2], 0.0f);
+      }
+    }
+  }
+  d_raw.reserve(np * sizeof(float4));
+  d_point_cloud.reserve(np * sizeof(int));
+  d_cloud_off.reserve((n + 1) * sizeof(int64_t));
+  LCR_HIP(hipMemcpyAsync(d_raw.p, st, total_raw * sizeof(float4), hipMemcpyHostToDevice, s));
+  LCR_HIP(hipMemcpyAsync(d_point_cloud.p, pc, total_raw * sizeof(int), hipMemcpyHostToDevice, s));
+  h_small.reserve((n + 1) * sizeof(int64_t) + 4096);
+  memcpy(h_small.p, raw_off.data(), (n + 1) * sizeof(int64_t));
+  LCR_HIP(hipMemcpyAsync(d_cloud_off.p, h_small.p, (n + 1) * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+  if (has_nrm) {
+    d_raw_nrm.reserve(np * sizeof(float4));
+    LCR_HIP(hipMemcpyAsync(d_raw_nrm.p, sn, total_raw * sizeof(float4), hipMemcpyHostToDevice, s));
+  }
+  LCR_HIP(hipStreamSynchronize(s));  // staging buffers are reused by the next upload
+}
+
+namespace {
+// Disjoint regions of the pinned scratch used by one grid build (async copies of
+// different regions may be in flight together).
+struct BuildRegions {
+  size_t bbox, occ, grids, poff, total;
+  explicit BuildRegions(int n) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    bbox = 0;
+    occ = al(bbox + (size_t)n * 7 * sizeof(float));
+    grids = al(occ + (size_t)n * sizeof(int));
+    poff = al(grids + (size_t)n * sizeof(GridDesc));
+    total = al(poff + (size_t)(n + 1) * sizeof(int64_t));
+  }
+};
+}  // namespace
+
+void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans) {
+  Profiler& prof = Profiler::get();
+  if (factor <= 0.0) factor = 2.5;
+  const BuildRegions R(n);
+  h_build.reserve(R.total);
+  char* hb = h_build.as<char>();
+  float* bb = reinterpret_cast<float*>(hb + R.bbox);
+  int* occ = reinterpret_cast<int*>(hb + R.occ);
+  GridDesc* hg = reinterpret_cast<GridDesc*>(hb + R.grids);
+  int64_t* hpoff = reinterpret_cast<int64_t*>(hb + R.poff);
+  const int span = prof.begin("grid_build", s, spans);
+  // 1. bounding boxes and finite counts
+  d_bbox_part.reserve((size_t)n * kBBoxChunks * 7 * sizeof(float));
+  d_bbox.reserve((size_t)n * 7 * sizeof(float));
+  launch_bbox(d_raw.as<float4>(), d_cloud_off.as<int64_t>(), n, d_bbox_part.as<float>(),
+              d_bbox.as<float>(), s);
+  LCR_HIP(hipMemcpyAsync(bb, d_bbox.p, (size_t)n * 7 * sizeof(float), hipMemcpyDeviceToHost, s));
+  LCR_HIP(hipStreamSynchronize(s));
+  std::vector<float> box(bb, bb + (size_t)n * 7);
+
+  // 2. probe grid (<= 65 cells on the longest axis) to measure occupancy
+  grids.assign(n, GridDesc{});
+  int64_t cells = 0;
+  for (int c = 0; c < n; ++c) {
+    GridDesc& g = grids[c];
+    const float* b = &box[(size_t)c * 7];
+    g.npts = (int)b[6];
+    g.nraw = (int)nraw[c];
+    if (g.npts == 0) {
+      g.ox = g.oy = g.oz = 0.f;
+      g.h = 1.f;
+      g.inv_h = 1.f;
+      g.nx = g.ny = g.nz = 1;
+    } else {
+      const double ext = std::max({(double)b[3] - b[0], (double)b[4] - b[1], (double)b[5] - b[2]});
+      const double h0 = std::max(ext / 64.0, 1e-6);
+      g.ox = b[0];
+      g.oy = b[1];
+      g.oz = b[2];
+      g.h = (float)h0;
+      g.inv_h = (float)(1.0 / h0);
+      g.nx = (int)std::min<double>(floor(((double)b[3] - b[0]) / h0) + 1, 65);
+      g.ny = (int)std::min<double>(floor(((double)b[4] - b[1]) / h0) + 1, 65);
+      g.nz = (int)std::min<double>(floor(((double)b[5] - b[2]) / h0) + 1, 65);
+    }
+    g.cell_off = cells;
+    cells += (int64_t)g.nx * g.ny * g.nz + 1;
+  }
+  d_grids.reserve((size_t)n * sizeof(GridDesc));
+  d_counts.reserve((size_t)cells * sizeof(int));
+  d_occ.reserve((size_t)n * sizeof(int));
+  d_cid.reserve((size_t)std::max<int64_t>(total_raw, 1) * sizeof(int));
+  memcpy(hg, grids.data(), (size_t)n * sizeof(GridDesc));
+  LCR_HIP(hipMemcpyAsync(d_grids.p, hg, (size_t)n * sizeof(GridDesc), hipMemcpyHostToDevice, s));
+  launch_fill_i32(d_counts.as<int>(), cells, 0, s);
+  launch_fill_i32(d_occ.as<int>(), n, 0, s);
+  launch_cell_count(d_raw.as<float4>(), d_point_cloud.as<int>(), d_grids.as<GridDesc>(),
+                    total_raw, d_cid.as<int>(), d_counts.as<int>(), d_occ.as<int>(), s);
+  LCR_HIP(hipMemcpyAsync(occ, d_occ.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+  LCR_HIP(hipStreamSynchronize(s));
+
+  // 3. final cell edge: factor * spacing, spacing ~ h0 / sqrt(points per occupied
+  //    probe cell) for surface-sampled clouds; capped cell count and extent
+  cells = 0;
+  total_pts = 0;
+  for (int c = 0; c < n; ++c) {
+    GridDesc& g = grids[c];
+    const float* b = &box[(size_t)c * 7];
+    g.pt_off = total_pts;
+    total_pts += g.npts;
+    if (g.npts > 0) {
+      const double h0 = g.h;
+      const double per_cell = (double)g.npts / std::max(occ[c], 1);
+      const double spacing = h0 / sqrt(std::max(per_cell, 1.0));
+      double h = std::max(factor * spacing, 1e-6);
+      const double ex = (double)b[3] - b[0], ey = (double)b[4] - b[1], ez = (double)b[5] - b[2];
+      const double max_cells = std::max<double>(16.0 * g.npts, 4096.0);
+      for (int it = 0; it < 64; ++it) {
+        const double nx = floor(ex / h) + 1, ny = floor(ey / h) + 1, nz = floor(ez / h) + 1;
+        if (nx <= kMaxDim && ny <= kMaxDim && nz <= kMaxDim && nx * ny * nz <= max_cells) break;
+        h *= 1.25;
+      }
+      g.h = (float)h;
+      g.inv_h = (float)(1.0 / (double)g.h);
+      g.nx = (int)(floor(ex * (double)g.inv_h) + 1);
+      g.ny = (int)(floor(ey * (double)g.inv_h) + 1);
+      g.nz = (int)(floor(ez * (double)g.inv_h) + 1);
+    }
+    g.cell_off = cells;
+    cells += (int64_t)g.nx * g.ny * g.nz + 1;
+  }
+  if (cells > (int64_t)1 << 30) throw Error{-5, "grid too large"};
+  total_cells = cells;
+  memcpy(hg, grids.data(), (size_t)n * sizeof(GridDesc));
+  LCR_HIP(hipMemcpyAsync(d_grids.p, hg, (size_t)n * sizeof(GridDesc), hipMemcpyHostToDevice, s));
+
+  // 4. count, scan, scatter, per-cell sort
+  d_counts.reserve((size_t)cells * sizeof(int));
+  d_cell_start.reserve((size_t)cells * sizeof(int));
+  d_scan_tmp.reserve((size_t)scan_tmp_ints(cells) * sizeof(int));
+  const size_t npts = (size_t)std::max<int64_t>(total_pts, 1);
+  d_pts.reserve(npts * sizeof(float4));
+  d_nrm.reserve(npts * sizeof(float4));
+  d_cloud_of_sorted.reserve(npts * sizeof(int));
+  launch_fill_i32(d_counts.as<int>(), cells, 0, s);
+  launch_cell_count(d_raw.as<float4>(), d_point_cloud.as<int>(), d_grids.as<GridDesc>(),
+                    total_raw, d_cid.as<int>(), d_counts.as<int>(), nullptr, s);
+  launch_exclusive_scan(d_counts.as<int>(), d_cell_start.as<int>(), cells, d_scan_tmp.as<int>(), s);
+  launch_fill_i32(d_counts.as<int>(), cells, 0, s);
+  launch_scatter(d_raw.as<float4>(), has_nrm ? d_raw_nrm.as<float4>() : nullptr, d_cid.as<int>(),
+                 total_raw, d_cell_start.as<int>(), d_counts.as<int>(), d_pts.as<float4>(),
+                 d_nrm.as<float4>(), d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), s);
+  launch_cell_sort(d_cell_start.as<int>(), cells - 1, d_pts.as<float4>(),
+                   has_nrm ? d_nrm.as<float4>() : nullptr, s);
+  // sorted point -> cloud
+  for (int c = 0; c < n; ++c) hpoff[c] = grids[c].pt_off;
+  hpoff[n] = total_pts;
+  d_poff.reserve((size_t)(n + 1) * sizeof(int64_t));
+  LCR_HIP(hipMemcpyAsync(d_poff.p, hpoff, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  launch_segments(d_poff.as<int64_t>(), n, total_pts, d_cloud_of_sorted.as<int>(), s);
+  prof.end(span, s, spans);
+  LCR_HIP(hipStreamSynchronize(s));  // the pinned regions are reused by the next build
+}
+
+void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
+                               std::vector<Profiler::Span>* spans) {
+  Profiler& prof = Profiler::get();
+  d_vp.reserve((size_t)n * 3 * sizeof(float));
+  h_vp.reserve((size_t)n * 3 * sizeof(float));
+  float* v = h_vp.as<float>();
+  for (int c = 0; c < n; ++c)
+    for (int a = 0; a < 3; ++a) v[3 * c + a] = vp_host ? vp_host[3 * c + a] : 0.0f;
+  LCR_HIP(hipMemcpyAsync(d_vp.p, v, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  const int span = prof.begin("normals", s, spans);
+  launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
+                 d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
+                 d_nrm.as<float4>(), s);
+  prof.end(span, s, spans);
+  has_nrm = true;
+  LCR_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ engine
+
+Engine::~Engine() { reset_graph(); }
+
+void Engine::reset_graph() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  graph_iters = -1;
+  Profiler& prof = Profiler::get();
+  if (!graph_spans.empty()) prof.collect(graph_spans, false);
+  graph_spans.clear();
+}
+
+void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
+                   const std::vector<double>& r2, int tile_pts) {
+  reset_graph();
+  tile = (tile_pts == 64 || tile_pts == 128 || tile_pts == 256) ? tile_pts : 256;
+  npairs = (int)src.size();
+  hp.assign(npairs, PairDesc{});
+  std::vector<int> tp;
+  ntiles = 0;
+  for (int p = 0; p < npairs; ++p) {
+    PairDesc& d = hp[p];
+    d.src = src[p];
+    d.tgt = tgt[p];
+    d.tile_begin = ntiles;
+    d.ntiles = (cs.grids[d.src].npts + tile - 1) / tile;
+    d.max_corr_d2 = r2[p];
+    for (int t = 0; t < d.ntiles; ++t) tp.push_back(p);
+    ntiles += d.ntiles;
+  }
+  d_pairs.reserve(std::max(npairs, 1) * sizeof(PairDesc));
+  d_tile_pair.reserve(std::max(ntiles, 1) * sizeof(int));
+  d_state.reserve((size_t)(npairs + 1) * sizeof(PairState));
+  d_partials.reserve((size_t)std::max(ntiles, 1) * kMaxVals * sizeof(double));
+  d_guess.reserve((size_t)std::max(npairs, 1) * 16 * sizeof(double));
+  h_state.reserve((size_t)(npairs + 1) * sizeof(PairState));
+  h_guess.reserve((size_t)std::max(npairs, 1) * 16 * sizeof(double));
+  if (npairs > 0)
+    LCR_HIP(hipMemcpy(d_pairs.p, hp.data(), npairs * sizeof(PairDesc), hipMemcpyHostToDevice));
+  if (ntiles > 0)
+    LCR_HIP(hipMemcpy(d_tile_pair.p, tp.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+  const int cus = device_cu_count();
+  const int per_cu = icp_max_blocks_per_cu(tile, c.point_to_plane != 0);
+  int nb = std::min(std::max(ntiles, 1), cus * per_cu);
+  nb = std::max(8, (nb + 7) / 8 * 8);
+  nblocks = nb;
+}
+
+IcpBuffers Engine::buffers(const CloudSet& cs) const {
+  IcpBuffers b;
+  b.grids = cs.d_grids.as<GridDesc>();
+  b.cell_start = cs.d_cell_start.as<int>();
+  b.pts = cs.d_pts.as<float4>();
+  b.nrm = cs.has_nrm ? cs.d_nrm.as<float4>() : nullptr;
+  b.pairs = d_pairs.as<PairDesc>();
+  b.tile_pair = d_tile_pair.as<int>();
+  b.state = d_state.as<PairState>();
+  b.partials = d_partials.as<double>();
+  b.ntiles = ntiles;
+  b.npairs = npairs;
+  b.tile = tile;
+  return b;
+}
+
+void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph) {
+  if (npairs == 0) return;
+  Profiler& prof = Profiler::get();
+  const IcpBuffers b = buffers(cs);
+  if (guesses) {
+    memcpy(h_guess.p, guesses, (size_t)npairs * 16 * sizeof(double));
+  } else {
+    double* g = h_guess.as<double>();
+    for (int p = 0; p < npairs; ++p)
+      for (int i = 0; i < 16; ++i) g[16 * p + i] = (i % 5 == 0) ? 1.0 : 0.0;
+  }
+  LCR_HIP(hipMemcpyAsync(d_guess.p, h_guess.p, (size_t)npairs * 16 * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  auto body = [&](std::vector<Profiler::Span>* spans) {
+    launch_state_init(d_state.as<PairState>(), d_guess.as<double>(), d_pairs.as<PairDesc>(),
+                      cs.d_grids.as<GridDesc>(), npairs, s);
+    if (ntiles > 0) {
+      for (int it = 0; it < c.max_iterations; ++it) {
+        const int sp = prof.begin("icp_iteration", s, spans);
+        launch_icp_iteration(b, c, nblocks, s);
+        prof.end(sp, s, spans);
+      }
+      const int sp = prof.begin("fitness", s, spans);
+      launch_fitness(b, DBL_MAX, nblocks, s);
+      prof.end(sp, s, spans);
+    }
+    LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
+                           hipMemcpyDeviceToHost, s));
+  };
+  if (!use_graph) {
+    std::vector<Profiler::Span> spans;
+    body(&spans);
+    LCR_HIP(hipStreamSynchronize(s));
+    if (!spans.empty()) prof.collect(spans, false);
+    return;
+  }
+  if (!exec || graph_iters != c.max_iterations || graph_prof != prof.on()) {
+    reset_graph();
+    graph_prof = prof.on();
+    LCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      body(&graph_spans);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      graph_spans.clear();
+      throw;
+    }
+    LCR_HIP(hipStreamEndCapture(s, &graph));
+    LCR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    graph_iters = c.max_iterations;
+  }
+  LCR_HIP(hipGraphLaunch(exec, s));
+  LCR_HIP(hipStreamSynchronize(s));
+  if (!graph_spans.empty()) prof.collect(graph_spans, true);
+}
+
+void Engine::run_fitness(hipStream_t s, const CloudSet& cs, double max_range) {
+  if (npairs == 0) return;
+  Profiler& prof = Profiler::get();
+  std::vector<Profiler::Span> spans;
+  const IcpBuffers b = buffers(cs);
+  if (ntiles > 0) {
+    const int sp = prof.begin("fitness", s, &spans);
+    launch_fitness(b, max_range, nblocks, s);
+    prof.end(sp, s, &spans);
+  }
+  LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
+                         hipMemcpyDeviceToHost, s));
+  LCR_HIP(hipStreamSynchronize(s));
+  if (!spans.empty()) prof.collect(spans, false);
+}
+
+}  // namespace lcr

@@ -1,0 +1,161 @@
+// runtime.hpp -- host runtime of liblcr: device buffers, the cloud store (upload +
+// batched grid build + normals), the batched ICP engine (one persistent launch
+// per iteration, captured once into a hipGraph), and the kernel profiler.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lcr_common.hpp"
+#include "lcr_kernels.hpp"
+
+namespace lcr {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define LCR_HIP(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::lcr::Error{-2, std::string(#expr) + " failed: " + hipGetErrorString(e_)};  \
+  } while (0)
+
+// Device allocation that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t n) {
+    if (n == 0) n = 256;
+    if (n <= bytes) return;
+    if (p) LCR_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    LCR_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Pinned host allocation that only grows.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void reserve(size_t n) {
+    if (n == 0) n = 256;
+    if (n <= bytes) return;
+    if (p) LCR_HIP(hipHostFree(p));
+    p = nullptr;
+    bytes = 0;
+    LCR_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+    bytes = n;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Per-kernel GPU time from HIP events recorded around each launch on its stream.
+class Profiler {
+ public:
+  struct Span {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  static Profiler& get();
+  bool on() const { return on_; }
+  void enable(bool v) { on_ = v; }
+  void reset();
+  // Records a begin event; returns the span index (or -1 when off).
+  int begin(const char* name, hipStream_t s, std::vector<Span>* into);
+  void end(int span, hipStream_t s, std::vector<Span>* into);
+  // Adds the elapsed times of completed spans to the totals (events stay owned
+  // by `spans` when `keep` is set, e.g. spans recorded inside a graph).
+  void collect(std::vector<Span>& spans, bool keep);
+  bool read(const std::string& name, int64_t* launches, double* ms);
+
+ private:
+  hipEvent_t take();
+  bool on_ = false;
+  std::mutex mu_;
+  std::map<std::string, std::pair<int64_t, double>> totals_;
+  std::vector<hipEvent_t> pool_;
+};
+
+// A set of clouds resident on one device: raw points (input order) and the
+// cell-sorted layout with one dense grid per cloud.
+struct CloudSet {
+  int n = 0;
+  std::vector<int64_t> nraw, raw_off;  // raw_off: n + 1 prefix
+  int64_t total_raw = 0;
+  bool has_nrm = false;
+  std::vector<GridDesc> grids;
+  int64_t total_pts = 0, total_cells = 0;
+
+  DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
+  DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
+  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff;
+  HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
+
+  // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;
+  // normals at the same stride from their own pointer).
+  void upload(hipStream_t s, const std::vector<const float*>& xyz,
+              const std::vector<const float*>& nrm, const std::vector<int64_t>& counts,
+              int64_t stride_bytes);
+  // Builds every cloud's grid; cell edge = factor * estimated point spacing.
+  void build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans);
+  // K7 normals for every cloud into d_nrm (cell-sorted order).
+  void compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
+                       std::vector<Profiler::Span>* spans);
+};
+
+// A batch of ICP instances over a CloudSet.
+struct Engine {
+  int npairs = 0, ntiles = 0, tile = 256, nblocks = 8;
+  IcpConst c{};
+  std::vector<PairDesc> hp;
+  DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess;
+  HostBuf h_state, h_guess;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<Profiler::Span> graph_spans;
+  bool graph_prof = false;
+  int graph_iters = -1;
+
+  ~Engine();
+  void reset_graph();
+  // (src, tgt) cloud indices per instance; squared radius per instance.
+  void setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
+             const std::vector<double>& r2, int tile_pts);
+  IcpBuffers buffers(const CloudSet& cs) const;
+  // Runs state init (from guesses, 16 doubles per instance or null) and up to
+  // max_iterations iterations plus the fitness pass; results land in h_state.
+  void run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph);
+  void run_fitness(hipStream_t s, const CloudSet& cs, double max_range);
+  const PairState* states() const { return h_state.as<PairState>(); }
+};
+
+int device_cu_count();
+
+}  // namespace lcr
