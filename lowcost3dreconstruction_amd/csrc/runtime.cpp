@@ -322,15 +322,13 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
 
 Engine::~Engine() { reset_graph(); }
 
+// Never throws: also runs from the destructor.
 void Engine::reset_graph() {
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   exec = nullptr;
   graph = nullptr;
   graph_iters = -1;
-  Profiler& prof = Profiler::get();
-  if (!graph_spans.empty()) prof.collect(graph_spans, false);
-  graph_spans.clear();
 }
 
 void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
@@ -371,6 +369,7 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
 
 IcpBuffers Engine::buffers(const CloudSet& cs) const {
   IcpBuffers b;
+  memset(&b, 0, sizeof b);  // padding takes part in the graph key comparison
   b.grids = cs.d_grids.as<GridDesc>();
   b.cell_start = cs.d_cell_start.as<int>();
   b.pts = cs.d_pts.as<float4>();
@@ -414,33 +413,46 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
     LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
                            hipMemcpyDeviceToHost, s));
   };
-  if (!use_graph) {
+  // HIP events recorded inside a captured graph cannot be timed afterwards, so
+  // with the profiler on the same launches are issued eagerly with an event pair
+  // around each kernel; otherwise the whole run replays one hipGraph.
+  if (!use_graph || prof.on()) {
     std::vector<Profiler::Span> spans;
     body(&spans);
     LCR_HIP(hipStreamSynchronize(s));
     if (!spans.empty()) prof.collect(spans, false);
     return;
   }
-  if (!exec || graph_iters != c.max_iterations || graph_prof != prof.on()) {
+  const GraphKey key = graph_key(b);
+  if (!exec || !(key == graph_key_)) {
     reset_graph();
-    graph_prof = prof.on();
     LCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-      body(&graph_spans);
+      body(nullptr);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
       if (g) (void)hipGraphDestroy(g);
-      graph_spans.clear();
       throw;
     }
     LCR_HIP(hipStreamEndCapture(s, &graph));
     LCR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    graph_iters = c.max_iterations;
+    graph_key_ = key;
   }
   LCR_HIP(hipGraphLaunch(exec, s));
   LCR_HIP(hipStreamSynchronize(s));
-  if (!graph_spans.empty()) prof.collect(graph_spans, true);
+}
+
+// Everything a captured graph bakes in: kernel arguments and buffer addresses.
+Engine::GraphKey Engine::graph_key(const IcpBuffers& b) const {
+  GraphKey k;
+  memset(&k, 0, sizeof k);
+  k.b = b;
+  k.c = c;
+  k.nblocks = nblocks;
+  k.guess = d_guess.p;
+  k.hstate = h_state.p;
+  return k;
 }
 
 void Engine::run_fitness(hipStream_t s, const CloudSet& cs, double max_range) {

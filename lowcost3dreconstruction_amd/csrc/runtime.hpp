@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string.h>
+
 #include <map>
 #include <mutex>
 #include <string>
@@ -139,9 +141,17 @@ struct Engine {
   HostBuf h_state, h_guess;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<Profiler::Span> graph_spans;
-  bool graph_prof = false;
   int graph_iters = -1;
+  struct GraphKey {
+    IcpBuffers b{};
+    IcpConst c{};
+    int nblocks = 0;
+    const void* guess = nullptr;
+    const void* hstate = nullptr;
+    bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof *this) == 0; }
+  };
+  GraphKey graph_key_{};
+  GraphKey graph_key(const IcpBuffers& b) const;
 
   ~Engine();
   void reset_graph();

@@ -219,6 +219,14 @@ def voxel_downsample(pts, leaf):
     return out
 
 
+def turntable_prior(yaw_step_deg, centre):
+    """The rig's prior for a pair (capture i onto capture i-1): a rotation by
+    -yaw_step about the vertical axis through the turntable centre, as
+    rotate_align/rotate_align.cpp:224-235 applies per capture."""
+    P = rigid(np.eye(3), np.asarray(centre, dtype=np.float64))
+    return P @ rigid(rot_axis_angle([0, 1, 0], np.deg2rad(-yaw_step_deg)), np.zeros(3)) @ np.linalg.inv(P)
+
+
 def turntable_pose(yaw, centre, jitter=np.zeros(3)):
     """Object -> camera: rotate by `yaw` about the vertical (camera Y) axis through
     the object centre, then place at `centre` (+ jitter)."""
@@ -292,7 +300,9 @@ def turntable_cycle(seed, n_captures=36, yaw_step_deg=10.0, crop=(0.05, 0.60), j
         pairs.append((0, n_captures - 1))
     pairs = np.asarray(pairs, dtype=np.int32)
     T_gt = np.stack([poses[t] @ np.linalg.inv(poses[s]) for s, t in pairs])
-    return Cycle(clouds, pairs, T_gt, np.stack(poses), meta=dict(seed=seed))
+    prior = turntable_prior(yaw_step_deg, centre)
+    return Cycle(clouds, pairs, T_gt, np.stack(poses),
+                 meta=dict(seed=seed, prior=np.stack([prior] * len(pairs))))
 
 
 def perturb(rng, max_angle_deg, max_trans):
