@@ -105,12 +105,39 @@ struct NNResult {
   int j;     // position of the match in the sorted point array (-1 if none)
 };
 
+// Lower bound, in cell units, of the distance from the query coordinate f (cell
+// units) to cell c along one axis, less the rounding slack.
+LCR_SHD float axis_gap(float f, int c, int qc, float slack) {
+  const float d = (c < qc) ? f - (float)(c + 1) : ((c > qc) ? (float)c - f : 0.0f);
+  return fmaxf(d - slack, 0.0f);
+}
+
+// Scans points [s, e) of the sorted array, keeping the lexicographic minimum of
+// (d2, original index).
+LCR_SHD void scan_points(const float4* __restrict__ P, int s, int e, float qx, float qy, float qz,
+                         float& best, int& bid, int& bj) {
+  for (int j = s; j < e; ++j) {
+    const float4 p = P[j];
+    const float d = flann_d2(qx, qy, qz, p);
+    const int id = f2i(p.w);
+    if (d < best || (d == best && id < bid)) {
+      best = d;
+      bid = id;
+      bj = j;
+    }
+  }
+}
+
 // Exact 1-NN of q among the grid's points.  With `bounded`, points farther than
 // `r` may be skipped (the result is then either the true nearest or a point the
-// caller rejects as d2 > r^2).
+// caller rejects as d2 > r^2).  `seed_j` (or -1) is a known candidate, e.g. the
+// previous iteration's match: it only tightens the pruning bound, the answer is
+// the same.  Inside each ring, rows and cells whose lower-bound distance exceeds
+// the current best are skipped (with a relative margin, so equal distances are
+// still visited and the smallest index wins).
 LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
-                                             const float4* __restrict__ P, float qx, float qy,
-                                             float qz, float r, bool bounded) {
+                          const float4* __restrict__ P, float qx, float qy, float qz, float r,
+                          bool bounded, int seed_j = -1) {
   NNResult res{INFINITY, 0x7fffffff, -1};
   if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
   const CellPos c = cell_pos(g, qx, qy, qz);
@@ -123,20 +150,52 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   }
   float best = INFINITY;
   int bid = 0x7fffffff, bj = -1;
+  if (seed_j >= 0) scan_points(P, seed_j, seed_j + 1, qx, qy, qz, best, bid, bj);
+  const float h2 = g.h * g.h;
   for (int k = k0; k <= kmax; ++k) {
-    for_ring_runs(g, c, k, [&](int c0, int c1) {
-      const int s = cs[c0], e = cs[c1 + 1];
-      for (int j = s; j < e; ++j) {
-        const float4 p = P[j];
-        const float d = flann_d2(qx, qy, qz, p);
-        const int id = f2i(p.w);
-        if (d < best || (d == best && id < bid)) {
-          best = d;
-          bid = id;
-          bj = j;
+    const int z0 = max(c.cz - k, 0), z1 = min(c.cz + k, g.nz - 1);
+    const int y0 = max(c.cy - k, 0), y1 = min(c.cy + k, g.ny - 1);
+    const int x0 = max(c.cx - k, 0), x1 = min(c.cx + k, g.nx - 1);
+    if (z0 <= z1 && y0 <= y1 && x0 <= x1) {
+      for (int z = z0; z <= z1; ++z) {
+        const float gz = axis_gap(c.fz, z, c.cz, c.slack);
+        const bool zface = (z == c.cz - k) || (z == c.cz + k);
+        for (int y = y0; y <= y1; ++y) {
+          const float gy = axis_gap(c.fy, y, c.cy, c.slack);
+          // bound on d2 for any point of this row (cell units^2 -> m^2)
+          const float thr = best * 1.00002f;
+          const float lyz = (gz * gz + gy * gy) * h2;
+          if (lyz > thr) continue;
+          int xa, xb;
+          bool two = false;
+          if (zface || y == c.cy - k || y == c.cy + k) {
+            xa = x0;
+            xb = x1;
+          } else {
+            xa = c.cx - k;
+            xb = c.cx + k;
+            two = true;
+          }
+          // trim the x range to cells whose x gap keeps the bound under thr
+          int xlo = xa, xhi = xb;
+          if (thr < INFINITY) {
+            const float rem = fmaxf(thr - lyz, 0.0f);
+            const float dxc = sqrtf(rem) * g.inv_h * 1.00001f + c.slack;
+            xlo = max(xa, (int)ceilf(c.fx - dxc - 1.0f));
+            xhi = min(xb, (int)floorf(c.fx + dxc));
+          }
+          const int row = (z * g.ny + y) * g.nx;
+          if (!two) {
+            if (xlo <= xhi) scan_points(P, cs[row + xlo], cs[row + xhi + 1], qx, qy, qz, best, bid, bj);
+          } else {
+            if (xa >= 0 && xa >= xlo && xa <= xhi)
+              scan_points(P, cs[row + xa], cs[row + xa + 1], qx, qy, qz, best, bid, bj);
+            if (k > 0 && xb <= g.nx - 1 && xb >= xlo && xb <= xhi)
+              scan_points(P, cs[row + xb], cs[row + xb + 1], qx, qy, qz, best, bid, bj);
+          }
         }
       }
-    });
+    }
     const float Lc = block_gap(g, c, k);
     if (Lc == INFINITY) break;
     const float L = (Lc - c.slack) * g.h;

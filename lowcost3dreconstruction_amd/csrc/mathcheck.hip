@@ -68,8 +68,10 @@ int lcrm_conv_step(int* io, double* dio, int max_iterations, const double* thr,
 // bounding-box minimum, points sorted by cell then by index), then nn_query for
 // every query: exercises the ring search and its certification bound on the CPU.
 // Returns 0, or -1 when the grid would be too large.
+// seed_ids (nullable): per query an original target index used as the search's
+// seed candidate (-1 = none).
 int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float h, float r,
-                 int bounded, int32_t* idx, float* d2) {
+                 int bounded, const int32_t* seed_ids, int32_t* idx, float* d2) {
   GridDesc g{};
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   int npts = 0;
@@ -118,9 +120,12 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
     const float* p = tgt + 3 * i;
     P[fill[cid[i]]++] = make_float4(p[0], p[1], p[2], __builtin_bit_cast(float, (int)i));
   }
+  std::vector<int> pos(nt, -1);
+  for (int j = 0; j < npts; ++j) pos[__builtin_bit_cast(int, P[j].w)] = j;
   for (int64_t i = 0; i < nq; ++i) {
+    const int seed = (seed_ids && seed_ids[i] >= 0) ? pos[seed_ids[i]] : -1;
     const NNResult res = nn_query(g, cs.data(), P.data(), q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
-                                  bounded != 0);
+                                  bounded != 0, seed);
     const bool keep = res.j >= 0 && (!bounded || (double)res.d2 <= (double)r * (double)r);
     idx[i] = keep ? res.id : -1;
     d2[i] = res.d2;

@@ -155,16 +155,18 @@ def test_mat4_mul(m):
     np.testing.assert_allclose(Cm.reshape(4, 4), A @ B, atol=1e-14)
 
 
-def _nn_grid(m, tgt, q, h, r, bounded):
+def _nn_grid(m, tgt, q, h, r, bounded, seeds=None):
     m.lcrm_nn_grid.argtypes = [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), C.c_int64,
                                C.c_float, C.c_float, C.c_int, C.POINTER(C.c_int32),
-                               C.POINTER(C.c_float)]
+                               C.POINTER(C.c_int32), C.POINTER(C.c_float)]
     tgt = np.ascontiguousarray(tgt, np.float32)
     q = np.ascontiguousarray(q, np.float32)
     idx = np.empty(len(q), np.int32)
     d2 = np.empty(len(q), np.float32)
     fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
-    rc = m.lcrm_nn_grid(fp(tgt), len(tgt), fp(q), len(q), h, r, int(bounded),
+    sp = None if seeds is None else np.ascontiguousarray(seeds, np.int32).ctypes.data_as(
+        C.POINTER(C.c_int32))
+    rc = m.lcrm_nn_grid(fp(tgt), len(tgt), fp(q), len(q), h, r, int(bounded), sp,
                         idx.ctypes.data_as(C.POINTER(C.c_int32)), fp(d2))
     assert rc == 0
     return idx, d2
@@ -187,7 +189,12 @@ def test_grid_ring_search_equals_bruteforce(m, h, r):
         rng.uniform(-3, 3, size=(50, 3)),                  # far outside the grid
         tgt[:20], [[np.inf, 0, 0]]]).astype(np.float32)
     bounded = np.isfinite(r)
-    idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded)
     ridx, rd2 = oracle.nn(tgt, q, max_dist=r, brute=True)
-    np.testing.assert_array_equal(idx, ridx)
-    np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
+    # no seed, a random (mostly wrong) seed, and the true answer as the seed
+    seeds_rand = rng.integers(-1, len(tgt), size=len(q))
+    seeds_rand[seeds_rand == 5] = -1  # the NaN point is not in the grid
+    seeds_true = np.where(ridx >= 0, ridx, -1)
+    for seeds in (None, seeds_rand, seeds_true):
+        idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded, seeds)
+        np.testing.assert_array_equal(idx, ridx)
+        np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
