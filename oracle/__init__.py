@@ -112,6 +112,9 @@ def lib():
                                              fp]
         L.oracle_transform_points.argtypes = [fp, C.c_int64, dp, fp]
         L.oracle_normals.argtypes = [fp, C.c_int64, C.c_int, fp, C.c_int, fp, fp, C.c_int]
+        L.oenc_forward.argtypes = [dp, fp, fp, C.c_int, C.c_int, dp, dp, ip]
+        L.oenc_loss_grad.argtypes = [dp, fp, fp, C.c_int, C.c_int, dp, C.c_int, dp]
+        L.oenc_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 
@@ -301,3 +304,38 @@ def normals(xyz, k, viewpoint=(0.0, 0.0, 0.0), flip_all=False, nthreads=8):
     lib().oracle_normals(_p(x, C.c_float), len(x), int(k), _p(vp, C.c_float), int(bool(flip_all)),
                          _p(nrm, C.c_float), _p(curv, C.c_float), int(nthreads))
     return nrm, curv
+
+
+# ---------------------------------------------------------------- learned proposal
+
+ENC_NPARAM = 174470
+ENC_LAYOUT = (("W1", (3, 64)), ("b1", (64,)), ("W2", (64, 128)), ("b2", (128,)),
+              ("W3", (128, 256)), ("b3", (256,)), ("Hw1", (512, 256)), ("hb1", (256,)),
+              ("Hw2", (256, 6)), ("hb2", (6,)))
+
+
+def enc_forward(w, src, tgt, quantize=True):
+    """FP64 forward of the Siamese encoder + head on one pair of normalised clouds.
+    Returns (out6, feat512, argmax512)."""
+    W = _f64(w)
+    s, t = _f32(src).reshape(-1, 3), _f32(tgt).reshape(-1, 3)
+    out, feat = np.empty(6), np.empty(512)
+    arg = np.empty(512, np.int32)
+    lib().oenc_forward(_p(W, C.c_double), _p(s, C.c_float), _p(t, C.c_float), len(s),
+                       int(bool(quantize)), _p(out, C.c_double), _p(feat, C.c_double),
+                       _p(arg, C.c_int32))
+    return out, feat, arg
+
+
+def enc_loss_grad(w, src, tgt, labels, quantize=True, grad=True):
+    """Mean squared loss over a batch (src, tgt: (B, n, 3); labels (B, 6)) and its
+    gradient with respect to the flat weights."""
+    W = _f64(w)
+    s, t = _f32(src), _f32(tgt)
+    B, n = s.shape[0], s.shape[1]
+    lab = _f64(labels).reshape(B, 6)
+    g = np.empty(ENC_NPARAM) if grad else None
+    loss = lib().oenc_loss_grad(_p(W, C.c_double), _p(s, C.c_float), _p(t, C.c_float), n, B,
+                                _p(lab, C.c_double), int(bool(quantize)),
+                                _p(g, C.c_double) if grad else None)
+    return (loss, g) if grad else loss

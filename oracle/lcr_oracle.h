@@ -22,8 +22,10 @@
  * PCL routine it follows.
  *
  * Pinning: every function is checked in tests/ against numpy-generated golden
- * vectors (tests/golden/), closed forms, invariants or brute force.  Functions
- * with no such check: none.
+ * vectors (tests/golden/), closed forms, invariants, brute force or finite
+ * differences.  Parity unpinned: oracle_icp in LCR_OR_PCL32 mode (PCL's float
+ * rounding cannot be checked without PCL; it serves only as the timed host
+ * baseline and agrees with LCR_OR_FP64 to 1e-3).
  *
  * Two arithmetic modes:
  *   LCR_OR_FP64  -- the FP64 oracle: the source is transformed from the original
@@ -166,6 +168,27 @@ void oracle_eigh3(const double A[9], double w[3], double V[9]);
  * flipped when flip_all.  Fewer than 3 neighbours -> NaN normal and curvature. */
 void oracle_normals(const float *xyz, int64_t n, int k, const float vp[3], int flip_all,
                     float *nrm, float *curv, int nthreads);
+
+/* ---- learned initial-pose proposal (lcr_encoder_oracle.c) ---- */
+/* flat FP64 weight layout, row-major [in][out] */
+#define ENC_W1 0
+#define ENC_B1 (ENC_W1 + 3 * 64)
+#define ENC_W2 (ENC_B1 + 64)
+#define ENC_B2 (ENC_W2 + 64 * 128)
+#define ENC_W3 (ENC_B2 + 128)
+#define ENC_B3 (ENC_W3 + 128 * 256)
+#define ENC_HW1 (ENC_B3 + 256)
+#define ENC_HB1 (ENC_HW1 + 512 * 256)
+#define ENC_HW2 (ENC_HB1 + 256)
+#define ENC_HB2 (ENC_HW2 + 256 * 6)
+#define ENC_NPARAM (ENC_HB2 + 6) /* 174470 */
+/* forward of one pair of n-point normalised clouds */
+void oenc_forward(const double *w, const float *src, const float *tgt, int n, int quantize,
+                  double out[6], double feat[512], int32_t argmax[512]);
+/* mean squared loss over `batch` pairs (src/tgt: batch * n * 3 floats, labels:
+ * batch * 6) and its gradient (nullable) */
+double oenc_loss_grad(const double *w, const float *src, const float *tgt, int n, int batch,
+                      const double *labels, int quantize, double *grad);
 
 #ifdef __cplusplus
 }
