@@ -156,6 +156,35 @@ void lcr_cycle_destroy(lcr_cycle *c);
 /* One-shot: create, run, results, destroy. */
 int lcr_icp_batch_align(const lcr_cycle_desc *d, lcr_cycle_result *r);
 
+/* ---- learned initial-pose proposal.  No counterpart in the reference: it takes
+ * the role of the coarse alignment before fine registration (Super4PCS at
+ * scripts/alignment.sh:107).  A Siamese point encoder (3->64->128->256, max-pool,
+ * shared by both clouds) and a head (512->256->6) map a pair of normalised clouds
+ * to an increment (axis-angle omega, translation tau).  Layers 2-3 run on BF16
+ * MFMA with FP32 accumulation; weights are FP32 (174470 floats, row-major
+ * [in][out]: W1 3x64, b1, W2 64x128, b2, W3 128x256, b3, Hw1 512x256, hb1, Hw2
+ * 256x6, hb2).  Clouds are n_pairs * n_points * 3 floats each. */
+typedef struct lcr_proposal lcr_proposal;
+int lcr_proposal_create(lcr_proposal **out);
+void lcr_proposal_destroy(lcr_proposal *p);
+int lcr_proposal_set_weights(lcr_proposal *p, const float *w, int64_t n);
+int lcr_proposal_get_weights(lcr_proposal *p, float *w, int64_t n);
+/* out: n_pairs * 6 */
+int lcr_proposal_forward(lcr_proposal *p, const float *src, const float *tgt, int32_t n_pairs,
+                         int32_t n_points, float *out);
+/* mean over pairs of the summed squared error against labels (n_pairs * 6) and
+ * its gradient with respect to the weights (grad: 174470 floats, nullable) */
+int lcr_proposal_loss_grad(lcr_proposal *p, const float *src, const float *tgt,
+                           const float *labels, int32_t n_pairs, int32_t n_points, float *loss,
+                           float *grad);
+/* uploads a training set to the device (kept resident for lcr_proposal_train) */
+int lcr_proposal_dataset(lcr_proposal *p, const float *src, const float *tgt, const float *labels,
+                         int32_t n_pairs, int32_t n_points);
+/* `steps` Adam steps on random batches of the resident set; losses: per step
+ * (nullable) */
+int lcr_proposal_train(lcr_proposal *p, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                       float *losses);
+
 /* ---- profiling: per-kernel GPU time measured with HIP events on the stream the
  * kernel runs on.  names: "icp_iteration", "fitness", "grid_build", "normals". */
 int lcr_prof_enable(int32_t on);

@@ -39,6 +39,9 @@ EXPORTS = (
     "lcr_normals", "lcr_cycle_create", "lcr_cycle_run", "lcr_cycle_results",
     "lcr_cycle_instances", "lcr_cycle_destroy", "lcr_icp_batch_align", "lcr_prof_enable",
     "lcr_prof_reset", "lcr_prof_read", "lcr_last_error", "lcr_version",
+    "lcr_proposal_create", "lcr_proposal_destroy", "lcr_proposal_set_weights",
+    "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
+    "lcr_proposal_dataset", "lcr_proposal_train",
 )
 
 
@@ -124,6 +127,15 @@ def _load():
     L.lcr_prof_enable.argtypes = [C.c_int32]
     L.lcr_prof_reset.argtypes = []
     L.lcr_prof_read.argtypes = [C.c_char_p, C.POINTER(C.c_int64), dp]
+    L.lcr_proposal_create.argtypes = [C.POINTER(vp)]
+    L.lcr_proposal_destroy.argtypes = [vp]
+    L.lcr_proposal_destroy.restype = None
+    L.lcr_proposal_set_weights.argtypes = [vp, fp, C.c_int64]
+    L.lcr_proposal_get_weights.argtypes = [vp, fp, C.c_int64]
+    L.lcr_proposal_forward.argtypes = [vp, fp, fp, C.c_int32, C.c_int32, fp]
+    L.lcr_proposal_loss_grad.argtypes = [vp, fp, fp, fp, C.c_int32, C.c_int32, fp, fp]
+    L.lcr_proposal_dataset.argtypes = [vp, fp, fp, fp, C.c_int32, C.c_int32]
+    L.lcr_proposal_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
     L.lcr_last_error.restype = C.c_char_p
     L.lcr_version.restype = C.c_char_p
     return L
@@ -380,6 +392,68 @@ class Cycle:
         ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
         _check(lib.lcr_cycle_instances(self._h, _dp(T), ip(it), ip(st), _dp(fit)))
         return dict(T=T, iterations=it, state=st, fitness=fit)
+
+
+ENC_NPARAM = 174470
+
+
+class Proposal:
+    """Learned initial-pose proposal: Siamese point encoder (3->64->128->256,
+    max-pool) + head (512->256->6 = axis-angle, translation), BF16 MFMA on the GPU.
+    Inputs are pairs of normalised clouds (see `normalise_pair`)."""
+
+    def __init__(self, weights=None):
+        h = C.c_void_p()
+        _check(lib.lcr_proposal_create(C.byref(h)))
+        self._h = h
+        if weights is not None:
+            self.set_weights(weights)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.lcr_proposal_destroy(h)
+            self._h = None
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(np.asarray(w, np.float32).reshape(-1))
+        _check(lib.lcr_proposal_set_weights(self._h, _fp(w), len(w)))
+
+    def get_weights(self):
+        w = np.empty(ENC_NPARAM, np.float32)
+        _check(lib.lcr_proposal_get_weights(self._h, _fp(w), len(w)))
+        return w
+
+    def forward(self, src, tgt):
+        s = np.ascontiguousarray(np.asarray(src, np.float32))
+        t = np.ascontiguousarray(np.asarray(tgt, np.float32))
+        B, n = s.shape[0], s.shape[1]
+        out = np.empty((B, 6), np.float32)
+        _check(lib.lcr_proposal_forward(self._h, _fp(s), _fp(t), B, n, _fp(out)))
+        return out
+
+    def loss_grad(self, src, tgt, labels):
+        s = np.ascontiguousarray(np.asarray(src, np.float32))
+        t = np.ascontiguousarray(np.asarray(tgt, np.float32))
+        lab = np.ascontiguousarray(np.asarray(labels, np.float32))
+        B, n = s.shape[0], s.shape[1]
+        loss = C.c_float()
+        g = np.empty(ENC_NPARAM, np.float32)
+        _check(lib.lcr_proposal_loss_grad(self._h, _fp(s), _fp(t), _fp(lab), B, n, C.byref(loss),
+                                          _fp(g)))
+        return loss.value, g
+
+    def dataset(self, src, tgt, labels):
+        s = np.ascontiguousarray(np.asarray(src, np.float32))
+        t = np.ascontiguousarray(np.asarray(tgt, np.float32))
+        lab = np.ascontiguousarray(np.asarray(labels, np.float32))
+        _check(lib.lcr_proposal_dataset(self._h, _fp(s), _fp(t), _fp(lab), s.shape[0], s.shape[1]))
+
+    def train(self, steps, batch, lr=1e-3, seed=0):
+        losses = np.empty(steps, np.float32)
+        _check(lib.lcr_proposal_train(self._h, int(steps), int(batch), float(lr), int(seed),
+                                      _fp(losses)))
+        return losses
 
 
 def prof_enable(on=True):

@@ -15,8 +15,16 @@
 using namespace lcr;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+int lcr::set_last_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+namespace {
+
 thread_local int g_device = 0;
 
 int fail(int code, const std::string& msg) {

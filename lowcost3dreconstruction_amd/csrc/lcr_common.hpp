@@ -72,6 +72,21 @@ struct IcpConst {
   double mse_abs;                // 1e-12
 };
 
+// Flat FP32 weight layout of the pose-proposal network, row-major [in][out]:
+// W1 3x64, b1, W2 64x128, b2, W3 128x256, b3, Hw1 512x256, hb1, Hw2 256x6, hb2.
+constexpr int kEncW1 = 0;
+constexpr int kEncB1 = kEncW1 + 3 * 64;
+constexpr int kEncW2 = kEncB1 + 64;
+constexpr int kEncB2 = kEncW2 + 64 * 128;
+constexpr int kEncW3 = kEncB2 + 128;
+constexpr int kEncB3 = kEncW3 + 128 * 256;
+constexpr int kEncHw1 = kEncB3 + 256;
+constexpr int kEncHb1 = kEncHw1 + 512 * 256;
+constexpr int kEncHw2 = kEncHb1 + 256;
+constexpr int kEncHb2 = kEncHw2 + 256 * 6;
+constexpr int kEncNParam = kEncHb2 + 6;
+static_assert(kEncNParam == 174470, "parameter count");
+
 // FP64 partial sums per tile.
 //   point-to-point: [0] n, [1..3] sum s, [4..6] sum t, [7..15] sum s_a t_b (a-major),
 //                   [16] sum d2

@@ -63,4 +63,37 @@ void launch_normals(const GridDesc* grids, const int* cell_start, const float4* 
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
                     int flip_all, float4* nrm_sorted, hipStream_t s);
 
+// ---- K9 pose-proposal network (encoder.hip)
+struct EncBuffers {
+  const float4* X;       // 2 * npairs clouds of npts normalised points (src, tgt, ...)
+  float* W;              // FP32 master weights (kEncNParam)
+  __bf16* W2t;           // bf16 W2 transposed [128][64]
+  __bf16* W3t;           // bf16 W3 transposed [256][128]
+  float* part_max;       // [cloud][chunk][256]
+  int* part_arg;
+  float* F;              // pooled features [cloud][256]
+  int* ARG;              // argmax point per feature
+  float* Z;              // head hidden [pair][256]
+  float* OUT;            // [pair][6]
+  float* DOUT;           // [pair][6]
+  float* loss_part;      // [pair][6]
+  float* DZ;             // [pair][256]
+  float* DG;             // [pair][512]
+  float* XA;             // rows [cloud*256 + c]: x (3), h1 (64), da1 (64), h2 (128), da2 (128)
+  float* H1A;
+  float* DA1;
+  float* H2A;
+  float* DA2;
+  float* DF;
+  float* part_grad;
+  float* G;              // gradient (kEncNParam)
+};
+void launch_enc_bf16_weights(const float* W, __bf16* W2t, __bf16* W3t, hipStream_t s);
+void launch_enc_forward(const EncBuffers& e, int npairs, int npts, const float* labels,
+                        hipStream_t s);
+void launch_enc_backward(const EncBuffers& e, int npairs, int npts, hipStream_t s);
+void launch_adam(float* W, const float* G, float* M, float* V, int n, float lr, int step,
+                 hipStream_t s);
+int64_t enc_part_grad_floats(int npairs);
+
 }  // namespace lcr

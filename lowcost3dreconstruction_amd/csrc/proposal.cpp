@@ -1,0 +1,346 @@
+// proposal.cpp -- host side of the learned initial-pose proposal (K9): weights,
+// device-resident training set, forward / backward / Adam on the GPU, and the
+// per-pair inference the batched cycle uses as an extra pose hypothesis.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "../../include/lcr.h"
+#include "lcr_kernels.hpp"
+#include "runtime.hpp"
+
+namespace lcr {
+
+namespace {
+
+// ---- device helpers local to the proposal
+
+__global__ void k_gather_batch(const float4* __restrict__ ds, const float* __restrict__ ds_lab,
+                               const int* __restrict__ idx, int npts, int batch,
+                               float4* __restrict__ X, float* __restrict__ lab) {
+  const int64_t per_pair = 2 * (int64_t)npts;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int64_t)batch * per_pair) {
+    const int b = (int)(i / per_pair);
+    const int64_t o = i % per_pair;
+    X[i] = ds[(int64_t)idx[b] * per_pair + o];
+  }
+  if (i < (int64_t)batch * 6) lab[i] = ds_lab[(int64_t)idx[i / 6] * 6 + i % 6];
+}
+
+__global__ void k_sum_loss(const float* __restrict__ part, int n, float* __restrict__ out) {
+  __shared__ float s[256];
+  float a = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) a += part[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+}  // namespace
+
+struct ProposalImpl {
+  int device = 0;
+  hipStream_t s = nullptr;
+  int cap_pairs = 0, cap_pts = 0;
+  int step = 0;
+  DevBuf W, M, V, G, W2t, W3t;
+  DevBuf X, part_max, part_arg, F, ARG, Z, OUT, DOUT, loss_part, DZ, DG, XA, H1A, DA1, H2A, DA2,
+      DF, part_grad, lab, idx, losses;
+  DevBuf ds_X, ds_lab;
+  int ds_pairs = 0, ds_pts = 0;
+  HostBuf h_io;
+
+  ProposalImpl() {
+    LCR_HIP(hipGetDevice(&device));
+    LCR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    W.reserve(kEncNParam * sizeof(float));
+    M.reserve(kEncNParam * sizeof(float));
+    V.reserve(kEncNParam * sizeof(float));
+    G.reserve(kEncNParam * sizeof(float));
+    W2t.reserve(64 * 128 * sizeof(__bf16));
+    W3t.reserve(128 * 256 * sizeof(__bf16));
+    LCR_HIP(hipMemsetAsync(W.p, 0, kEncNParam * sizeof(float), s));
+    reset_optimizer();
+  }
+  ~ProposalImpl() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void bind() const { LCR_HIP(hipSetDevice(device)); }
+  void reset_optimizer() {
+    LCR_HIP(hipMemsetAsync(M.p, 0, kEncNParam * sizeof(float), s));
+    LCR_HIP(hipMemsetAsync(V.p, 0, kEncNParam * sizeof(float), s));
+    step = 0;
+  }
+  void reserve(int npairs, int npts) {
+    if (npairs <= cap_pairs && npts <= cap_pts) return;
+    cap_pairs = std::max(cap_pairs, npairs);
+    cap_pts = std::max(cap_pts, npts);
+    const int64_t P = cap_pairs, C = 2 * P, N = cap_pts;
+    const int64_t chunks = (N + 255) / 256;
+    X.reserve(C * N * sizeof(float4));
+    part_max.reserve(C * chunks * 256 * sizeof(float));
+    part_arg.reserve(C * chunks * 256 * sizeof(int));
+    F.reserve(C * 256 * sizeof(float));
+    ARG.reserve(C * 256 * sizeof(int));
+    Z.reserve(P * 256 * sizeof(float));
+    OUT.reserve(P * 6 * sizeof(float));
+    DOUT.reserve(P * 6 * sizeof(float));
+    loss_part.reserve(P * 6 * sizeof(float));
+    DZ.reserve(P * 256 * sizeof(float));
+    DG.reserve(P * 512 * sizeof(float));
+    XA.reserve(C * 256 * 3 * sizeof(float));
+    H1A.reserve(C * 256 * 64 * sizeof(float));
+    DA1.reserve(C * 256 * 64 * sizeof(float));
+    H2A.reserve(C * 256 * 128 * sizeof(float));
+    DA2.reserve(C * 256 * 128 * sizeof(float));
+    DF.reserve(C * 256 * sizeof(float));
+    part_grad.reserve(enc_part_grad_floats((int)P) * sizeof(float));
+    lab.reserve(P * 6 * sizeof(float));
+    idx.reserve(P * sizeof(int));
+  }
+  EncBuffers bufs() const {
+    EncBuffers e;
+    e.X = X.as<float4>();
+    e.W = W.as<float>();
+    e.W2t = W2t.as<__bf16>();
+    e.W3t = W3t.as<__bf16>();
+    e.part_max = part_max.as<float>();
+    e.part_arg = part_arg.as<int>();
+    e.F = F.as<float>();
+    e.ARG = ARG.as<int>();
+    e.Z = Z.as<float>();
+    e.OUT = OUT.as<float>();
+    e.DOUT = DOUT.as<float>();
+    e.loss_part = loss_part.as<float>();
+    e.DZ = DZ.as<float>();
+    e.DG = DG.as<float>();
+    e.XA = XA.as<float>();
+    e.H1A = H1A.as<float>();
+    e.DA1 = DA1.as<float>();
+    e.H2A = H2A.as<float>();
+    e.DA2 = DA2.as<float>();
+    e.DF = DF.as<float>();
+    e.part_grad = part_grad.as<float>();
+    e.G = G.as<float>();
+    return e;
+  }
+  // host clouds (npairs * npts * 3 floats each) -> X as (src0, tgt0, src1, tgt1, ...)
+  void upload_pairs(const float* src, const float* tgt, int npairs, int npts) {
+    reserve(npairs, npts);
+    const int64_t n = (int64_t)npairs * 2 * npts;
+    h_io.reserve((size_t)n * sizeof(float4));
+    float4* h = h_io.as<float4>();
+    for (int b = 0; b < npairs; ++b)
+      for (int side = 0; side < 2; ++side) {
+        const float* in = (side == 0 ? src : tgt) + (int64_t)b * npts * 3;
+        float4* o = h + ((int64_t)b * 2 + side) * npts;
+        for (int p = 0; p < npts; ++p) o[p] = make_float4(in[3 * p], in[3 * p + 1], in[3 * p + 2], 0.f);
+      }
+    LCR_HIP(hipMemcpyAsync(X.p, h, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s));
+    LCR_HIP(hipStreamSynchronize(s));
+  }
+};
+
+}  // namespace lcr
+
+using namespace lcr;
+
+struct lcr_proposal {
+  ProposalImpl impl;
+};
+
+static inline int lcr_set_error_(int code, const char* msg) { return set_last_error(code, msg); }
+
+#define PGUARD(body)                                   \
+  try {                                                \
+    body                                               \
+  } catch (const Error& e) {                           \
+    return lcr_set_error_(e.code, e.msg.c_str());      \
+  } catch (const std::bad_alloc&) {                    \
+    return lcr_set_error_(LCR_E_HIP, "out of memory"); \
+  }
+
+extern "C" {
+
+int lcr_proposal_create(lcr_proposal** out) {
+  PGUARD({
+    if (!out) return lcr_set_error_(LCR_E_ARG, "out is NULL");
+    *out = new lcr_proposal();
+    return LCR_OK;
+  })
+}
+
+void lcr_proposal_destroy(lcr_proposal* p) {
+  if (!p) return;
+  try {
+    p->impl.bind();
+  } catch (...) {
+  }
+  delete p;
+}
+
+int lcr_proposal_set_weights(lcr_proposal* p, const float* w, int64_t n) {
+  PGUARD({
+    if (!p || !w || n != kEncNParam) return lcr_set_error_(LCR_E_ARG, "expected 174470 weights");
+    ProposalImpl& P = p->impl;
+    P.bind();
+    LCR_HIP(hipMemcpyAsync(P.W.p, w, n * sizeof(float), hipMemcpyHostToDevice, P.s));
+    launch_enc_bf16_weights(P.W.as<float>(), P.W2t.as<__bf16>(), P.W3t.as<__bf16>(), P.s);
+    P.reset_optimizer();
+    LCR_HIP(hipStreamSynchronize(P.s));
+    return LCR_OK;
+  })
+}
+
+int lcr_proposal_get_weights(lcr_proposal* p, float* w, int64_t n) {
+  PGUARD({
+    if (!p || !w || n != kEncNParam) return lcr_set_error_(LCR_E_ARG, "expected 174470 weights");
+    ProposalImpl& P = p->impl;
+    P.bind();
+    LCR_HIP(hipMemcpyAsync(w, P.W.p, n * sizeof(float), hipMemcpyDeviceToHost, P.s));
+    LCR_HIP(hipStreamSynchronize(P.s));
+    return LCR_OK;
+  })
+}
+
+int lcr_proposal_forward(lcr_proposal* p, const float* src, const float* tgt, int32_t n_pairs,
+                         int32_t n_points, float* out) {
+  PGUARD({
+    if (!p || !src || !tgt || !out || n_pairs < 1 || n_points < 1)
+      return lcr_set_error_(LCR_E_ARG, "invalid arguments");
+    ProposalImpl& P = p->impl;
+    P.bind();
+    P.upload_pairs(src, tgt, n_pairs, n_points);
+    launch_enc_forward(P.bufs(), n_pairs, n_points, nullptr, P.s);
+    LCR_HIP(hipMemcpyAsync(out, P.OUT.p, (size_t)n_pairs * 6 * sizeof(float), hipMemcpyDeviceToHost,
+                           P.s));
+    LCR_HIP(hipStreamSynchronize(P.s));
+    return LCR_OK;
+  })
+}
+
+int lcr_proposal_loss_grad(lcr_proposal* p, const float* src, const float* tgt, const float* labels,
+                           int32_t n_pairs, int32_t n_points, float* loss, float* grad) {
+  PGUARD({
+    if (!p || !src || !tgt || !labels || n_pairs < 1 || n_points < 1)
+      return lcr_set_error_(LCR_E_ARG, "invalid arguments");
+    ProposalImpl& P = p->impl;
+    P.bind();
+    P.upload_pairs(src, tgt, n_pairs, n_points);
+    LCR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 6 * sizeof(float),
+                           hipMemcpyHostToDevice, P.s));
+    const EncBuffers e = P.bufs();
+    launch_enc_forward(e, n_pairs, n_points, P.lab.as<float>(), P.s);
+    launch_enc_backward(e, n_pairs, n_points, P.s);
+    std::vector<float> lp((size_t)n_pairs * 6);
+    LCR_HIP(hipMemcpyAsync(lp.data(), P.loss_part.p, lp.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, P.s));
+    if (grad)
+      LCR_HIP(hipMemcpyAsync(grad, P.G.p, kEncNParam * sizeof(float), hipMemcpyDeviceToHost, P.s));
+    LCR_HIP(hipStreamSynchronize(P.s));
+    if (loss) {
+      double s = 0.0;
+      for (float v : lp) s += v;
+      *loss = (float)s;
+    }
+    return LCR_OK;
+  })
+}
+
+int lcr_proposal_dataset(lcr_proposal* p, const float* src, const float* tgt, const float* labels,
+                         int32_t n_pairs, int32_t n_points) {
+  PGUARD({
+    if (!p || !src || !tgt || !labels || n_pairs < 1 || n_points < 1)
+      return lcr_set_error_(LCR_E_ARG, "invalid arguments");
+    ProposalImpl& P = p->impl;
+    P.bind();
+    const int64_t per = 2 * (int64_t)n_points;
+    P.ds_X.reserve((size_t)n_pairs * per * sizeof(float4));
+    P.ds_lab.reserve((size_t)n_pairs * 6 * sizeof(float));
+    // upload in slices through the pinned staging buffer
+    const int slice = std::max(1, (int)std::min<int64_t>(n_pairs, (256ll << 20) / (per * 16)));
+    std::vector<float4> tmp;
+    for (int b0 = 0; b0 < n_pairs; b0 += slice) {
+      const int nb = std::min(slice, n_pairs - b0);
+      P.h_io.reserve((size_t)nb * per * sizeof(float4));
+      float4* h = P.h_io.as<float4>();
+      for (int b = 0; b < nb; ++b)
+        for (int side = 0; side < 2; ++side) {
+          const float* in = (side == 0 ? src : tgt) + (int64_t)(b0 + b) * n_points * 3;
+          float4* o = h + ((int64_t)b * 2 + side) * n_points;
+          for (int q = 0; q < n_points; ++q)
+            o[q] = make_float4(in[3 * q], in[3 * q + 1], in[3 * q + 2], 0.f);
+        }
+      LCR_HIP(hipMemcpyAsync(P.ds_X.as<float4>() + (int64_t)b0 * per, h, (size_t)nb * per * 16,
+                             hipMemcpyHostToDevice, P.s));
+      LCR_HIP(hipStreamSynchronize(P.s));
+    }
+    LCR_HIP(hipMemcpyAsync(P.ds_lab.p, labels, (size_t)n_pairs * 6 * sizeof(float),
+                           hipMemcpyHostToDevice, P.s));
+    LCR_HIP(hipStreamSynchronize(P.s));
+    P.ds_pairs = n_pairs;
+    P.ds_pts = n_points;
+    return LCR_OK;
+  })
+}
+
+int lcr_proposal_train(lcr_proposal* p, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                       float* losses) {
+  PGUARD({
+    if (!p || steps < 1 || batch < 1) return lcr_set_error_(LCR_E_ARG, "invalid arguments");
+    ProposalImpl& P = p->impl;
+    if (P.ds_pairs < 1) return lcr_set_error_(LCR_E_STATE, "no training set");
+    P.bind();
+    batch = std::min(batch, P.ds_pairs);
+    P.reserve(batch, P.ds_pts);
+    P.losses.reserve((size_t)steps * sizeof(float));
+    std::mt19937_64 rng(seed);
+    std::vector<int> all((size_t)steps * batch);
+    std::vector<int> perm(P.ds_pairs);
+    for (int i = 0; i < P.ds_pairs; ++i) perm[i] = i;
+    size_t pos = perm.size();
+    for (auto& v : all) {
+      if (pos == perm.size()) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        pos = 0;
+      }
+      v = perm[pos++];
+    }
+    DevBuf d_all;
+    d_all.reserve(all.size() * sizeof(int));
+    LCR_HIP(hipMemcpyAsync(d_all.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice,
+                           P.s));
+    const EncBuffers e = P.bufs();
+    const int64_t nx = (int64_t)batch * 2 * P.ds_pts;
+    for (int st = 0; st < steps; ++st) {
+      hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, P.s,
+                         P.ds_X.as<float4>(), P.ds_lab.as<float>(), d_all.as<int>() + (int64_t)st * batch,
+                         P.ds_pts, batch, P.X.as<float4>(), P.lab.as<float>());
+      launch_enc_forward(e, batch, P.ds_pts, P.lab.as<float>(), P.s);
+      hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, P.s, P.loss_part.as<float>(), batch * 6,
+                         P.losses.as<float>() + st);
+      launch_enc_backward(e, batch, P.ds_pts, P.s);
+      P.step += 1;
+      launch_adam(P.W.as<float>(), P.G.as<float>(), P.M.as<float>(), P.V.as<float>(), kEncNParam, lr,
+                  P.step, P.s);
+      launch_enc_bf16_weights(P.W.as<float>(), P.W2t.as<__bf16>(), P.W3t.as<__bf16>(), P.s);
+    }
+    if (losses)
+      LCR_HIP(hipMemcpyAsync(losses, P.losses.p, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost,
+                             P.s));
+    LCR_HIP(hipStreamSynchronize(P.s));
+    return LCR_OK;
+  })
+}
+
+}  // extern "C"

@@ -168,4 +168,7 @@ struct Engine {
 
 int device_cu_count();
 
+// Records the message returned by lcr_last_error() on this thread; returns code.
+int set_last_error(int code, const std::string& msg);
+
 }  // namespace lcr
