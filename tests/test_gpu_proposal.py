@@ -1,0 +1,96 @@
+"""The learned pose proposal's BF16 MFMA kernels against the FP64 oracle
+(oracle/lcr_encoder_oracle.c, same bf16 rounding points).
+
+Bars: forward outputs within 1e-2 relative; gradients of the head within 1e-3 and
+of the per-point layers within 3e-2 relative (Frobenius, per weight block) -- the
+device sums in FP32 and rounds h1/h2 to bf16 from FP32 values, so a few
+activations differ by one bf16 ulp and a near-tied max-pool channel can pick a
+different point."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+
+def init_weights(rng):
+    w = np.empty(oracle.ENC_NPARAM)
+    off = 0
+    for name, shape in oracle.ENC_LAYOUT:
+        n = int(np.prod(shape))
+        if name.startswith(("W", "Hw")):
+            w[off:off + n] = rng.normal(scale=np.sqrt(2.0 / shape[0]), size=n)
+        else:
+            w[off:off + n] = rng.normal(scale=0.05, size=n)
+        off += n
+    return w.astype(np.float32)
+
+
+def blocks(g):
+    off, out = 0, {}
+    for name, shape in oracle.ENC_LAYOUT:
+        n = int(np.prod(shape))
+        out[name] = g[off:off + n]
+        off += n
+    return out
+
+
+@pytest.mark.parametrize("npts", [64, 300, 1024])
+def test_forward_matches_oracle(lcr, npts):
+    rng = np.random.default_rng(npts)
+    w = init_weights(rng)
+    B = 3
+    src = rng.normal(size=(B, npts, 3)).astype(np.float32)
+    tgt = rng.normal(size=(B, npts, 3)).astype(np.float32)
+    print_case(test="proposal_forward", npts=npts)
+    p = lcr.Proposal(w)
+    out = p.forward(src, tgt)
+    for b in range(B):
+        ref, _, _ = oracle.enc_forward(w.astype(np.float64), src[b], tgt[b], quantize=True)
+        assert np.max(np.abs(out[b] - ref)) <= 1e-2 * max(1.0, np.max(np.abs(ref))), (out[b], ref)
+
+
+def test_gradient_matches_oracle(lcr):
+    rng = np.random.default_rng(7)
+    w = init_weights(rng)
+    B, n = 4, 128
+    src = rng.normal(size=(B, n, 3)).astype(np.float32)
+    tgt = rng.normal(size=(B, n, 3)).astype(np.float32)
+    lab = rng.normal(size=(B, 6)).astype(np.float32)
+    print_case(test="proposal_grad")
+    p = lcr.Proposal(w)
+    loss, g = p.loss_grad(src, tgt, lab)
+    rloss, rg = oracle.enc_loss_grad(w.astype(np.float64), src, tgt, lab, quantize=True)
+    assert abs(loss - rloss) <= 1e-2 * max(1.0, rloss)
+    gb, rb = blocks(g.astype(np.float64)), blocks(rg)
+    for name in rb:
+        den = max(np.linalg.norm(rb[name]), 1e-12)
+        err = np.linalg.norm(gb[name] - rb[name]) / den
+        tol = 1e-3 if name in ("Hw1", "hb1", "Hw2", "hb2") else 3e-2
+        assert err <= tol, (name, err)
+
+
+def test_training_reduces_loss(lcr):
+    """A small regression task (recover a random rotation's axis-angle from a cloud
+    and its rotated copy): the device's Adam steps lower the loss."""
+    rng = np.random.default_rng(3)
+    B, n = 256, 256
+    base = rng.normal(size=(B, n, 3)).astype(np.float32)
+    ang = rng.uniform(0, 0.6, size=B)
+    ax = rng.normal(size=(B, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    om = ax * ang[:, None]
+    tgt = np.empty_like(base)
+    for b in range(B):
+        K = np.array([[0, -ax[b, 2], ax[b, 1]], [ax[b, 2], 0, -ax[b, 0]], [-ax[b, 1], ax[b, 0], 0]])
+        R = np.eye(3) + np.sin(ang[b]) * K + (1 - np.cos(ang[b])) * K @ K
+        tgt[b] = base[b] @ R.T
+    labels = np.concatenate([-om, np.zeros((B, 3))], axis=1).astype(np.float32)
+    print_case(test="proposal_train")
+    p = lcr.Proposal(init_weights(rng) * 0.5)
+    p.dataset(base, tgt, labels)
+    losses = p.train(steps=300, batch=64, lr=1e-3, seed=1)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.7 * losses[:20].mean(), (losses[:20].mean(), losses[-20:].mean())
