@@ -180,6 +180,20 @@ int lcr_proposal_loss_grad(lcr_proposal *p, const float *src, const float *tgt,
 /* uploads a training set to the device (kept resident for lcr_proposal_train) */
 int lcr_proposal_dataset(lcr_proposal *p, const float *src, const float *tgt, const float *labels,
                          int32_t n_pairs, int32_t n_points);
+/* Generates a training set of n_pairs on the device and keeps it resident.  Per
+ * pair: shapes = 3 ellipsoids x (centre 3, row-major rotation 9, semi-axes 3) = 45
+ * floats, weights = the 3 ellipsoids' surface-area shares, perturb = the motion
+ * applied to the source (row-major 4x4); both clouds are independent area-uniform
+ * samples with N(0, noise^2) per coordinate, normalised by the target's centroid
+ * and RMS radius; labels = the inverse motion in that frame (omega, tau).
+ * raw (nullable): the un-normalised clouds, n_pairs * 2 * n_points * 3 floats
+ * (source, target per pair); frame (nullable): n_pairs * 4 (centroid, radius). */
+int lcr_proposal_synthesize(lcr_proposal *p, const float *shapes, const float *weights,
+                            const double *perturb, int32_t n_pairs, int32_t n_points, float noise,
+                            uint64_t seed, float *raw, float *frame);
+/* forward on resident pairs [first, first + count): out (and labels, nullable)
+ * count * 6 floats */
+int lcr_proposal_predict(lcr_proposal *p, int32_t first, int32_t count, float *out, float *labels);
 /* `steps` Adam steps on random batches of the resident set; losses: per step
  * (nullable) */
 int lcr_proposal_train(lcr_proposal *p, int32_t steps, int32_t batch, float lr, uint64_t seed,

@@ -41,7 +41,8 @@ EXPORTS = (
     "lcr_prof_reset", "lcr_prof_read", "lcr_last_error", "lcr_version",
     "lcr_proposal_create", "lcr_proposal_destroy", "lcr_proposal_set_weights",
     "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
-    "lcr_proposal_dataset", "lcr_proposal_train",
+    "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
+    "lcr_proposal_predict",
 )
 
 
@@ -136,6 +137,9 @@ def _load():
     L.lcr_proposal_loss_grad.argtypes = [vp, fp, fp, fp, C.c_int32, C.c_int32, fp, fp]
     L.lcr_proposal_dataset.argtypes = [vp, fp, fp, fp, C.c_int32, C.c_int32]
     L.lcr_proposal_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
+    L.lcr_proposal_synthesize.argtypes = [vp, fp, fp, dp, C.c_int32, C.c_int32, C.c_float,
+                                          C.c_uint64, fp, fp]
+    L.lcr_proposal_predict.argtypes = [vp, C.c_int32, C.c_int32, fp, fp]
     L.lcr_last_error.restype = C.c_char_p
     L.lcr_version.restype = C.c_char_p
     return L
@@ -448,6 +452,25 @@ class Proposal:
         t = np.ascontiguousarray(np.asarray(tgt, np.float32))
         lab = np.ascontiguousarray(np.asarray(labels, np.float32))
         _check(lib.lcr_proposal_dataset(self._h, _fp(s), _fp(t), _fp(lab), s.shape[0], s.shape[1]))
+
+    def synthesize(self, shapes, weights, perturb, n_points, noise=0.001, seed=0, raw=False):
+        """Generate and keep resident a training set (see lcr.h).  Returns (raw clouds
+        (P, 2, n, 3) or None, frames (P, 4): target centroid and RMS radius)."""
+        sh = np.ascontiguousarray(np.asarray(shapes, np.float32).reshape(-1, 45))
+        wt = np.ascontiguousarray(np.asarray(weights, np.float32).reshape(-1, 3))
+        P = np.ascontiguousarray(np.asarray(perturb, np.float64).reshape(-1, 16))
+        n = len(sh)
+        out_raw = np.empty((n, 2, n_points, 3), np.float32) if raw else None
+        frame = np.empty((n, 4), np.float32)
+        _check(lib.lcr_proposal_synthesize(self._h, _fp(sh), _fp(wt), _dp(P), n, int(n_points),
+                                           float(noise), int(seed), _fp(out_raw), _fp(frame)))
+        return out_raw, frame
+
+    def predict(self, first, count):
+        out = np.empty((count, 6), np.float32)
+        lab = np.empty((count, 6), np.float32)
+        _check(lib.lcr_proposal_predict(self._h, int(first), int(count), _fp(out), _fp(lab)))
+        return out, lab
 
     def train(self, steps, batch, lr=1e-3, seed=0):
         losses = np.empty(steps, np.float32)

@@ -95,5 +95,9 @@ void launch_enc_backward(const EncBuffers& e, int npairs, int npts, hipStream_t 
 void launch_adam(float* W, const float* G, float* M, float* V, int n, float lr, int step,
                  hipStream_t s);
 int64_t enc_part_grad_floats(int npairs);
+// training pairs generated on the device (synth.hip)
+void launch_synth_pairs(const float* shapes, const float* weights, const double* P,
+                        const double* Pinv, int npairs, int npts, float noise, uint64_t seed,
+                        float4* X, float* labels, float* raw, float* frame, hipStream_t s);
 
 }  // namespace lcr

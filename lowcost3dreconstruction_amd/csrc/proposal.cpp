@@ -161,23 +161,25 @@ struct lcr_proposal {
 
 static inline int lcr_set_error_(int code, const char* msg) { return set_last_error(code, msg); }
 
-#define PGUARD(body)                                   \
-  try {                                                \
-    body                                               \
-  } catch (const Error& e) {                           \
-    return lcr_set_error_(e.code, e.msg.c_str());      \
-  } catch (const std::bad_alloc&) {                    \
-    return lcr_set_error_(LCR_E_HIP, "out of memory"); \
+template <class F>
+static int pguard(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    return lcr_set_error_(e.code, e.msg.c_str());
+  } catch (const std::bad_alloc&) {
+    return lcr_set_error_(LCR_E_HIP, "out of memory");
   }
+}
 
 extern "C" {
 
 int lcr_proposal_create(lcr_proposal** out) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!out) return lcr_set_error_(LCR_E_ARG, "out is NULL");
     *out = new lcr_proposal();
     return LCR_OK;
-  })
+  });
 }
 
 void lcr_proposal_destroy(lcr_proposal* p) {
@@ -190,7 +192,7 @@ void lcr_proposal_destroy(lcr_proposal* p) {
 }
 
 int lcr_proposal_set_weights(lcr_proposal* p, const float* w, int64_t n) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!p || !w || n != kEncNParam) return lcr_set_error_(LCR_E_ARG, "expected 174470 weights");
     ProposalImpl& P = p->impl;
     P.bind();
@@ -199,23 +201,23 @@ int lcr_proposal_set_weights(lcr_proposal* p, const float* w, int64_t n) {
     P.reset_optimizer();
     LCR_HIP(hipStreamSynchronize(P.s));
     return LCR_OK;
-  })
+  });
 }
 
 int lcr_proposal_get_weights(lcr_proposal* p, float* w, int64_t n) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!p || !w || n != kEncNParam) return lcr_set_error_(LCR_E_ARG, "expected 174470 weights");
     ProposalImpl& P = p->impl;
     P.bind();
     LCR_HIP(hipMemcpyAsync(w, P.W.p, n * sizeof(float), hipMemcpyDeviceToHost, P.s));
     LCR_HIP(hipStreamSynchronize(P.s));
     return LCR_OK;
-  })
+  });
 }
 
 int lcr_proposal_forward(lcr_proposal* p, const float* src, const float* tgt, int32_t n_pairs,
                          int32_t n_points, float* out) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!p || !src || !tgt || !out || n_pairs < 1 || n_points < 1)
       return lcr_set_error_(LCR_E_ARG, "invalid arguments");
     ProposalImpl& P = p->impl;
@@ -226,12 +228,12 @@ int lcr_proposal_forward(lcr_proposal* p, const float* src, const float* tgt, in
                            P.s));
     LCR_HIP(hipStreamSynchronize(P.s));
     return LCR_OK;
-  })
+  });
 }
 
 int lcr_proposal_loss_grad(lcr_proposal* p, const float* src, const float* tgt, const float* labels,
                            int32_t n_pairs, int32_t n_points, float* loss, float* grad) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!p || !src || !tgt || !labels || n_pairs < 1 || n_points < 1)
       return lcr_set_error_(LCR_E_ARG, "invalid arguments");
     ProposalImpl& P = p->impl;
@@ -254,12 +256,12 @@ int lcr_proposal_loss_grad(lcr_proposal* p, const float* src, const float* tgt, 
       *loss = (float)s;
     }
     return LCR_OK;
-  })
+  });
 }
 
 int lcr_proposal_dataset(lcr_proposal* p, const float* src, const float* tgt, const float* labels,
                          int32_t n_pairs, int32_t n_points) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!p || !src || !tgt || !labels || n_pairs < 1 || n_points < 1)
       return lcr_set_error_(LCR_E_ARG, "invalid arguments");
     ProposalImpl& P = p->impl;
@@ -291,12 +293,87 @@ int lcr_proposal_dataset(lcr_proposal* p, const float* src, const float* tgt, co
     P.ds_pairs = n_pairs;
     P.ds_pts = n_points;
     return LCR_OK;
-  })
+  });
+}
+
+int lcr_proposal_synthesize(lcr_proposal* p, const float* shapes, const float* weights,
+                            const double* perturb, int32_t n_pairs, int32_t n_points, float noise,
+                            uint64_t seed, float* raw, float* frame) {
+  return pguard([&]() -> int {
+    if (!p || !shapes || !weights || !perturb || n_pairs < 1 || n_points < 1)
+      return lcr_set_error_(LCR_E_ARG, "invalid arguments");
+    ProposalImpl& P = p->impl;
+    P.bind();
+    const int64_t per = 2 * (int64_t)n_points;
+    P.ds_X.reserve((size_t)n_pairs * per * sizeof(float4));
+    P.ds_lab.reserve((size_t)n_pairs * 6 * sizeof(float));
+    std::vector<double> Pinv((size_t)n_pairs * 16);
+    for (int b = 0; b < n_pairs; ++b) {
+      const double* M = perturb + 16 * (int64_t)b;
+      double* I = &Pinv[16 * (size_t)b];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) I[4 * r + c] = M[4 * c + r];  // R^T
+      for (int r = 0; r < 3; ++r)
+        I[4 * r + 3] = -(I[4 * r] * M[3] + I[4 * r + 1] * M[7] + I[4 * r + 2] * M[11]);
+      I[12] = I[13] = I[14] = 0.0;
+      I[15] = 1.0;
+    }
+    DevBuf d_shapes, d_w, d_P, d_Pi, d_raw, d_frame;
+    d_shapes.reserve((size_t)n_pairs * 45 * sizeof(float));
+    d_w.reserve((size_t)n_pairs * 3 * sizeof(float));
+    d_P.reserve((size_t)n_pairs * 16 * sizeof(double));
+    d_Pi.reserve((size_t)n_pairs * 16 * sizeof(double));
+    LCR_HIP(hipMemcpyAsync(d_shapes.p, shapes, (size_t)n_pairs * 45 * 4, hipMemcpyHostToDevice, P.s));
+    LCR_HIP(hipMemcpyAsync(d_w.p, weights, (size_t)n_pairs * 3 * 4, hipMemcpyHostToDevice, P.s));
+    LCR_HIP(hipMemcpyAsync(d_P.p, perturb, (size_t)n_pairs * 16 * 8, hipMemcpyHostToDevice, P.s));
+    LCR_HIP(hipMemcpyAsync(d_Pi.p, Pinv.data(), (size_t)n_pairs * 16 * 8, hipMemcpyHostToDevice, P.s));
+    if (raw) d_raw.reserve((size_t)n_pairs * per * 3 * sizeof(float));
+    if (frame) d_frame.reserve((size_t)n_pairs * 4 * sizeof(float));
+    launch_synth_pairs(d_shapes.as<float>(), d_w.as<float>(), d_P.as<double>(), d_Pi.as<double>(),
+                       n_pairs, n_points, noise, seed, P.ds_X.as<float4>(), P.ds_lab.as<float>(),
+                       raw ? d_raw.as<float>() : nullptr, frame ? d_frame.as<float>() : nullptr, P.s);
+    if (raw)
+      LCR_HIP(hipMemcpyAsync(raw, d_raw.p, (size_t)n_pairs * per * 3 * 4, hipMemcpyDeviceToHost, P.s));
+    if (frame)
+      LCR_HIP(hipMemcpyAsync(frame, d_frame.p, (size_t)n_pairs * 4 * 4, hipMemcpyDeviceToHost, P.s));
+    LCR_HIP(hipStreamSynchronize(P.s));
+    P.ds_pairs = n_pairs;
+    P.ds_pts = n_points;
+    return LCR_OK;
+  });
+}
+
+int lcr_proposal_predict(lcr_proposal* p, int32_t first, int32_t count, float* out, float* labels) {
+  return pguard([&]() -> int {
+    if (!p || !out || count < 1 || first < 0) return lcr_set_error_(LCR_E_ARG, "invalid arguments");
+    ProposalImpl& P = p->impl;
+    if (first + count > P.ds_pairs) return lcr_set_error_(LCR_E_ARG, "range beyond the resident set");
+    P.bind();
+    const int chunk = 512;
+    for (int b0 = 0; b0 < count; b0 += chunk) {
+      const int nb = std::min(chunk, count - b0);
+      P.reserve(nb, P.ds_pts);
+      std::vector<int> ids(nb);
+      for (int i = 0; i < nb; ++i) ids[i] = first + b0 + i;
+      LCR_HIP(hipMemcpyAsync(P.idx.p, ids.data(), nb * sizeof(int), hipMemcpyHostToDevice, P.s));
+      const int64_t nx = (int64_t)nb * 2 * P.ds_pts;
+      hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, P.s,
+                         P.ds_X.as<float4>(), P.ds_lab.as<float>(), P.idx.as<int>(), P.ds_pts, nb,
+                         P.X.as<float4>(), P.lab.as<float>());
+      launch_enc_forward(P.bufs(), nb, P.ds_pts, nullptr, P.s);
+      LCR_HIP(hipMemcpyAsync(out + (int64_t)b0 * 6, P.OUT.p, (size_t)nb * 6 * 4, hipMemcpyDeviceToHost, P.s));
+      if (labels)
+        LCR_HIP(hipMemcpyAsync(labels + (int64_t)b0 * 6, P.lab.p, (size_t)nb * 6 * 4,
+                               hipMemcpyDeviceToHost, P.s));
+      LCR_HIP(hipStreamSynchronize(P.s));
+    }
+    return LCR_OK;
+  });
 }
 
 int lcr_proposal_train(lcr_proposal* p, int32_t steps, int32_t batch, float lr, uint64_t seed,
                        float* losses) {
-  PGUARD({
+  return pguard([&]() -> int {
     if (!p || steps < 1 || batch < 1) return lcr_set_error_(LCR_E_ARG, "invalid arguments");
     ProposalImpl& P = p->impl;
     if (P.ds_pairs < 1) return lcr_set_error_(LCR_E_STATE, "no training set");
@@ -340,7 +417,7 @@ int lcr_proposal_train(lcr_proposal* p, int32_t steps, int32_t batch, float lr, 
                              P.s));
     LCR_HIP(hipStreamSynchronize(P.s));
     return LCR_OK;
-  })
+  });
 }
 
 }  // extern "C"
