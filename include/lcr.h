@@ -116,6 +116,8 @@ int lcr_normals(const float *xyz, int64_t n, int64_t stride_bytes, int32_t k, co
 /* ---- batched alignment: every adjacent pair of a capture cycle (and optional
  * pose hypotheses per pair) in one launch per ICP iteration.  Replaces the
  * per-pair process loop of scripts/alignment.sh:106-113. */
+struct lcr_proposal;
+
 typedef struct {
   int32_t n_clouds;
   const float *const *xyz;      /* per cloud: packed x,y,z floats */
@@ -130,6 +132,11 @@ typedef struct {
   lcr_icp_params params;
   int32_t normals_k;            /* k of the on-device normal estimation (default 30) */
   const float *viewpoints;      /* per cloud 3 floats, or NULL = origin */
+  struct lcr_proposal *proposal; /* nullable: adds one learned hypothesis per pair,
+                                   seeded from the pair's first guess; the best
+                                   hypothesis by fitness wins, so a worse learned
+                                   guess falls back to the plain one */
+  int32_t proposal_points;      /* points per cloud fed to the network (default 8192) */
 } lcr_cycle_desc;
 
 typedef struct {
@@ -138,7 +145,8 @@ typedef struct {
   int32_t *converged;   /* n_pairs */
   int32_t *state;       /* n_pairs */
   double *fitness;      /* n_pairs: getFitnessScore() of the best hypothesis */
-  int32_t *best_hyp;    /* n_pairs: index of the chosen hypothesis (nullable) */
+  int32_t *best_hyp;    /* n_pairs: index of the chosen hypothesis (nullable); n_hyp
+                           means the learned one */
 } lcr_cycle_result;
 
 typedef struct lcr_cycle lcr_cycle;
@@ -149,7 +157,7 @@ int lcr_cycle_create(const lcr_cycle_desc *d, lcr_cycle **out);
 int lcr_cycle_run(lcr_cycle *c);
 int lcr_cycle_results(lcr_cycle *c, lcr_cycle_result *r);
 /* per-instance (pair x hypothesis) results before selection; arrays of
- * n_pairs * n_hyp entries (each pointer nullable) */
+ * n_pairs * (n_hyp + 1 if a proposal is set, else n_hyp) entries (each nullable) */
 int lcr_cycle_instances(lcr_cycle *c, double *T, int32_t *iterations, int32_t *state,
                         double *fitness);
 void lcr_cycle_destroy(lcr_cycle *c);

@@ -79,6 +79,8 @@ class CycleDesc(C.Structure):
         ("params", IcpParams),
         ("normals_k", C.c_int32),
         ("viewpoints", C.POINTER(C.c_float)),
+        ("proposal", C.c_void_p),
+        ("proposal_points", C.c_int32),
     ]
 
 
@@ -320,7 +322,8 @@ class Cycle:
 
     def __init__(self, clouds, pairs, normals=None, guesses=None, n_hyp=1, max_corr_dist=0.05,
                  max_iterations=60, transformation_epsilon=1e-9, euclidean_fitness_epsilon=1e-3,
-                 point_to_plane=True, normals_k=30, viewpoints=None, cell_factor=0.0, tile=0):
+                 point_to_plane=True, normals_k=30, viewpoints=None, cell_factor=0.0, tile=0,
+                 proposal=None, proposal_points=8192):
         self.clouds = [_f32(c) for c in clouds]
         self.normals = [_f32(n) for n in normals] if normals is not None else None
         pr = np.asarray(pairs, np.int32).reshape(-1, 2)
@@ -359,6 +362,10 @@ class Cycle:
         d.params = p
         d.normals_k = int(normals_k)
         d.viewpoints = _fp(self._vps) if self._vps is not None else None
+        self._proposal = proposal  # kept alive while the cycle uses it
+        d.proposal = proposal._h if proposal is not None else None
+        d.proposal_points = int(proposal_points)
+        self.n_total_hyp = self.n_hyp + (1 if proposal is not None else 0)
         self._desc = d
         h = C.c_void_p()
         _check(lib.lcr_cycle_create(C.byref(d), C.byref(h)))
@@ -388,7 +395,7 @@ class Cycle:
                     best_hyp=best)
 
     def instances(self):
-        n = self.n_pairs * self.n_hyp
+        n = self.n_pairs * self.n_total_hyp
         T = np.empty((n, 4, 4), np.float64)
         it = np.empty(n, np.int32)
         st = np.empty(n, np.int32)

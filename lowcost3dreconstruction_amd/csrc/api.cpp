@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -153,6 +154,8 @@ struct lcr_cycle {
   Engine eng;
   bool ran = false;
   bool need_normals = false;
+  int nhyp_total = 1;
+  DevBuf d_pair_src, d_pair_tgt;
   std::vector<PairState> states;
 };
 
@@ -486,12 +489,24 @@ int lcr_cycle_create(const lcr_cycle_desc* d, lcr_cycle** out) {
     c->cs.upload(c->stream.s, xyz, nrm, cnt, 3 * sizeof(float));
     c->src.assign(d->pair_src, d->pair_src + d->n_pairs);
     c->tgt.assign(d->pair_tgt, d->pair_tgt + d->n_pairs);
-    const int ninst = d->n_pairs * d->n_hyp;
-    c->guesses.resize((size_t)ninst * 16);
-    for (int i = 0; i < ninst; ++i)
-      for (int k = 0; k < 16; ++k)
-        c->guesses[16 * (size_t)i + k] = d->guesses ? d->guesses[16 * (size_t)i + k]
-                                                    : ((k % 5 == 0) ? 1.0 : 0.0);
+    if (d->proposal && d->proposal_points < 0) return fail(LCR_E_ARG, "proposal_points < 0");
+    const int H = d->n_hyp + (d->proposal ? 1 : 0);
+    c->nhyp_total = H;
+    const int ninst = d->n_pairs * H;
+    c->guesses.assign((size_t)ninst * 16, 0.0);
+    for (int p = 0; p < d->n_pairs; ++p)
+      for (int hy = 0; hy < H; ++hy)
+        for (int k = 0; k < 16; ++k) {
+          const bool given = d->guesses && hy < d->n_hyp;
+          c->guesses[16 * ((size_t)p * H + hy) + k] =
+              given ? d->guesses[16 * ((size_t)p * d->n_hyp + hy) + k] : ((k % 5 == 0) ? 1.0 : 0.0);
+        }
+    if (d->n_pairs > 0) {
+      c->d_pair_src.reserve((size_t)d->n_pairs * sizeof(int));
+      c->d_pair_tgt.reserve((size_t)d->n_pairs * sizeof(int));
+      LCR_HIP(hipMemcpy(c->d_pair_src.p, d->pair_src, d->n_pairs * sizeof(int), hipMemcpyHostToDevice));
+      LCR_HIP(hipMemcpy(c->d_pair_tgt.p, d->pair_tgt, d->n_pairs * sizeof(int), hipMemcpyHostToDevice));
+    }
     c->vps.assign((size_t)d->n_clouds * 3, 0.0f);
     if (d->viewpoints)
       for (int i = 0; i < 3 * d->n_clouds; ++i) c->vps[i] = d->viewpoints[i];
@@ -520,13 +535,14 @@ int lcr_cycle_run(lcr_cycle* c) {
     if (c->need_normals) c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
     if (d.params.point_to_plane && !c->cs.has_nrm)
       return fail(LCR_E_NORMALS, "point-to-plane ICP needs normals");
-    const int ninst = d.n_pairs * d.n_hyp;
+    const int H = c->nhyp_total;
+    const int ninst = d.n_pairs * H;
     std::vector<int> src(ninst), tgt(ninst);
     std::vector<double> r2(ninst, d.params.max_corr_dist * d.params.max_corr_dist);
     for (int p = 0; p < d.n_pairs; ++p)
-      for (int hy = 0; hy < d.n_hyp; ++hy) {
-        src[p * d.n_hyp + hy] = c->src[p];
-        tgt[p * d.n_hyp + hy] = c->tgt[p];
+      for (int hy = 0; hy < H; ++hy) {
+        src[p * H + hy] = c->src[p];
+        tgt[p * H + hy] = c->tgt[p];
       }
     c->eng.c = make_const(d.params);
     const int tile = d.params.tile ? d.params.tile : 256;
@@ -537,7 +553,19 @@ int lcr_cycle_run(lcr_cycle* c) {
     }
     if (!same) c->eng.setup(c->cs, src, tgt, r2, tile);
     if (!spans.empty()) prof.collect(spans, false);
-    c->eng.run(c->stream.s, c->cs, c->guesses.data(), true);
+    std::function<void()> pre;
+    if (d.proposal && d.n_pairs > 0) {
+      ProposalImpl* P = proposal_impl(d.proposal);
+      const int npts = d.proposal_points > 0 ? d.proposal_points : 8192;
+      pre = [&, P, npts]() {
+        const int sp = prof.begin("proposal", c->stream.s, &spans);
+        proposal_guesses(*P, c->stream.s, c->cs, c->d_pair_src.as<int>(), c->d_pair_tgt.as<int>(),
+                         d.n_pairs, npts, c->eng.d_guess.as<double>(), H, d.n_hyp);
+        prof.end(sp, c->stream.s, &spans);
+      };
+    }
+    c->eng.run(c->stream.s, c->cs, c->guesses.data(), true, pre);
+    if (!spans.empty()) prof.collect(spans, false);
     c->states.assign(c->eng.states(), c->eng.states() + ninst);
     c->ran = true;
     return LCR_OK;
@@ -561,7 +589,7 @@ int lcr_cycle_instances(lcr_cycle* c, double* T, int32_t* iterations, int32_t* s
 int lcr_cycle_results(lcr_cycle* c, lcr_cycle_result* r) {
   if (!c || !r) return fail(LCR_E_ARG, "NULL argument");
   if (!c->ran) return fail(LCR_E_STATE, "lcr_cycle_run has not run");
-  const int H = c->desc.n_hyp;
+  const int H = c->nhyp_total;
   for (int p = 0; p < c->desc.n_pairs; ++p) {
     int best = 0;
     for (int h = 1; h < H; ++h)

@@ -46,6 +46,125 @@ __global__ void k_sum_loss(const float* __restrict__ part, int n, float* __restr
   if (threadIdx.x == 0) *out = s[0];
 }
 
+// Inputs of the network for every pair of a cycle: n_out points subsampled evenly
+// from each raw cloud, the source moved by the pair's seed T0, both normalised by
+// the target sample's centroid and RMS radius.  One block per pair.
+__global__ void __launch_bounds__(256)
+k_prop_prepare(const float4* __restrict__ raw, const int64_t* __restrict__ cloud_off,
+               const int* __restrict__ src, const int* __restrict__ tgt,
+               const double* __restrict__ guesses, int guess_stride, int n_out,
+               float4* __restrict__ X, float* __restrict__ frame) {
+  __shared__ double red[256][4];
+  __shared__ double cs[4];
+  const int pr = blockIdx.x, t = threadIdx.x;
+  const int64_t s0 = cloud_off[src[pr]], ns = cloud_off[src[pr] + 1] - s0;
+  const int64_t t0 = cloud_off[tgt[pr]], nt = cloud_off[tgt[pr] + 1] - t0;
+  float4* Xs = X + (int64_t)pr * 2 * n_out;
+  float4* Xt = Xs + n_out;
+  double a[4] = {0, 0, 0, 0};
+  for (int i = t; i < n_out; i += 256) {
+    const float4 q = raw[t0 + (int64_t)i * nt / n_out];
+    if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) {
+      a[0] += q.x;
+      a[1] += q.y;
+      a[2] += q.z;
+      a[3] += 1.0;
+    }
+  }
+  for (int k = 0; k < 4; ++k) red[t][k] = a[k];
+  __syncthreads();
+  if (t == 0) {
+    double s[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 256; ++k)
+      for (int m = 0; m < 4; ++m) s[m] += red[k][m];
+    const double n = s[3] > 0 ? s[3] : 1.0;
+    for (int m = 0; m < 3; ++m) cs[m] = s[m] / n;
+  }
+  __syncthreads();
+  double ss = 0.0, cnt = 0.0;
+  for (int i = t; i < n_out; i += 256) {
+    const float4 q = raw[t0 + (int64_t)i * nt / n_out];
+    if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) {
+      const double dx = q.x - cs[0], dy = q.y - cs[1], dz = q.z - cs[2];
+      ss += dx * dx + dy * dy + dz * dz;
+      cnt += 1.0;
+    }
+  }
+  red[t][0] = ss;
+  red[t][1] = cnt;
+  __syncthreads();
+  if (t == 0) {
+    double s = 0, c = 0;
+    for (int k = 0; k < 256; ++k) {
+      s += red[k][0];
+      c += red[k][1];
+    }
+    const double r = c > 0 ? sqrt(s / c) : 1.0;
+    cs[3] = r > 1e-12 ? r : 1.0;
+    for (int m = 0; m < 4; ++m) frame[pr * 4 + m] = (float)cs[m];
+  }
+  __syncthreads();
+  const double* T = guesses + (int64_t)pr * guess_stride * 16;
+  const float inv = (float)(1.0 / cs[3]);
+  const float c0 = (float)cs[0], c1 = (float)cs[1], c2 = (float)cs[2];
+  for (int i = t; i < n_out; i += 256) {
+    const float4 q = raw[t0 + (int64_t)i * nt / n_out];
+    Xt[i] = (isfinite(q.x) && isfinite(q.y) && isfinite(q.z))
+                ? make_float4((q.x - c0) * inv, (q.y - c1) * inv, (q.z - c2) * inv, 0.f)
+                : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 p = raw[s0 + (int64_t)i * ns / n_out];
+    if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+      float m[3];
+      for (int r = 0; r < 3; ++r)
+        m[r] = (float)(T[4 * r] * p.x + T[4 * r + 1] * p.y + T[4 * r + 2] * p.z + T[4 * r + 3]);
+      Xs[i] = make_float4((m[0] - c0) * inv, (m[1] - c1) * inv, (m[2] - c2) * inv, 0.f);
+    } else {
+      Xs[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+
+// learned guess = [R(omega) | s tau + c - R c] * T0, written to the pair's learned slot
+__global__ void k_prop_guess(const float* __restrict__ out, const float* __restrict__ frame,
+                             int npairs, int guess_stride, int slot, double* __restrict__ guesses) {
+  const int pr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pr >= npairs) return;
+  const double w0 = out[pr * 6], w1 = out[pr * 6 + 1], w2 = out[pr * 6 + 2];
+  const double th = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+  double R[9];
+  if (th < 1e-12) {
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  } else {
+    const double kx = w0 / th, ky = w1 / th, kz = w2 / th, s = sin(th), c = cos(th), v = 1.0 - c;
+    R[0] = c + kx * kx * v;
+    R[1] = kx * ky * v - kz * s;
+    R[2] = kx * kz * v + ky * s;
+    R[3] = ky * kx * v + kz * s;
+    R[4] = c + ky * ky * v;
+    R[5] = ky * kz * v - kx * s;
+    R[6] = kz * kx * v - ky * s;
+    R[7] = kz * ky * v + kx * s;
+    R[8] = c + kz * kz * v;
+  }
+  const double cx = frame[pr * 4], cy = frame[pr * 4 + 1], cz = frame[pr * 4 + 2], sg = frame[pr * 4 + 3];
+  double D[16];
+  for (int r = 0; r < 3; ++r) {
+    for (int k = 0; k < 3; ++k) D[4 * r + k] = R[3 * r + k];
+    D[4 * r + 3] = sg * out[pr * 6 + 3 + r] + (r == 0 ? cx : (r == 1 ? cy : cz)) -
+                   (R[3 * r] * cx + R[3 * r + 1] * cy + R[3 * r + 2] * cz);
+  }
+  D[12] = D[13] = D[14] = 0.0;
+  D[15] = 1.0;
+  const double* T0 = guesses + (int64_t)pr * guess_stride * 16;
+  double* G = guesses + ((int64_t)pr * guess_stride + slot) * 16;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 4; ++k) s += D[4 * i + k] * T0[4 * k + j];
+      G[4 * i + j] = s;
+    }
+}
+
 }  // namespace
 
 struct ProposalImpl {
@@ -55,7 +174,7 @@ struct ProposalImpl {
   int step = 0;
   DevBuf W, M, V, G, W2t, W3t;
   DevBuf X, part_max, part_arg, F, ARG, Z, OUT, DOUT, loss_part, DZ, DG, XA, H1A, DA1, H2A, DA2,
-      DF, part_grad, lab, idx, losses;
+      DF, part_grad, lab, idx, losses, frame;
   DevBuf ds_X, ds_lab;
   int ds_pairs = 0, ds_pts = 0;
   HostBuf h_io;
@@ -151,6 +270,21 @@ struct ProposalImpl {
   }
 };
 
+// Runs the network on every pair of a cycle (on the cycle's stream) and writes each
+// pair's learned guess into its slot `slot` of the instance guesses (instances
+// are pair-major with `stride` slots; slot 0 holds the plain seed).
+void proposal_guesses(ProposalImpl& P, hipStream_t s, const CloudSet& cs, const int* d_src,
+                      const int* d_tgt, int npairs, int npts, double* d_guess, int stride, int slot) {
+  P.reserve(npairs, npts);
+  P.frame.reserve((size_t)npairs * 4 * sizeof(float));
+  hipLaunchKernelGGL(k_prop_prepare, dim3(npairs), dim3(256), 0, s, cs.d_raw.as<float4>(),
+                     cs.d_cloud_off.as<int64_t>(), d_src, d_tgt, d_guess, stride, npts,
+                     P.X.as<float4>(), P.frame.as<float>());
+  launch_enc_forward(P.bufs(), npairs, npts, nullptr, s);
+  hipLaunchKernelGGL(k_prop_guess, dim3((npairs + 63) / 64), dim3(64), 0, s, P.OUT.as<float>(),
+                     P.frame.as<float>(), npairs, stride, slot, d_guess);
+}
+
 }  // namespace lcr
 
 using namespace lcr;
@@ -158,6 +292,8 @@ using namespace lcr;
 struct lcr_proposal {
   ProposalImpl impl;
 };
+
+lcr::ProposalImpl* lcr::proposal_impl(lcr_proposal* p) { return p ? &p->impl : nullptr; }
 
 static inline int lcr_set_error_(int code, const char* msg) { return set_last_error(code, msg); }
 

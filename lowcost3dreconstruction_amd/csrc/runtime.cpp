@@ -384,7 +384,8 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   return b;
 }
 
-void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph) {
+void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph,
+                 const std::function<void()>& pre) {
   if (npairs == 0) return;
   Profiler& prof = Profiler::get();
   const IcpBuffers b = buffers(cs);
@@ -397,6 +398,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   }
   LCR_HIP(hipMemcpyAsync(d_guess.p, h_guess.p, (size_t)npairs * 16 * sizeof(double),
                          hipMemcpyHostToDevice, s));
+  if (pre) pre();
   auto body = [&](std::vector<Profiler::Span>* spans) {
     launch_state_init(d_state.as<PairState>(), d_guess.as<double>(), d_pairs.as<PairDesc>(),
                       cs.d_grids.as<GridDesc>(), npairs, s);

@@ -6,6 +6,7 @@
 
 #include <string.h>
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -13,6 +14,8 @@
 
 #include "lcr_common.hpp"
 #include "lcr_kernels.hpp"
+
+struct lcr_proposal;
 
 namespace lcr {
 
@@ -161,7 +164,10 @@ struct Engine {
   IcpBuffers buffers(const CloudSet& cs) const;
   // Runs state init (from guesses, 16 doubles per instance or null) and up to
   // max_iterations iterations plus the fitness pass; results land in h_state.
-  void run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph);
+  // `pre` (optional) runs on stream s after the guesses reach the device and
+  // before the first iteration (it may rewrite d_guess).
+  void run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph,
+           const std::function<void()>& pre = {});
   void run_fitness(hipStream_t s, const CloudSet& cs, double max_range);
   const PairState* states() const { return h_state.as<PairState>(); }
 };
@@ -170,5 +176,13 @@ int device_cu_count();
 
 // Records the message returned by lcr_last_error() on this thread; returns code.
 int set_last_error(int code, const std::string& msg);
+
+// The pose proposal behind an lcr_proposal handle, and its per-cycle inference:
+// writes each pair's learned guess into slot `slot` of d_guess (pair-major,
+// `stride` slots per pair, slot 0 = the plain seed), on stream s.
+struct ProposalImpl;
+ProposalImpl* proposal_impl(lcr_proposal* p);
+void proposal_guesses(ProposalImpl& P, hipStream_t s, const CloudSet& cs, const int* d_src,
+                      const int* d_tgt, int npairs, int npts, double* d_guess, int stride, int slot);
 
 }  // namespace lcr
