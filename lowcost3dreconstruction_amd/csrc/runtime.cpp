@@ -326,9 +326,12 @@ Engine::~Engine() { reset_graph(); }
 void Engine::reset_graph() {
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
+  if (tail_exec) (void)hipGraphExecDestroy(tail_exec);
+  if (tail_graph) (void)hipGraphDestroy(tail_graph);
   exec = nullptr;
   graph = nullptr;
-  graph_iters = -1;
+  tail_exec = nullptr;
+  tail_graph = nullptr;
 }
 
 void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
@@ -384,6 +387,28 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   return b;
 }
 
+namespace {
+__global__ void k_count_active(const PairState* __restrict__ st, int n, int* __restrict__ out) {
+  __shared__ int s[256];
+  int a = 0;
+  for (int i = threadIdx.x; i < n; i += 256) a += st[i].active ? 1 : 0;
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+}  // namespace
+
+// The run is: state init -> chunks of kChunk iterations -> fitness.  Each chunk
+// ends with a count of still-active instances copied to pinned memory; the host
+// stops launching chunks once it reads zero, so converged cycles do not replay
+// empty iterations.  With the profiler off each chunk (and the tail) replays a
+// captured hipGraph; HIP events recorded inside a captured graph cannot be timed
+// afterwards, so with the profiler on the same launches are issued eagerly with
+// an event pair around each kernel.
 void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph,
                  const std::function<void()>& pre) {
   if (npairs == 0) return;
@@ -399,50 +424,73 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   LCR_HIP(hipMemcpyAsync(d_guess.p, h_guess.p, (size_t)npairs * 16 * sizeof(double),
                          hipMemcpyHostToDevice, s));
   if (pre) pre();
-  auto body = [&](std::vector<Profiler::Span>* spans) {
-    launch_state_init(d_state.as<PairState>(), d_guess.as<double>(), d_pairs.as<PairDesc>(),
-                      cs.d_grids.as<GridDesc>(), npairs, s);
-    if (ntiles > 0) {
-      for (int it = 0; it < c.max_iterations; ++it) {
-        const int sp = prof.begin("icp_iteration", s, spans);
-        launch_icp_iteration(b, c, nblocks, s);
-        prof.end(sp, s, spans);
-      }
-      const int sp = prof.begin("fitness", s, spans);
-      launch_fitness(b, DBL_MAX, nblocks, s);
-      prof.end(sp, s, spans);
+  launch_state_init(d_state.as<PairState>(), d_guess.as<double>(), d_pairs.as<PairDesc>(),
+                    cs.d_grids.as<GridDesc>(), npairs, s);
+  d_active.reserve(sizeof(int));
+  h_active.reserve(sizeof(int));
+  const bool eager = !use_graph || prof.on();
+  std::vector<Profiler::Span> spans;
+  auto chunk = [&](std::vector<Profiler::Span>* sp) {
+    for (int it = 0; it < kChunk; ++it) {
+      const int k = prof.begin("icp_iteration", s, sp);
+      launch_icp_iteration(b, c, nblocks, s);
+      prof.end(k, s, sp);
     }
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, d_state.as<PairState>(), npairs,
+                       d_active.as<int>());
+    LCR_HIP(hipMemcpyAsync(h_active.p, d_active.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  };
+  auto tail = [&](std::vector<Profiler::Span>* sp) {
+    const int k = prof.begin("fitness", s, sp);
+    launch_fitness(b, DBL_MAX, nblocks, s);
+    prof.end(k, s, sp);
     LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
                            hipMemcpyDeviceToHost, s));
   };
-  // HIP events recorded inside a captured graph cannot be timed afterwards, so
-  // with the profiler on the same launches are issued eagerly with an event pair
-  // around each kernel; otherwise the whole run replays one hipGraph.
-  if (!use_graph || prof.on()) {
-    std::vector<Profiler::Span> spans;
-    body(&spans);
-    LCR_HIP(hipStreamSynchronize(s));
-    if (!spans.empty()) prof.collect(spans, false);
-    return;
-  }
-  const GraphKey key = graph_key(b);
-  if (!exec || !(key == graph_key_)) {
-    reset_graph();
-    LCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      body(nullptr);
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
+  if (!eager) {
+    const GraphKey key = graph_key(b);
+    if (!exec || !tail_exec || !(key == graph_key_)) {
+      reset_graph();
+      capture(s, [&] { chunk(nullptr); }, &graph, &exec);
+      capture(s, [&] { tail(nullptr); }, &tail_graph, &tail_exec);
+      graph_key_ = key;
     }
-    LCR_HIP(hipStreamEndCapture(s, &graph));
-    LCR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    graph_key_ = key;
   }
-  LCR_HIP(hipGraphLaunch(exec, s));
+  const int max_chunks = ntiles > 0 ? (c.max_iterations + kChunk - 1) / kChunk : 0;
+  for (int k = 0; k < max_chunks; ++k) {
+    if (eager)
+      chunk(&spans);
+    else
+      LCR_HIP(hipGraphLaunch(exec, s));
+    LCR_HIP(hipStreamSynchronize(s));
+    if (*h_active.as<int>() == 0) break;
+  }
+  if (ntiles > 0) {
+    if (eager)
+      tail(&spans);
+    else
+      LCR_HIP(hipGraphLaunch(tail_exec, s));
+  } else {
+    LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
+                           hipMemcpyDeviceToHost, s));
+  }
   LCR_HIP(hipStreamSynchronize(s));
+  if (!spans.empty()) prof.collect(spans, false);
+}
+
+void Engine::capture(hipStream_t s, const std::function<void()>& f, hipGraph_t* g,
+                     hipGraphExec_t* e) {
+  LCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    f();
+  } catch (...) {
+    hipGraph_t t = nullptr;
+    (void)hipStreamEndCapture(s, &t);
+    if (t) (void)hipGraphDestroy(t);
+    throw;
+  }
+  LCR_HIP(hipStreamEndCapture(s, g));
+  LCR_HIP(hipGraphInstantiate(e, *g, nullptr, nullptr, 0));
 }
 
 // Everything a captured graph bakes in: kernel arguments and buffer addresses.
@@ -454,6 +502,8 @@ Engine::GraphKey Engine::graph_key(const IcpBuffers& b) const {
   k.nblocks = nblocks;
   k.guess = d_guess.p;
   k.hstate = h_state.p;
+  k.active = d_active.p;
+  k.hactive = h_active.p;
   return k;
 }
 

@@ -142,19 +142,25 @@ struct Engine {
   std::vector<PairDesc> hp;
   DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess;
   HostBuf h_state, h_guess;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int graph_iters = -1;
+  static constexpr int kChunk = 4;  // iterations per captured chunk
+  DevBuf d_active;
+  HostBuf h_active;
+  hipGraph_t graph = nullptr, tail_graph = nullptr;
+  hipGraphExec_t exec = nullptr, tail_exec = nullptr;
   struct GraphKey {
     IcpBuffers b{};
     IcpConst c{};
     int nblocks = 0;
     const void* guess = nullptr;
     const void* hstate = nullptr;
+    const void* active = nullptr;
+    const void* hactive = nullptr;
     bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof *this) == 0; }
   };
   GraphKey graph_key_{};
   GraphKey graph_key(const IcpBuffers& b) const;
+  static void capture(hipStream_t s, const std::function<void()>& f, hipGraph_t* g,
+                      hipGraphExec_t* e);
 
   ~Engine();
   void reset_graph();
