@@ -393,6 +393,7 @@ int lcr_icp_correspondences(lcr_icp* h, const double* T, int32_t* idx, float* d2
     LCR_HIP(hipMemcpyAsync(slot, h->h_idx.p, sizeof probe, hipMemcpyHostToDevice, h->stream.s));
     IcpBuffers b = h->eng.buffers(h->cs);
     b.state = slot;
+    b.prev = nullptr;
     h->d_idx.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t));
     h->d_d2.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(float));
     if (n > 0) {

@@ -129,8 +129,11 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
       float qx, qy, qz;
       xform(st->T, s, qx, qy, qz);
       const float r = (float)sqrt(pd.max_corr_d2);
+      const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
+      const int seed = b.prev ? b.prev[slot] : -1;
       const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r,
-                                   isfinite(r));
+                                   isfinite(r), seed);
+      if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
         const float4 t = b.pts[nn.j];
         const float4 n = P2PL ? b.nrm[nn.j] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -207,8 +210,9 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
       const float4 s = b.pts[gs.pt_off + local];
       float qx, qy, qz;
       xform(st->T, s, qx, qy, qz);
+      const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
       const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, INFINITY,
-                                   false);
+                                   false, seed);
       if (nn.j >= 0 && (double)nn.d2 <= max_range) {
         v[0] = nn.d2;
         v[1] = 1.0;

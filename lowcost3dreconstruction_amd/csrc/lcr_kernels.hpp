@@ -33,6 +33,8 @@ struct IcpBuffers {
   const int* tile_pair;   // tile -> instance
   PairState* state;
   double* partials;       // ntiles * kMaxVals
+  int* prev;              // ntiles * tile: last match (sorted target position) per source
+                          // point, -1 = none; seeds the next search (nullable)
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)

@@ -356,6 +356,7 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   d_tile_pair.reserve(std::max(ntiles, 1) * sizeof(int));
   d_state.reserve((size_t)(npairs + 1) * sizeof(PairState));
   d_partials.reserve((size_t)std::max(ntiles, 1) * kMaxVals * sizeof(double));
+  d_prev.reserve((size_t)std::max(ntiles, 1) * tile * sizeof(int));
   d_guess.reserve((size_t)std::max(npairs, 1) * 16 * sizeof(double));
   h_state.reserve((size_t)(npairs + 1) * sizeof(PairState));
   h_guess.reserve((size_t)std::max(npairs, 1) * 16 * sizeof(double));
@@ -381,6 +382,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.tile_pair = d_tile_pair.as<int>();
   b.state = d_state.as<PairState>();
   b.partials = d_partials.as<double>();
+  b.prev = d_prev.as<int>();
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;
@@ -426,6 +428,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   if (pre) pre();
   launch_state_init(d_state.as<PairState>(), d_guess.as<double>(), d_pairs.as<PairDesc>(),
                     cs.d_grids.as<GridDesc>(), npairs, s);
+  launch_fill_i32(d_prev.as<int>(), (int64_t)ntiles * tile, -1, s);  // no previous matches
   d_active.reserve(sizeof(int));
   h_active.reserve(sizeof(int));
   const bool eager = !use_graph || prof.on();

@@ -140,7 +140,7 @@ struct Engine {
   int npairs = 0, ntiles = 0, tile = 256, nblocks = 8;
   IcpConst c{};
   std::vector<PairDesc> hp;
-  DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess;
+  DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess, d_prev;
   HostBuf h_state, h_guess;
   static constexpr int kChunk = 4;  // iterations per captured chunk
   DevBuf d_active;
