@@ -532,7 +532,11 @@ int lcr_cycle_run(lcr_cycle* c) {
     std::vector<Profiler::Span> spans;
     const int k = d.normals_k > 0 ? d.normals_k : 30;
     c->cs.has_nrm = !c->need_normals && c->cs.has_nrm;
-    c->cs.build(c->stream.s, d.params.cell_factor, &spans);
+    // cells sized for a k-neighbourhood (~sqrt(k) point spacings) when normals are
+    // estimated here, else for 1-NN
+    double factor = d.params.cell_factor;
+    if (factor <= 0.0 && c->need_normals) factor = std::max(2.5, 0.75 * sqrt((double)k));
+    c->cs.build(c->stream.s, factor, &spans);
     if (c->need_normals) c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
     if (d.params.point_to_plane && !c->cs.has_nrm)
       return fail(LCR_E_NORMALS, "point-to-plane ICP needs normals");

@@ -105,8 +105,7 @@ __device__ __forceinline__ int cell_of(const GridDesc& g, const float4& p) {
   return (z * g.ny + y) * g.nx + x;
 }
 
-// Per point: global cell index (or -1 for non-finite points) and per-cell counts;
-// occupied[cloud] counts cells that became non-empty.
+// Per point: global cell index (or -1 for non-finite points) and per-cell counts.
 __global__ void k_cell_count(const float4* __restrict__ raw, const int* __restrict__ point_cloud,
                              const GridDesc* __restrict__ grids, int64_t n,
                              int* __restrict__ cid, int* __restrict__ counts,
@@ -122,8 +121,26 @@ __global__ void k_cell_count(const float4* __restrict__ raw, const int* __restri
   const GridDesc g = grids[c];
   const int gc = (int)g.cell_off + cell_of(g, p);
   cid[i] = gc;
-  const int old = atomicAdd(&counts[gc], 1);
-  if (old == 0 && occupied) atomicAdd(&occupied[c], 1);
+  atomicAdd(&counts[gc], 1);
+}
+
+// occupied[cloud] = number of non-empty cells of the cloud's grid (one atomic per
+// block after a block reduction)
+__global__ void k_occupancy(const GridDesc* __restrict__ grids, const int* __restrict__ counts,
+                            int* __restrict__ occupied) {
+  __shared__ int s[256];
+  const GridDesc g = grids[blockIdx.y];
+  const int64_t ncells = (int64_t)g.nx * g.ny * g.nz;
+  int a = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ncells; i += (int64_t)gridDim.x * 256)
+    a += counts[g.cell_off + i] > 0 ? 1 : 0;
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && s[0] > 0) atomicAdd(&occupied[blockIdx.y], s[0]);
 }
 
 constexpr int kScanThreads = 1024;
@@ -238,6 +255,45 @@ __global__ void k_cell_sort(const int* __restrict__ cell_start, int64_t ncells_t
   }
 }
 
+// Bounding-volume tree over each cloud's cell-sorted points: leaf i of cloud c
+// covers sorted points [16 i, 16 i + 16); padding leaves are empty boxes.
+__global__ void k_bvh_leaves(const GridDesc* __restrict__ grids, const float4* __restrict__ pts,
+                             float4* __restrict__ nodes) {
+  const GridDesc g = grids[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nslots = (int64_t)1 << g.depth;
+  if (i >= nslots) return;
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+  float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  const int64_t s = i * kLeaf, e = min<int64_t>(s + kLeaf, g.npts);
+  for (int64_t j = s; j < e; ++j) {
+    const float4 p = pts[g.pt_off + j];
+    lo.x = fminf(lo.x, p.x);
+    lo.y = fminf(lo.y, p.y);
+    lo.z = fminf(lo.z, p.z);
+    hi.x = fmaxf(hi.x, p.x);
+    hi.y = fmaxf(hi.y, p.y);
+    hi.z = fmaxf(hi.z, p.z);
+  }
+  float4* n = nodes + 2 * (g.node_off + nslots + i);
+  n[0] = lo;
+  n[1] = hi;
+}
+
+// one tree level, `step` levels above each cloud's leaves
+__global__ void k_bvh_up(const GridDesc* __restrict__ grids, int step, float4* __restrict__ nodes) {
+  const GridDesc g = grids[blockIdx.y];
+  const int level = g.depth - step;
+  if (level < 0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ((int64_t)1 << level)) return;
+  const int64_t node = ((int64_t)1 << level) + i;
+  float4* N = nodes + 2 * g.node_off;
+  const float4 l0 = N[4 * node], l1 = N[4 * node + 1], r0 = N[4 * node + 2], r1 = N[4 * node + 3];
+  N[2 * node] = make_float4(fminf(l0.x, r0.x), fminf(l0.y, r0.y), fminf(l0.z, r0.z), 0.f);
+  N[2 * node + 1] = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
+}
+
 __global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -256,6 +312,19 @@ void launch_bbox(const float4* raw, const int64_t* cloud_off, int nclouds, float
                      chunks, out);
 }
 
+void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const float4* pts,
+                      float4* nodes, hipStream_t s) {
+  if (nclouds <= 0) return;
+  const int64_t slots = (int64_t)1 << max_depth;
+  hipLaunchKernelGGL(k_bvh_leaves, dim3(blocks_for(slots, 256), nclouds), dim3(256), 0, s, grids,
+                     pts, nodes);
+  for (int step = 1; step <= max_depth; ++step) {
+    const int64_t n = (int64_t)1 << (max_depth - step);
+    hipLaunchKernelGGL(k_bvh_up, dim3(blocks_for(n, 256), nclouds), dim3(256), 0, s, grids, step,
+                       nodes);
+  }
+}
+
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
@@ -266,6 +335,12 @@ void launch_cell_count(const float4* raw, const int* point_cloud, const GridDesc
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, raw, point_cloud,
                      grids, n, cid, counts, occupied);
+}
+
+void launch_occupancy(const GridDesc* grids, int nclouds, const int* counts, int* occupied,
+                      hipStream_t s) {
+  if (nclouds <= 0) return;
+  hipLaunchKernelGGL(k_occupancy, dim3(64, nclouds), dim3(256), 0, s, grids, counts, occupied);
 }
 
 int64_t scan_tmp_ints(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
       const float r = (float)sqrt(pd.max_corr_d2);
       const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
       const int seed = b.prev ? b.prev[slot] : -1;
-      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r,
+      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r,
                                    isfinite(r), seed);
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
@@ -211,8 +211,8 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
       float qx, qy, qz;
       xform(st->T, s, qx, qy, qz);
       const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
-      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, INFINITY,
-                                   false, seed);
+      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz,
+                                   INFINITY, false, seed);
       if (nn.j >= 0 && (double)nn.d2 <= max_range) {
         v[0] = nn.d2;
         v[1] = 1.0;
@@ -282,7 +282,7 @@ __global__ void k_correspondences(const IcpBuffers b, int pair, int32_t* idx, fl
   xform(b.state[pair].T, s, qx, qy, qz);
   const float r = (float)sqrt(pd.max_corr_d2);
   const NNResult nn =
-      nn_query(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r, isfinite(r));
+      nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, isfinite(r));
   const int o = __float_as_int(s.w);
   const bool keep = nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2;
   idx[o] = keep ? nn.id : -1;

@@ -34,7 +34,13 @@ struct GridDesc {
   int64_t pt_off;     // first point of the cloud in the sorted arrays
   int npts;           // finite points in the grid
   int nraw;           // points handed in (including non-finite ones)
+  int64_t node_off;   // first node of this cloud's bounding-volume tree (2 float4 each)
+  int depth;          // tree depth: leaves are nodes [2^depth, 2^(depth+1)), 1-based
+  int pad_;
 };
+
+// Points per leaf of the bounding-volume tree over a cloud's cell-sorted points.
+constexpr int kLeaf = 16;
 
 // One ICP instance (a pair, or one pose hypothesis of a pair).
 struct PairDesc {

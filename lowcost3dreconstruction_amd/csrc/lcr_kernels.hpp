@@ -15,6 +15,8 @@ constexpr int kBBoxChunks = 64;  // partial boxes per cloud (part holds 7 floats
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_cell_count(const float4* raw, const int* point_cloud, const GridDesc* grids,
                        int64_t n, int* cid, int* counts, int* occupied, hipStream_t s);
+void launch_occupancy(const GridDesc* grids, int nclouds, const int* counts, int* occupied,
+                      hipStream_t s);
 int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
 void launch_scatter(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n,
@@ -22,6 +24,9 @@ void launch_scatter(const float4* raw, const float4* raw_nrm, const int* cid, in
                     const int64_t* cloud_off, const int* point_cloud, hipStream_t s);
 void launch_cell_sort(const int* cell_start, int64_t ncells_total, float4* pts, float4* nrm,
                       hipStream_t s);
+// bounding-volume trees (GridDesc::node_off / depth set by the host)
+void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const float4* pts,
+                      float4* nodes, hipStream_t s);
 
 // ---- K2-K6 ICP iteration, fitness, output (icp.hip)
 struct IcpBuffers {
@@ -29,6 +34,7 @@ struct IcpBuffers {
   const int* cell_start;
   const float4* pts;      // cell-sorted points of all clouds
   const float4* nrm;      // cell-sorted normals (nullable for point-to-point)
+  const float4* nodes;    // bounding-volume trees (2 float4 per node)
   const PairDesc* pairs;
   const int* tile_pair;   // tile -> instance
   PairState* state;

@@ -128,6 +128,61 @@ LCR_SHD void scan_points(const float4* __restrict__ P, int s, int e, float qx, f
   }
 }
 
+// Squared distance from q to the box [lo, hi] (0 inside), float.
+LCR_SHD float box_d2(const float4& lo, const float4& hi, float qx, float qy, float qz) {
+  const float dx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.0f);
+  const float dy = fmaxf(fmaxf(lo.y - qy, qy - hi.y), 0.0f);
+  const float dz = fmaxf(fmaxf(lo.z - qz, qz - hi.z), 0.0f);
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Exact 1-NN by traversal of the cloud's bounding-volume tree (an implicit
+// complete binary tree over runs of kLeaf cell-sorted points; node i has children
+// 2i and 2i+1, empty padding leaves have lo > hi).  Depth-first, nearer child
+// first, no stack: `order` keeps one bit per depth saying whether the right child
+// was entered first.  A subtree is skipped only when its box is farther than the
+// bound by a relative margin, so equal distances are still visited.  `limit2`
+// (may be +inf) caps the bound for radius-limited searches.
+LCR_SHD void bvh_nn(const GridDesc& g, const float4* __restrict__ nodes,
+                    const float4* __restrict__ P, float qx, float qy, float qz, float limit2,
+                    float& best, int& bid, int& bj) {
+  const float4* N = nodes + 2 * g.node_off;
+  const unsigned first_leaf = 1u << g.depth;
+  const int end = (int)g.pt_off + g.npts;
+  unsigned i = 1, order = 0;
+  int depth = 0;
+  for (;;) {
+    const float bound = fminf(best, limit2) * 1.00002f;
+    bool descend = box_d2(N[2 * i], N[2 * i + 1], qx, qy, qz) <= bound;
+    if (descend && i >= first_leaf) {
+      const int s = (int)g.pt_off + (int)(i - first_leaf) * kLeaf;
+      scan_points(P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, bj);
+      descend = false;
+    }
+    if (descend) {
+      const unsigned l = 2 * i;
+      const float dl = box_d2(N[2 * l], N[2 * l + 1], qx, qy, qz);
+      const float dr = box_d2(N[2 * l + 2], N[2 * l + 3], qx, qy, qz);
+      const bool right_first = dr < dl;
+      ++depth;
+      order = right_first ? (order | (1u << depth)) : (order & ~(1u << depth));
+      i = right_first ? l + 1 : l;
+      continue;
+    }
+    // next subtree: the sibling if i was entered first, else climb
+    for (;;) {
+      if (depth == 0) return;
+      const unsigned right_first = (order >> depth) & 1u;
+      if ((i & 1u) == right_first) {
+        i ^= 1u;
+        break;
+      }
+      i >>= 1;
+      --depth;
+    }
+  }
+}
+
 // Exact 1-NN of q among the grid's points.  With `bounded`, points farther than
 // `r` may be skipped (the result is then either the true nearest or a point the
 // caller rejects as d2 > r^2).  `seed_j` (or -1) is a known candidate, e.g. the
@@ -135,9 +190,14 @@ LCR_SHD void scan_points(const float4* __restrict__ P, int s, int e, float qx, f
 // the same.  Inside each ring, rows and cells whose lower-bound distance exceeds
 // the current best are skipped (with a relative margin, so equal distances are
 // still visited and the smallest index wins).
+//
+// After `ring_limit` rings without certification the rest of the search runs as a
+// tree traversal (bvh_nn) seeded with the best point found so far, so queries far
+// from the cloud cost O(log N) instead of O(distance^3) cells.
 LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
-                          const float4* __restrict__ P, float qx, float qy, float qz, float r,
-                          bool bounded, int seed_j = -1) {
+                          const float4* __restrict__ P, const float4* __restrict__ nodes,
+                          float qx, float qy, float qz, float r, bool bounded, int seed_j = -1,
+                          int ring_limit = 2) {
   NNResult res{INFINITY, 0x7fffffff, -1};
   if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
   const CellPos c = cell_pos(g, qx, qy, qz);
@@ -152,6 +212,13 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   int bid = 0x7fffffff, bj = -1;
   if (seed_j >= 0) scan_points(P, seed_j, seed_j + 1, qx, qy, qz, best, bid, bj);
   const float h2 = g.h * g.h;
+  if (nodes && k0 > ring_limit) {  // far outside the grid: straight to the tree
+    bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
+    res.d2 = best;
+    res.id = bid;
+    res.j = bj;
+    return res;
+  }
   for (int k = k0; k <= kmax; ++k) {
     const int z0 = max(c.cz - k, 0), z1 = min(c.cz + k, g.nz - 1);
     const int y0 = max(c.cy - k, 0), y1 = min(c.cy + k, g.ny - 1);
@@ -201,6 +268,10 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
     const float L = (Lc - c.slack) * g.h;
     if (L > 0.0f && best < L * L * 0.99998f) break;
     if (bounded && L > rlim) break;
+    if (nodes && k - k0 >= ring_limit) {
+      bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
+      break;
+    }
   }
   res.d2 = best;
   res.id = bid;

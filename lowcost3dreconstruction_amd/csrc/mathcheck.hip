@@ -9,6 +9,7 @@
 #include "lcr_search.hpp"
 
 #include <math.h>
+#include <algorithm>
 #include <vector>
 
 using namespace lcr;
@@ -70,8 +71,10 @@ int lcrm_conv_step(int* io, double* dio, int max_iterations, const double* thr,
 // Returns 0, or -1 when the grid would be too large.
 // seed_ids (nullable): per query an original target index used as the search's
 // seed candidate (-1 = none).
+// ring_limit < 0: no tree (ring search only); otherwise the tree takes over after
+// ring_limit rings, as on the device.
 int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float h, float r,
-                 int bounded, const int32_t* seed_ids, int32_t* idx, float* d2) {
+                 int bounded, const int32_t* seed_ids, int ring_limit, int32_t* idx, float* d2) {
   GridDesc g{};
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   int npts = 0;
@@ -120,12 +123,36 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
     const float* p = tgt + 3 * i;
     P[fill[cid[i]]++] = make_float4(p[0], p[1], p[2], __builtin_bit_cast(float, (int)i));
   }
+  // bounding-volume tree in the device layout
+  const int64_t leaves = (npts + kLeaf - 1) / kLeaf;
+  int depth = 0;
+  while (((int64_t)1 << depth) < leaves) ++depth;
+  g.depth = depth;
+  g.node_off = 0;
+  g.pt_off = 0;
+  const int64_t nslots = (int64_t)1 << depth;
+  std::vector<float4> nodes((size_t)(2 * nslots) * 2);
+  for (int64_t i = 0; i < nslots; ++i) {
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    for (int64_t j = i * kLeaf; j < std::min<int64_t>(i * kLeaf + kLeaf, npts); ++j) {
+      lo.x = fminf(lo.x, P[j].x); lo.y = fminf(lo.y, P[j].y); lo.z = fminf(lo.z, P[j].z);
+      hi.x = fmaxf(hi.x, P[j].x); hi.y = fmaxf(hi.y, P[j].y); hi.z = fmaxf(hi.z, P[j].z);
+    }
+    nodes[2 * (nslots + i)] = lo;
+    nodes[2 * (nslots + i) + 1] = hi;
+  }
+  for (int64_t node = nslots - 1; node >= 1; --node) {
+    const float4 l0 = nodes[4 * node], l1 = nodes[4 * node + 1], r0 = nodes[4 * node + 2], r1 = nodes[4 * node + 3];
+    nodes[2 * node] = make_float4(fminf(l0.x, r0.x), fminf(l0.y, r0.y), fminf(l0.z, r0.z), 0.f);
+    nodes[2 * node + 1] = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
+  }
   std::vector<int> pos(nt, -1);
   for (int j = 0; j < npts; ++j) pos[__builtin_bit_cast(int, P[j].w)] = j;
   for (int64_t i = 0; i < nq; ++i) {
     const int seed = (seed_ids && seed_ids[i] >= 0) ? pos[seed_ids[i]] : -1;
-    const NNResult res = nn_query(g, cs.data(), P.data(), q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
-                                  bounded != 0, seed);
+    const NNResult res = nn_query(g, cs.data(), P.data(), ring_limit >= 0 ? nodes.data() : nullptr,
+                                  q[3 * i], q[3 * i + 1], q[3 * i + 2], r, bounded != 0, seed,
+                                  ring_limit >= 0 ? ring_limit : 0);
     const bool keep = res.j >= 0 && (!bounded || (double)res.d2 <= (double)r * (double)r);
     idx[i] = keep ? res.id : -1;
     d2[i] = res.d2;

@@ -233,7 +233,8 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   launch_fill_i32(d_counts.as<int>(), cells, 0, s);
   launch_fill_i32(d_occ.as<int>(), n, 0, s);
   launch_cell_count(d_raw.as<float4>(), d_point_cloud.as<int>(), d_grids.as<GridDesc>(),
-                    total_raw, d_cid.as<int>(), d_counts.as<int>(), d_occ.as<int>(), s);
+                    total_raw, d_cid.as<int>(), d_counts.as<int>(), nullptr, s);
+  launch_occupancy(d_grids.as<GridDesc>(), n, d_counts.as<int>(), d_occ.as<int>(), s);
   LCR_HIP(hipMemcpyAsync(occ, d_occ.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
   LCR_HIP(hipStreamSynchronize(s));
 
@@ -269,6 +270,20 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   }
   if (cells > (int64_t)1 << 30) throw Error{-5, "grid too large"};
   total_cells = cells;
+  // bounding-volume trees: leaves of kLeaf sorted points, complete binary tree
+  int64_t nodes = 0;
+  max_depth = 0;
+  for (int c = 0; c < n; ++c) {
+    GridDesc& g = grids[c];
+    const int64_t leaves = (g.npts + kLeaf - 1) / kLeaf;
+    int d = 0;
+    while (((int64_t)1 << d) < leaves) ++d;
+    g.depth = d;
+    g.node_off = nodes;
+    nodes += (int64_t)2 << d;  // 1-based heap: indices 1 .. 2^(d+1) - 1
+    max_depth = std::max(max_depth, d);
+  }
+  total_nodes = nodes;
   memcpy(hg, grids.data(), (size_t)n * sizeof(GridDesc));
   LCR_HIP(hipMemcpyAsync(d_grids.p, hg, (size_t)n * sizeof(GridDesc), hipMemcpyHostToDevice, s));
 
@@ -290,6 +305,8 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
                  d_nrm.as<float4>(), d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), s);
   launch_cell_sort(d_cell_start.as<int>(), cells - 1, d_pts.as<float4>(),
                    has_nrm ? d_nrm.as<float4>() : nullptr, s);
+  d_nodes.reserve((size_t)std::max<int64_t>(total_nodes, 1) * 2 * sizeof(float4));
+  launch_bvh_build(d_grids.as<GridDesc>(), n, max_depth, d_pts.as<float4>(), d_nodes.as<float4>(), s);
   // sorted point -> cloud
   for (int c = 0; c < n; ++c) hpoff[c] = grids[c].pt_off;
   hpoff[n] = total_pts;
@@ -378,6 +395,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.cell_start = cs.d_cell_start.as<int>();
   b.pts = cs.d_pts.as<float4>();
   b.nrm = cs.has_nrm ? cs.d_nrm.as<float4>() : nullptr;
+  b.nodes = cs.d_nodes.as<float4>();
   b.pairs = d_pairs.as<PairDesc>();
   b.tile_pair = d_tile_pair.as<int>();
   b.state = d_state.as<PairState>();

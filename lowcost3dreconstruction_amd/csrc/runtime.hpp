@@ -116,11 +116,12 @@ struct CloudSet {
   int64_t total_raw = 0;
   bool has_nrm = false;
   std::vector<GridDesc> grids;
-  int64_t total_pts = 0, total_cells = 0;
+  int64_t total_pts = 0, total_cells = 0, total_nodes = 0;
+  int max_depth = 0;
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
   DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
-  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff;
+  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes;
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
 
   // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;

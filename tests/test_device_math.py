@@ -155,9 +155,9 @@ def test_mat4_mul(m):
     np.testing.assert_allclose(Cm.reshape(4, 4), A @ B, atol=1e-14)
 
 
-def _nn_grid(m, tgt, q, h, r, bounded, seeds=None):
+def _nn_grid(m, tgt, q, h, r, bounded, seeds=None, ring_limit=2):
     m.lcrm_nn_grid.argtypes = [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), C.c_int64,
-                               C.c_float, C.c_float, C.c_int, C.POINTER(C.c_int32),
+                               C.c_float, C.c_float, C.c_int, C.POINTER(C.c_int32), C.c_int,
                                C.POINTER(C.c_int32), C.POINTER(C.c_float)]
     tgt = np.ascontiguousarray(tgt, np.float32)
     q = np.ascontiguousarray(q, np.float32)
@@ -166,7 +166,7 @@ def _nn_grid(m, tgt, q, h, r, bounded, seeds=None):
     fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
     sp = None if seeds is None else np.ascontiguousarray(seeds, np.int32).ctypes.data_as(
         C.POINTER(C.c_int32))
-    rc = m.lcrm_nn_grid(fp(tgt), len(tgt), fp(q), len(q), h, r, int(bounded), sp,
+    rc = m.lcrm_nn_grid(fp(tgt), len(tgt), fp(q), len(q), h, r, int(bounded), sp, ring_limit,
                         idx.ctypes.data_as(C.POINTER(C.c_int32)), fp(d2))
     assert rc == 0
     return idx, d2
@@ -195,6 +195,8 @@ def test_grid_ring_search_equals_bruteforce(m, h, r):
     seeds_rand[seeds_rand == 5] = -1  # the NaN point is not in the grid
     seeds_true = np.where(ridx >= 0, ridx, -1)
     for seeds in (None, seeds_rand, seeds_true):
-        idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded, seeds)
-        np.testing.assert_array_equal(idx, ridx)
-        np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
+        # ring search alone, tree after 2 rings (the device default), tree at once
+        for ring_limit in (-1, 2, 0):
+            idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded, seeds, ring_limit)
+            np.testing.assert_array_equal(idx, ridx)
+            np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
