@@ -86,6 +86,33 @@ def aligned_mask(T, refs):
     return np.array(ok), np.array(errs)
 
 
+def copy_calibration(mib):
+    """A copy kernel of known size: each launch reads and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}),
+          flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per iteration launch from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy calibration), or
+    None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "icp_iteration_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return float(t["traffic_bytes_per_launch"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +123,13 @@ def main():
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-pairs", type=int, default=36)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times "
+                         "(reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one MI355X, so "
               f"--gpus {args.gpus} is not supported (the pairs of a cycle shard across "
@@ -219,7 +252,7 @@ def main():
                       "note": "one pair at a time through the per-pair API (normals + upload + align + fitness)"},
         "speedup_vs_plain_gpu": value / plain_value if plain_value > 0 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "icp_iteration", "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "kernel_ms_per_step": {"icp_iteration": iter_ms / args.steps, "fitness": fit_ms / args.steps,

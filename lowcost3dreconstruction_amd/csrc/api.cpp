@@ -269,13 +269,13 @@ static void prepare(lcr_icp* h) {
     h->cs.upload(h->stream.s, xyz, nrm_up, cnt, 3 * sizeof(float));
     h->cs.build(h->stream.s, h->params.cell_factor, nullptr);
     const double r = h->params.max_corr_dist;
-    h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 256);
+    h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 64);
     h->dirty = false;
   } else {
     // parameters may have changed: refresh the instance radius
     const double r = h->params.max_corr_dist;
     if (h->eng.hp.size() == 1 && h->eng.hp[0].max_corr_d2 != r * r) {
-      h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 256);
+      h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 64);
     }
   }
 }
@@ -531,13 +531,15 @@ int lcr_cycle_run(lcr_cycle* c) {
     Profiler& prof = Profiler::get();
     std::vector<Profiler::Span> spans;
     const int k = d.normals_k > 0 ? d.normals_k : 30;
-    c->cs.has_nrm = !c->need_normals && c->cs.has_nrm;
-    // cells sized for a k-neighbourhood (~sqrt(k) point spacings) when normals are
-    // estimated here, else for 1-NN
-    double factor = d.params.cell_factor;
-    if (factor <= 0.0 && c->need_normals) factor = std::max(2.5, 0.75 * sqrt((double)k));
-    c->cs.build(c->stream.s, factor, &spans);
-    if (c->need_normals) c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
+    if (c->need_normals) {
+      // normals on a grid with cells sized for a k-neighbourhood (~sqrt(k) point
+      // spacings), then the ICP grid with cells sized for 1-NN
+      c->cs.has_nrm = false;
+      c->cs.build(c->stream.s, std::max(2.5, 0.75 * sqrt((double)k)), &spans);
+      c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
+      c->cs.normals_to_raw(c->stream.s);
+    }
+    c->cs.build(c->stream.s, d.params.cell_factor, &spans);
     if (d.params.point_to_plane && !c->cs.has_nrm)
       return fail(LCR_E_NORMALS, "point-to-plane ICP needs normals");
     const int H = c->nhyp_total;
@@ -550,7 +552,7 @@ int lcr_cycle_run(lcr_cycle* c) {
         tgt[p * H + hy] = c->tgt[p];
       }
     c->eng.c = make_const(d.params);
-    const int tile = d.params.tile ? d.params.tile : 256;
+    const int tile = d.params.tile ? d.params.tile : 64;
     bool same = c->eng.npairs == ninst && c->eng.tile == tile && c->eng.hp.size() == (size_t)ninst;
     if (same) {
       for (int i = 0; i < ninst && same; ++i)

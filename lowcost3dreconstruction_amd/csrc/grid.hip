@@ -216,43 +216,39 @@ __global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restri
   out[i] += sums[i / kScanTile];
 }
 
-__global__ void k_scatter(const float4* __restrict__ raw, const float4* __restrict__ raw_nrm,
-                          const int* __restrict__ cid, int64_t n, const int* __restrict__ cell_start,
-                          int* __restrict__ fill, float4* __restrict__ pts,
-                          float4* __restrict__ nrm, const int64_t* __restrict__ cloud_off,
-                          const int* __restrict__ point_cloud) {
+// Cell-sorted placement, step 1: each finite point takes a slot of its cell in
+// arbitrary (atomic) order and leaves its raw index and cell there.
+__global__ void k_scatter(const int* __restrict__ cid, int64_t n, const int* __restrict__ cell_start,
+                          int* __restrict__ fill, int* __restrict__ slot_raw,
+                          int* __restrict__ slot_cell) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = cid[i];
   if (c < 0) return;
   const int pos = cell_start[c] + atomicAdd(&fill[c], 1);
-  float4 p = raw[i];
-  const int local = (int)(i - cloud_off[point_cloud[i]]);
-  p.w = __int_as_float(local);
-  pts[pos] = p;
-  if (raw_nrm) nrm[pos] = raw_nrm[i];
+  slot_raw[pos] = (int)i;
+  slot_cell[pos] = c;
 }
 
-// One thread per cell: insertion sort of the cell's points by original index.
-__global__ void k_cell_sort(const int* __restrict__ cell_start, int64_t ncells_total,
-                            float4* __restrict__ pts, float4* __restrict__ nrm) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells_total) return;
+// Step 2, one thread per slot: the point's final position is its cell's start plus
+// the number of the cell's points with a smaller raw index (raw order within a
+// cloud is original-index order), so points of a cell end up sorted by original
+// index whatever order step 1 produced.
+__global__ void k_place(const float4* __restrict__ raw, const float4* __restrict__ raw_nrm,
+                        const int* __restrict__ slot_raw, const int* __restrict__ slot_cell,
+                        int64_t n_sorted, const int* __restrict__ cell_start,
+                        const int64_t* __restrict__ cloud_off, const int* __restrict__ point_cloud,
+                        float4* __restrict__ pts, float4* __restrict__ nrm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_sorted) return;
+  const int c = slot_cell[j], key = slot_raw[j];
   const int b = cell_start[c], e = cell_start[c + 1];
-  if (e - b < 2) return;
-  for (int i = b + 1; i < e; ++i) {
-    const float4 p = pts[i];
-    const float4 q = nrm ? nrm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int key = __float_as_int(p.w);
-    int j = i - 1;
-    while (j >= b && __float_as_int(pts[j].w) > key) {
-      pts[j + 1] = pts[j];
-      if (nrm) nrm[j + 1] = nrm[j];
-      --j;
-    }
-    pts[j + 1] = p;
-    if (nrm) nrm[j + 1] = q;
-  }
+  int rank = 0;
+  for (int k = b; k < e; ++k) rank += slot_raw[k] < key ? 1 : 0;
+  float4 p = raw[key];
+  p.w = __int_as_float((int)(key - cloud_off[point_cloud[key]]));
+  pts[b + rank] = p;
+  if (raw_nrm) nrm[b + rank] = raw_nrm[key];
 }
 
 // Bounding-volume tree over each cloud's cell-sorted points: leaf i of cloud c
@@ -294,6 +290,22 @@ __global__ void k_bvh_up(const GridDesc* __restrict__ grids, int step, float4* _
   N[2 * node + 1] = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
 }
 
+// sorted-order normals back to input order: raw[cloud_off[c] + id] = nrm[j]
+__global__ void k_unsort_normals(const float4* __restrict__ pts, const float4* __restrict__ nrm,
+                                 const int* __restrict__ cloud_of_sorted,
+                                 const int64_t* __restrict__ cloud_off, int64_t n,
+                                 float4* __restrict__ raw_nrm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int id = __float_as_int(pts[j].w);
+  raw_nrm[cloud_off[cloud_of_sorted[j]] + id] = nrm[j];
+}
+
+__global__ void k_fill_f4(float4* __restrict__ p, int64_t n, float4 v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -325,6 +337,17 @@ void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const f
   }
 }
 
+void launch_unsort_normals(const float4* pts, const float4* nrm, const int* cloud_of_sorted,
+                           const int64_t* cloud_off, int64_t n_sorted, float4* raw_nrm,
+                           int64_t n_raw, hipStream_t s) {
+  if (n_raw > 0)
+    hipLaunchKernelGGL(k_fill_f4, dim3(blocks_for(n_raw, 256)), dim3(256), 0, s, raw_nrm, n_raw,
+                       make_float4(NAN, NAN, NAN, NAN));
+  if (n_sorted > 0)
+    hipLaunchKernelGGL(k_unsort_normals, dim3(blocks_for(n_sorted, 256)), dim3(256), 0, s, pts, nrm,
+                       cloud_of_sorted, cloud_off, n_sorted, raw_nrm);
+}
+
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
@@ -354,19 +377,16 @@ void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStre
   hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, s, out, n, tmp);
 }
 
-void launch_scatter(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n,
-                    const int* cell_start, int* fill, float4* pts, float4* nrm,
-                    const int64_t* cloud_off, const int* point_cloud, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, raw, raw_nrm, cid, n,
-                     cell_start, fill, pts, nrm, cloud_off, point_cloud);
-}
-
-void launch_cell_sort(const int* cell_start, int64_t ncells_total, float4* pts, float4* nrm,
-                      hipStream_t s) {
-  if (ncells_total <= 0) return;
-  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(ncells_total, 256)), dim3(256), 0, s,
-                     cell_start, ncells_total, pts, nrm);
+void launch_place_points(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n_raw,
+                         int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
+                         int* slot_cell, const int64_t* cloud_off, const int* point_cloud,
+                         float4* pts, float4* nrm, hipStream_t s) {
+  if (n_raw <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n_raw, 256)), dim3(256), 0, s, cid, n_raw,
+                     cell_start, fill, slot_raw, slot_cell);
+  if (n_sorted <= 0) return;
+  hipLaunchKernelGGL(k_place, dim3(blocks_for(n_sorted, 256)), dim3(256), 0, s, raw, raw_nrm,
+                     slot_raw, slot_cell, n_sorted, cell_start, cloud_off, point_cloud, pts, nrm);
 }
 
 }  // namespace lcr

@@ -124,15 +124,22 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     double v[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = 0.0;
-    if (local < gs.npts) {
-      const float4 s = b.pts[gs.pt_off + local];
-      float qx, qy, qz;
-      xform(st->T, s, qx, qy, qz);
-      const float r = (float)sqrt(pd.max_corr_d2);
-      const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
+    const bool own = local < gs.npts;
+    const float r = (float)sqrt(pd.max_corr_d2);
+    const bool bounded = isfinite(r);
+    const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    NNResult nn{INFINITY, 0x7fffffff, -1, false};
+    if (own) {
+      xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
       const int seed = b.prev ? b.prev[slot] : -1;
-      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r,
-                                   isfinite(r), seed);
+      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded, seed,
+                    2, true);
+    }
+    const float rlim = r * 1.00001f;
+    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
+                nn.id, nn.j);
+    if (own) {
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
         const float4 t = b.pts[nn.j];
@@ -206,17 +213,19 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
     const GridDesc gt = b.grids[pd.tgt];
     const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
     double v[2] = {0.0, 0.0};
-    if (local < gs.npts) {
-      const float4 s = b.pts[gs.pt_off + local];
-      float qx, qy, qz;
-      xform(st->T, s, qx, qy, qz);
+    const bool own = local < gs.npts;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    NNResult nn{INFINITY, 0x7fffffff, -1, false};
+    if (own) {
+      xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
       const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
-      const NNResult nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz,
-                                   INFINITY, false, seed);
-      if (nn.j >= 0 && (double)nn.d2 <= max_range) {
-        v[0] = nn.d2;
-        v[1] = 1.0;
-      }
+      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, INFINITY, false,
+                    seed, 2, true);
+    }
+    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, INFINITY, nn.d2, nn.id, nn.j);
+    if (own && nn.j >= 0 && (double)nn.d2 <= max_range) {
+      v[0] = nn.d2;
+      v[1] = 1.0;
     }
     block_sum<2>(v, lds);
     if (threadIdx.x == 0) {

@@ -19,11 +19,15 @@ void launch_occupancy(const GridDesc* grids, int nclouds, const int* counts, int
                       hipStream_t s);
 int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
-void launch_scatter(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n,
-                    const int* cell_start, int* fill, float4* pts, float4* nrm,
-                    const int64_t* cloud_off, const int* point_cloud, hipStream_t s);
-void launch_cell_sort(const int* cell_start, int64_t ncells_total, float4* pts, float4* nrm,
-                      hipStream_t s);
+// Places every finite raw point at its cell-sorted position (cells in order,
+// points of a cell by original index); fill must be zero, slot_* hold n_sorted ints.
+void launch_place_points(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n_raw,
+                         int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
+                         int* slot_cell, const int64_t* cloud_off, const int* point_cloud,
+                         float4* pts, float4* nrm, hipStream_t s);
+void launch_unsort_normals(const float4* pts, const float4* nrm, const int* cloud_of_sorted,
+                           const int64_t* cloud_off, int64_t n_sorted, float4* raw_nrm,
+                           int64_t n_raw, hipStream_t s);
 // bounding-volume trees (GridDesc::node_off / depth set by the host)
 void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const float4* pts,
                       float4* nodes, hipStream_t s);
@@ -69,7 +73,8 @@ int icp_max_blocks_per_cu(int tile, bool p2pl);
 // ---- K7 normals (normals.hip)
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
-                    int flip_all, float4* nrm_sorted, hipStream_t s);
+                    int flip_all, double* mom, int* nsel, int* deferred, float4* nrm_sorted,
+                    hipStream_t s);
 
 // ---- K9 pose-proposal network (encoder.hip)
 struct EncBuffers {

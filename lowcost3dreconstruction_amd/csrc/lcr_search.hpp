@@ -100,9 +100,10 @@ LCR_SHD void for_ring_runs(const GridDesc& g, const CellPos& c, int k, F&& f) {
 }
 
 struct NNResult {
-  float d2;  // FLANN-form squared distance (+inf if none)
-  int id;    // original index of the match (INT_MAX if none)
-  int j;     // position of the match in the sorted point array (-1 if none)
+  float d2;   // FLANN-form squared distance (+inf if none)
+  int id;     // original index of the match (INT_MAX if none)
+  int j;      // position of the match in the sorted point array (-1 if none)
+  bool tree;  // deferred: finish with a tree traversal from (d2, id, j)
 };
 
 // Lower bound, in cell units, of the distance from the query coordinate f (cell
@@ -194,11 +195,15 @@ LCR_SHD void bvh_nn(const GridDesc& g, const float4* __restrict__ nodes,
 // After `ring_limit` rings without certification the rest of the search runs as a
 // tree traversal (bvh_nn) seeded with the best point found so far, so queries far
 // from the cloud cost O(log N) instead of O(distance^3) cells.
+//
+// With `defer_tree`, the tree part is left to the caller (NNResult::tree), which
+// runs it for a whole wavefront at once (bvh_nn_wave); a query that found nothing
+// in its first ring is deferred straight away.
 LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
                           const float4* __restrict__ P, const float4* __restrict__ nodes,
                           float qx, float qy, float qz, float r, bool bounded, int seed_j = -1,
-                          int ring_limit = 2) {
-  NNResult res{INFINITY, 0x7fffffff, -1};
+                          int ring_limit = 2, bool defer_tree = false) {
+  NNResult res{INFINITY, 0x7fffffff, -1, false};
   if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
   const CellPos c = cell_pos(g, qx, qy, qz);
   const int k0 = first_ring(g, c);
@@ -213,7 +218,10 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   if (seed_j >= 0) scan_points(P, seed_j, seed_j + 1, qx, qy, qz, best, bid, bj);
   const float h2 = g.h * g.h;
   if (nodes && k0 > ring_limit) {  // far outside the grid: straight to the tree
-    bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
+    if (defer_tree)
+      res.tree = true;
+    else
+      bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
     res.d2 = best;
     res.id = bid;
     res.j = bj;
@@ -268,8 +276,11 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
     const float L = (Lc - c.slack) * g.h;
     if (L > 0.0f && best < L * L * 0.99998f) break;
     if (bounded && L > rlim) break;
-    if (nodes && k - k0 >= ring_limit) {
-      bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
+    if (nodes && (k - k0 >= ring_limit || (defer_tree && best == INFINITY))) {
+      if (defer_tree)
+        res.tree = true;
+      else
+        bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
       break;
     }
   }
@@ -277,6 +288,55 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   res.id = bid;
   res.j = bj;
   return res;
+}
+
+// bvh_nn for the lanes of a wavefront at once (`active` lanes; every lane of the
+// wavefront must call it).  The traversal is shared: a subtree is entered when any
+// active lane may hold a closer point in it, and near-first order follows the
+// majority of the active lanes.  The node index is wave-uniform, so node boxes
+// and leaf points are read once per wavefront.  Queries of one wavefront are
+// neighbouring cell-sorted source points, so their paths mostly coincide.
+__device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const float4* __restrict__ nodes,
+                                            const float4* __restrict__ P, bool active, float qx,
+                                            float qy, float qz, float limit2, float& best,
+                                            int& bid, int& bj) {
+  const uint64_t live = __ballot(active);
+  if (live == 0) return;
+  const int nlive = __popcll(live);
+  const float4* N = nodes + 2 * g.node_off;
+  const unsigned first_leaf = 1u << g.depth;
+  const int end = (int)g.pt_off + g.npts;
+  unsigned i = 1, order = 0;
+  int depth = 0;
+  for (;;) {
+    const float bound = fminf(best, limit2) * 1.00002f;
+    bool descend = __ballot(active && box_d2(N[2 * i], N[2 * i + 1], qx, qy, qz) <= bound) != 0;
+    if (descend && i >= first_leaf) {
+      const int s = (int)g.pt_off + (int)(i - first_leaf) * kLeaf;
+      if (active) scan_points(P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, bj);
+      descend = false;
+    }
+    if (descend) {
+      const unsigned l = 2 * i;
+      const float dl = box_d2(N[2 * l], N[2 * l + 1], qx, qy, qz);
+      const float dr = box_d2(N[2 * l + 2], N[2 * l + 3], qx, qy, qz);
+      const bool right_first = 2 * __popcll(__ballot(active && dr < dl)) > nlive;
+      ++depth;
+      order = right_first ? (order | (1u << depth)) : (order & ~(1u << depth));
+      i = right_first ? l + 1 : l;
+      continue;
+    }
+    for (;;) {
+      if (depth == 0) return;
+      const unsigned right_first = (order >> depth) & 1u;
+      if ((i & 1u) == right_first) {
+        i ^= 1u;
+        break;
+      }
+      i >>= 1;
+      --depth;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ reductions

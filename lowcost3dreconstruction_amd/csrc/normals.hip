@@ -8,16 +8,19 @@
 // (normal_estimation.cpp:112-118).  Fewer than 3 neighbours -> NaN normal.
 //
 // One wavefront per query, 64 consecutive cell-sorted queries per wavefront:
-//   1. gather the FLANN-form d2 of every point in rings 0..1 around the query's
-//      cell into LDS (coalesced runs of cell-sorted points);
-//   2. find the exact K-th smallest d2 by bisection over the float bit pattern
-//      (count via wave ballots), resolving equal distances by original index;
-//   3. certify: the K-th distance must be below the distance to the searched
-//      block's boundary, else grow the block by one ring and repeat;
+//   1. stage every point of the 3x3x3 cells around the query's cell in LDS (one
+//      lane per row of cells finds the runs, then a flat coalesced copy); the
+//      next query in the same cell reuses the staged block;
+//   2. find the exact K-th smallest FLANN-form d2 by bisection over the float bit
+//      pattern (count via wave ballots), resolving equal distances by original
+//      index;
+//   3. certify: the K-th distance must be below the distance to the staged
+//      block's boundary, else stage the next larger block and repeat;
 //   4. accumulate FP64 moments of the selected points relative to the query.
-// After its 64 queries each lane solves one FP64 3x3 eigenproblem, so the solve
-// runs 64-wide.  Blocks whose candidates overflow LDS take a slower exact path that
-// rescans the block from memory for every bisection step.
+// A second pass solves one FP64 3x3 eigenproblem per lane from the stored moment
+// sums (keeping the solver's registers out of the search).  Queries whose candidate
+// block overflows LDS are listed and finished by a rescan pass that reads the block
+// from memory for every count.
 #include <hip/hip_runtime.h>
 
 #include "lcr_common.hpp"
@@ -149,47 +152,105 @@ __device__ void select_rescan(const GridDesc& g, const int* cs, const float4* pt
   });
 }
 
-__device__ void select_lds(const GridDesc& g, const int* cs, const float4* pts, const float4& q,
-                           int kk, unsigned* sd, int* sj, Moments& acc) {
+// The wave's staged candidate block: every point of the (2K+1)^3 cells around
+// `cell` (a global cell index), copied to LDS.  Consecutive cell-sorted queries
+// mostly share a cell, so most queries reuse the block without touching memory.
+struct Staged {
+  int64_t cell;
+  int K, cnt;
+  bool ovf;
+};
+
+// Stages the points of block K around cp into sp.  Each lane looks up one
+// x-contiguous row of cells (all cell_start loads in flight at once), a wave
+// prefix sum places the rows, then the points are copied four per lane at a time.
+__device__ __forceinline__ void stage_block(const GridDesc& g, const int* cs, const float4* __restrict__ pts,
+                            const CellPos& cp, int K, float4* sp, Staged& st) {
+  const int lane = threadIdx.x & 63;
+  wave_lds_sync();  // the previous block's reads come before these writes
+  st.K = K;
+  st.cnt = 0;
+  st.ovf = false;
+  const int z0 = max(cp.cz - K, 0), z1 = min(cp.cz + K, g.nz - 1);
+  const int y0 = max(cp.cy - K, 0), y1 = min(cp.cy + K, g.ny - 1);
+  const int x0 = max(cp.cx - K, 0), x1 = min(cp.cx + K, g.nx - 1);
+  if (z0 > z1 || y0 > y1 || x0 > x1) return;
+  const int nyb = y1 - y0 + 1, nrows = (z1 - z0 + 1) * nyb;
+  int cnt = 0;
+  for (int rb = 0; rb < nrows; rb += 64) {
+    const int r = rb + lane;
+    int s = 0, len = 0;
+    if (r < nrows) {
+      const int z = z0 + r / nyb, y = y0 + r % nyb;
+      const int row = (z * g.ny + y) * g.nx;
+      s = cs[row + x0];
+      len = cs[row + x1 + 1] - s;
+    }
+    int inc = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    const int total = __shfl(inc, 63, 64);
+    if (cnt + total > kCap) {
+      st.ovf = true;
+      return;
+    }
+    const int excl = inc - len;
+    const int nr = min(64, nrows - rb);
+    for (int t0 = 0; t0 < total; t0 += 256) {
+      // positions past the end are clamped to the last one (loaded, not stored)
+      int tt[4], jb[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) tt[u] = min(t0 + lane + 64 * u, total - 1);
+      // row of position t: the last row whose first position is <= t
+      for (int rr = 0; rr < nr; ++rr) {
+        const int ex = __builtin_amdgcn_readlane(excl, rr);
+        const int sr = __builtin_amdgcn_readlane(s, rr);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (ex <= tt[u]) jb[u] = sr - ex;
+      }
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = pts[jb[u] + tt[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (t0 + lane + 64 * u < total) sp[cnt + t0 + lane + 64 * u] = v[u];
+    }
+    cnt += total;
+  }
+  st.cnt = cnt;
+  wave_lds_sync();
+}
+
+// Returns false (acc untouched) when the candidates overflow LDS; the query is
+// then left to the rescan pass.
+__device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, const float4* pts,
+                                              const float4& q, int kk, float4* sp, Staged& st,
+                                              Moments& acc) {
   const int lane = threadIdx.x & 63;
   const CellPos cp = cell_pos(g, q.x, q.y, q.z);
-  int cnt = 0;
-  bool ovf = false;
-  auto gather = [&](int ring) {
-    for_ring_runs(g, cp, ring, [&](int c0, int c1) {
-      if (ovf) return;
-      const int s = cs[c0], e = cs[c1 + 1], len = e - s;
-      if (cnt + len > kCap) {
-        ovf = true;
-        return;
-      }
-      for (int t = lane; t < len; t += 64) {
-        sd[cnt + t] = __float_as_uint(flann_d2(q.x, q.y, q.z, pts[s + t]));
-        sj[cnt + t] = s + t;
-      }
-      cnt += len;
-    });
-    wave_lds_sync();
-  };
-  gather(0);
-  gather(1);
-  int K = 1;
+  const int64_t cell = (int64_t)g.cell_off + ((int64_t)cp.cz * g.ny + cp.cy) * g.nx + cp.cx;
+  if (st.cell != cell) {
+    st.cell = cell;
+    stage_block(g, cs, pts, cp, 1, sp, st);
+  }
+  int K = st.K;
   unsigned slot[kSlots];
   unsigned T = 0;
   for (;;) {
-    if (ovf) {
-      select_rescan(g, cs, pts, q, cp, K, kk, acc);
-      return;
-    }
+    if (st.ovf) return false;
     const float Lc = block_gap(g, cp, K);
     const bool covers = (Lc == INFINITY);
-    if (cnt >= kk) {
-      const int nslots = (cnt + 63) >> 6;
+    if (st.cnt >= kk) {
 #pragma unroll
       for (int m = 0; m < kSlots; ++m) {
         const int i = lane + 64 * m;
-        slot[m] = (m < nslots && i < cnt) ? sd[i] : 0xffffffffu;
+        slot[m] = i < st.cnt ? __float_as_uint(flann_d2(q.x, q.y, q.z, sp[i])) : 0xffffffffu;
       }
+      const int nslots = (st.cnt + 63) >> 6;
       unsigned lo = 0u, hi = 0x7f800000u;
       while (lo < hi) {
         const unsigned mid = lo + ((hi - lo) >> 1);
@@ -207,9 +268,9 @@ __device__ void select_lds(const GridDesc& g, const int* cs, const float4* pts, 
       if (covers || (L > 0.0f && __uint_as_float(T) < L * L * 0.99998f)) break;
     }
     ++K;
-    gather(K);
+    stage_block(g, cs, pts, cp, K, sp, st);
   }
-  const int nslots = (cnt + 63) >> 6;
+  const int nslots = (st.cnt + 63) >> 6;
   int clt = 0, cle = 0;
 #pragma unroll
   for (int m = 0; m < kSlots; ++m)
@@ -219,12 +280,13 @@ __device__ void select_lds(const GridDesc& g, const int* cs, const float4* pts, 
     }
   int idcut = 0x7fffffff;
   if (cle > kk) {
+    // equal distances at the threshold: keep the smallest original indices
     const int need = kk - clt;
     int ids[kSlots];
 #pragma unroll
     for (int m = 0; m < kSlots; ++m) {
       const int i = lane + 64 * m;
-      ids[m] = (m < nslots && i < cnt && slot[m] == T) ? __float_as_int(pts[sj[i]].w) : 0x7fffffff;
+      ids[m] = (i < st.cnt && slot[m] == T) ? __float_as_int(sp[i].w) : 0x7fffffff;
     }
     unsigned lo = 0u, hi = 0x7fffffffu;
     while (lo < hi) {
@@ -243,98 +305,159 @@ __device__ void select_lds(const GridDesc& g, const int* cs, const float4* pts, 
 #pragma unroll
   for (int m = 0; m < kSlots; ++m) {
     const int i = lane + 64 * m;
-    if (m < nslots && i < cnt) {
-      const unsigned d = slot[m];
-      if (d < T || d == T) {
-        const float4 p = pts[sj[i]];
-        if (d < T || __float_as_int(p.w) <= idcut) add_moment(acc, q, p);
-      }
+    if (i < st.cnt && slot[m] <= T) {
+      const float4 p = sp[i];
+      if (slot[m] < T || __float_as_int(p.w) <= idcut) add_moment(acc, q, p);
     }
   }
+  return true;
 }
 
+// Search pass: the FP64 moment sums of each point's K-neighbourhood, 9 doubles
+// per point (and the neighbour count), in cell-sorted order.
 __global__ void __launch_bounds__(kWaves * 64)
-k_normals(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
-          const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted, int64_t total,
-          int k, const float* __restrict__ vp, int flip_all, float4* __restrict__ out) {
-  __shared__ unsigned s_d[kWaves][kCap];
-  __shared__ int s_j[kWaves][kCap];
+k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
+                 const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
+                 int64_t total, int k, double* __restrict__ mom, int* __restrict__ nsel,
+                 int* __restrict__ deferred) {
+  __shared__ float4 s_p[kWaves][kCap];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 64;
   if (base >= total) return;
-  Moments mine;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) mine.v[i] = 0.0;
+  const int64_t jq = base + lane;
+  const bool own = jq < total;
+  const float4 qmine = own ? pts[jq] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int cmine = own ? cloud_of_sorted[jq] : -1;
+  const int nq = (int)min<int64_t>(64, total - base);
+  Staged st;
+  st.cell = -1;
+  st.K = 0;
+  st.cnt = 0;
+  st.ovf = false;
   int mine_n = 0;
-  for (int qi = 0; qi < 64; ++qi) {
-    const int64_t j = base + qi;
-    if (j >= total) break;
+  for (int qi = 0; qi < nq; ++qi) {
+    float4 q;
+    q.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.x), qi));
+    q.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.y), qi));
+    q.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.z), qi));
+    q.w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.w), qi));
+    const GridDesc g = grids[__builtin_amdgcn_readlane(cmine, qi)];
+    const int kk = min(k, g.npts);
+    Moments acc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
+    if (kk >= 3 && !select_staged(g, cell_start + g.cell_off, pts, q, kk, s_p[w], st, acc)) {
+      // deferred[0] counts the list deferred[1..]
+      if (lane == 0) deferred[1 + atomicAdd(deferred, 1)] = (int)(base + qi);
+      if (lane == qi) mine_n = kk;
+      continue;
+    }
+    double mv = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double s = wave_sum(acc.v[i]);
+      if (lane == i) mv = s;
+    }
+    if (lane < 9) mom[(base + qi) * 9 + lane] = mv;
+    if (lane == qi) mine_n = kk;
+  }
+  if (own) nsel[jq] = mine_n;
+}
+
+// Rescan pass for the deferred queries (candidate blocks too large for LDS), one
+// wavefront per query, reading the block from memory for every count.
+__global__ void __launch_bounds__(256)
+k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
+                const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted, int k,
+                const int* __restrict__ deferred, double* __restrict__ mom) {
+  const int lane = threadIdx.x & 63;
+  const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
+  const int count = deferred[0];
+  for (int wi = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); wi < count;
+       wi += nwaves) {
+    const int64_t j = deferred[1 + wi];
     const float4 q = pts[j];
     const GridDesc g = grids[cloud_of_sorted[j]];
     const int kk = min(k, g.npts);
     Moments acc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
-    if (kk >= 3) select_lds(g, cell_start + g.cell_off, pts, q, kk, s_d[w], s_j[w], acc);
+    select_rescan(g, cell_start + g.cell_off, pts, q, cell_pos(g, q.x, q.y, q.z), 1, kk, acc);
+    double mv = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) acc.v[i] = wave_sum(acc.v[i]);
-    if (lane == qi) {
-      mine = acc;
-      mine_n = kk;
+    for (int i = 0; i < 9; ++i) {
+      const double s = wave_sum(acc.v[i]);
+      if (lane == i) mv = s;
     }
-    wave_lds_sync();
+    if (lane < 9) mom[j * 9 + lane] = mv;
   }
-  const int64_t jq = base + lane;
+}
+
+// Solve pass, one point per lane: covariance, FP64 eigen-decomposition, normal
+// (eigenvector of the smallest eigenvalue) oriented towards the viewpoint.
+__global__ void __launch_bounds__(256)
+k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
+               const double* __restrict__ mom, const int* __restrict__ nsel, int64_t total,
+               const float* __restrict__ vp, int flip_all, float4* __restrict__ out) {
+  const int64_t jq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jq >= total) return;
-  const float4 q = pts[jq];
-  float4 o;
-  if (mine_n < 3) {
-    o = make_float4(NAN, NAN, NAN, NAN);
-  } else {
-    const double n = (double)mine_n;
-    const double mx = mine.v[0] / n, my = mine.v[1] / n, mz = mine.v[2] / n;
-    double C[3][3];
-    C[0][0] = mine.v[3] / n - mx * mx;
-    C[0][1] = mine.v[4] / n - mx * my;
-    C[0][2] = mine.v[5] / n - mx * mz;
-    C[1][1] = mine.v[6] / n - my * my;
-    C[1][2] = mine.v[7] / n - my * mz;
-    C[2][2] = mine.v[8] / n - mz * mz;
-    C[1][0] = C[0][1];
-    C[2][0] = C[0][2];
-    C[2][1] = C[1][2];
-    double wv[3], V[3][3];
-    eigh3(C, wv, V);
-    double nx = V[0][0], ny = V[1][0], nz = V[2][0];
-    const double sum = wv[0] + wv[1] + wv[2];
-    const double curv = sum != 0.0 ? fabs(wv[0] / sum) : 0.0;
-    const float* v = vp + 3 * cloud_of_sorted[jq];
-    const double dot = ((double)v[0] - q.x) * nx + ((double)v[1] - q.y) * ny + ((double)v[2] - q.z) * nz;
-    if (dot < 0.0) {
-      nx = -nx;
-      ny = -ny;
-      nz = -nz;
-    }
-    if (flip_all) {
-      nx = -nx;
-      ny = -ny;
-      nz = -nz;
-    }
-    o = make_float4((float)nx, (float)ny, (float)nz, (float)curv);
+  const int cnt = nsel[jq];
+  if (cnt < 3) {
+    out[jq] = make_float4(NAN, NAN, NAN, NAN);
+    return;
   }
-  out[jq] = o;
+  const float4 q = pts[jq];
+  double m[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) m[i] = mom[jq * 9 + i];
+  const double n = (double)cnt;
+  const double mx = m[0] / n, my = m[1] / n, mz = m[2] / n;
+  double C[3][3];
+  C[0][0] = m[3] / n - mx * mx;
+  C[0][1] = m[4] / n - mx * my;
+  C[0][2] = m[5] / n - mx * mz;
+  C[1][1] = m[6] / n - my * my;
+  C[1][2] = m[7] / n - my * mz;
+  C[2][2] = m[8] / n - mz * mz;
+  C[1][0] = C[0][1];
+  C[2][0] = C[0][2];
+  C[2][1] = C[1][2];
+  double wv[3], V[3][3];
+  eigh3(C, wv, V);
+  double nx = V[0][0], ny = V[1][0], nz = V[2][0];
+  const double sum = wv[0] + wv[1] + wv[2];
+  const double curv = sum != 0.0 ? fabs(wv[0] / sum) : 0.0;
+  const float* v = vp + 3 * cloud_of_sorted[jq];
+  const double dot = ((double)v[0] - q.x) * nx + ((double)v[1] - q.y) * ny + ((double)v[2] - q.z) * nz;
+  if (dot < 0.0) {
+    nx = -nx;
+    ny = -ny;
+    nz = -nz;
+  }
+  if (flip_all) {
+    nx = -nx;
+    ny = -ny;
+    nz = -nz;
+  }
+  out[jq] = make_float4((float)nx, (float)ny, (float)nz, (float)curv);
 }
 
 }  // namespace
 
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
-                    int flip_all, float4* nrm_sorted, hipStream_t s) {
+                    int flip_all, double* mom, int* nsel, int* deferred, float4* nrm_sorted,
+                    hipStream_t s) {
   if (npts_total <= 0) return;
   const int64_t per_block = (int64_t)kWaves * 64;
   const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
-  hipLaunchKernelGGL(k_normals, dim3(blocks), dim3(kWaves * 64), 0, s, grids, cell_start, pts,
-                     cloud_of_sorted, npts_total, k, vp, flip_all, nrm_sorted);
+  launch_fill_i32(deferred, 1, 0, s);
+  hipLaunchKernelGGL(k_normal_moments, dim3(blocks), dim3(kWaves * 64), 0, s, grids, cell_start,
+                     pts, cloud_of_sorted, npts_total, k, mom, nsel, deferred);
+  hipLaunchKernelGGL(k_normal_rescan, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
+                     cloud_of_sorted, k, deferred, mom);
+  hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s,
+                     pts, cloud_of_sorted, mom, nsel, npts_total, vp, flip_all, nrm_sorted);
 }
 
 }  // namespace lcr

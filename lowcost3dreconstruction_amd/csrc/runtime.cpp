@@ -300,11 +300,13 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
                     total_raw, d_cid.as<int>(), d_counts.as<int>(), nullptr, s);
   launch_exclusive_scan(d_counts.as<int>(), d_cell_start.as<int>(), cells, d_scan_tmp.as<int>(), s);
   launch_fill_i32(d_counts.as<int>(), cells, 0, s);
-  launch_scatter(d_raw.as<float4>(), has_nrm ? d_raw_nrm.as<float4>() : nullptr, d_cid.as<int>(),
-                 total_raw, d_cell_start.as<int>(), d_counts.as<int>(), d_pts.as<float4>(),
-                 d_nrm.as<float4>(), d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), s);
-  launch_cell_sort(d_cell_start.as<int>(), cells - 1, d_pts.as<float4>(),
-                   has_nrm ? d_nrm.as<float4>() : nullptr, s);
+  d_slot_raw.reserve(npts * sizeof(int));
+  d_slot_cell.reserve(npts * sizeof(int));
+  launch_place_points(d_raw.as<float4>(), has_nrm ? d_raw_nrm.as<float4>() : nullptr,
+                      d_cid.as<int>(), total_raw, total_pts, d_cell_start.as<int>(),
+                      d_counts.as<int>(), d_slot_raw.as<int>(), d_slot_cell.as<int>(),
+                      d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), d_pts.as<float4>(),
+                      d_nrm.as<float4>(), s);
   d_nodes.reserve((size_t)std::max<int64_t>(total_nodes, 1) * 2 * sizeof(float4));
   launch_bvh_build(d_grids.as<GridDesc>(), n, max_depth, d_pts.as<float4>(), d_nodes.as<float4>(), s);
   // sorted point -> cloud
@@ -327,12 +329,23 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
     for (int a = 0; a < 3; ++a) v[3 * c + a] = vp_host ? vp_host[3 * c + a] : 0.0f;
   LCR_HIP(hipMemcpyAsync(d_vp.p, v, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
   const int span = prof.begin("normals", s, spans);
+  const size_t np = (size_t)std::max<int64_t>(total_pts, 1);
+  d_mom.reserve(np * 9 * sizeof(double));
+  d_nsel.reserve(np * sizeof(int));
+  d_defer.reserve((np + 1) * sizeof(int));
   launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
                  d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
-                 d_nrm.as<float4>(), s);
+                 d_mom.as<double>(), d_nsel.as<int>(), d_defer.as<int>(), d_nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
   LCR_HIP(hipStreamSynchronize(s));
+}
+
+void CloudSet::normals_to_raw(hipStream_t s) {
+  d_raw_nrm.reserve((size_t)std::max<int64_t>(total_raw, 1) * sizeof(float4));
+  launch_unsort_normals(d_pts.as<float4>(), d_nrm.as<float4>(), d_cloud_of_sorted.as<int>(),
+                        d_cloud_off.as<int64_t>(), total_pts, d_raw_nrm.as<float4>(), total_raw, s);
+  has_nrm = true;
 }
 
 // ------------------------------------------------------------------ engine

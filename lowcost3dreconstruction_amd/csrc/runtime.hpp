@@ -121,7 +121,8 @@ struct CloudSet {
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
   DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
-  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes;
+  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_mom, d_nsel, d_defer;
+  DevBuf d_slot_raw, d_slot_cell;
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
 
   // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;
@@ -134,6 +135,9 @@ struct CloudSet {
   // K7 normals for every cloud into d_nrm (cell-sorted order).
   void compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
                        std::vector<Profiler::Span>* spans);
+  // Copies the cell-sorted normals back to input order (d_raw_nrm), so the next
+  // build() carries them into its own order.
+  void normals_to_raw(hipStream_t s);
 };
 
 // A batch of ICP instances over a CloudSet.
