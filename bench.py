@@ -47,6 +47,9 @@ def make_workload(seed):
     return cyc
 
 
+PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
 def icp_kw():
     return dict(max_corr_dist=0.01, max_iterations=60, transformation_epsilon=1e-9,
                 euclidean_fitness_epsilon=1e-3)
@@ -123,6 +126,11 @@ def main():
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-pairs", type=int, default=36)
+    ap.add_argument("--precision", choices=sorted(PRECISION), default="fp32",
+                    help="correspondence search of the measured run: exact FP32 scan, or "
+                         "FP16/BF16 matrix-core distance tiles with exact FP32 rerank")
+    ap.add_argument("--compare-precisions", type=int, default=3, metavar="STEPS",
+                    help="also time every precision mode for STEPS steps (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -146,7 +154,7 @@ def main():
 
     # ---------------- batched cycle (the measured path)
     cycle = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=cyc.meta["prior"], point_to_plane=True,
-                      normals_k=30, **icp_kw())
+                      normals_k=30, precision_mode=PRECISION[args.precision], **icp_kw())
     for _ in range(args.warmup):
         cycle.run()
     t_start = time.perf_counter()
@@ -173,6 +181,29 @@ def main():
         f"per step: icp {iter_ms / args.steps:.3f} ms ({n_iter_launches // args.steps} launches), "
         f"fitness {fit_ms / args.steps:.3f} ms, grids {grid_ms / args.steps:.3f} ms, "
         f"normals {nrm_ms / args.steps:.3f} ms")
+
+    # every precision mode on the same cycle (results must agree bitwise)
+    modes = {}
+    if args.compare_precisions > 0:
+        for name, mode in sorted(PRECISION.items()):
+            cm = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=cyc.meta["prior"], point_to_plane=True,
+                           normals_k=30, precision_mode=mode, **icp_kw())
+            cm.run()
+            lcr.prof_reset()
+            lcr.prof_enable(True)
+            t_m = time.perf_counter()
+            for _ in range(args.compare_precisions):
+                cm.run()
+            ms = (time.perf_counter() - t_m) / args.compare_precisions * 1e3
+            lcr.prof_enable(False)
+            _, it_ms = lcr.prof_read("icp_iteration")
+            same = bool(np.array_equal(cm.results()["T"], res["T"]))
+            modes[name] = {"profiled_ms_per_step": ms,
+                           "icp_iteration_ms_per_step": it_ms / args.compare_precisions,
+                           "same_transforms_as_measured_run": same}
+            log(f"precision {name}: profiled step {ms:.3f} ms, icp {modes[name]['icp_iteration_ms_per_step']:.3f} ms, "
+                f"identical results: {same}")
+            del cm
 
     # algorithmic bytes of the iteration kernel: per instance and executed
     # iteration 12 B per source point + 24 B per target point (xyz + normal)
@@ -242,6 +273,7 @@ def main():
             "points_total": int(sum(sizes)), "pairs": int(len(cyc.pairs)),
             "max_iterations": 60, "max_corr_dist_m": 0.01, "normals_k": 30, "seed": args.seed,
             "initial_pose": "turntable prior (10 deg yaw about the rig axis)",
+            "correspondence_search": args.precision,
             "parallelism": "single GPU",
         },
         "aligned_pairs": n_aligned,
@@ -255,6 +287,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "icp_iteration", "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        "precision_modes": modes,
         "kernel_ms_per_step": {"icp_iteration": iter_ms / args.steps, "fitness": fit_ms / args.steps,
                                "grid_build": grid_ms / args.steps, "normals": nrm_ms / args.steps,
                                "profiled_step": prof_step_s * 1e3},

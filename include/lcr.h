@@ -40,6 +40,8 @@ extern "C" {
 
 /* correspondence search arithmetic */
 #define LCR_PREC_FP32 0 /* exact FP32 scan (FLANN distance form) */
+#define LCR_PREC_FP16 1 /* FP16 matrix-core distance tiles + exact FP32 rerank (same matches) */
+#define LCR_PREC_BF16 2 /* BF16 matrix-core distance tiles + exact FP32 rerank (same matches) */
 
 /* ICP parameters.  Mirrors the setters called at fine_registration.cpp:112-118:
  * setMaxCorrespondenceDistance, setMaximumIterations, setTransformationEpsilon,
@@ -53,7 +55,8 @@ typedef struct {
   int32_t point_to_plane;
   int32_t precision_mode; /* LCR_PREC_* */
   double cell_factor;     /* grid cell edge / estimated point spacing; <= 0 = default */
-  int32_t tile;           /* source points per work tile: 64, 128 or 256; 0 = default */
+  int32_t tile;           /* source points per work tile: 64, 128 or 256; 0 = default
+                             (LCR_PREC_FP16/BF16 always use 64) */
 } lcr_icp_params;
 
 typedef struct lcr_icp lcr_icp;
@@ -161,6 +164,26 @@ int lcr_cycle_results(lcr_cycle *c, lcr_cycle_result *r);
 int lcr_cycle_instances(lcr_cycle *c, double *T, int32_t *iterations, int32_t *state,
                         double *fitness);
 void lcr_cycle_destroy(lcr_cycle *c);
+
+/* Autotuner (no counterpart in the reference, which has no launch shape to
+ * choose).  Times repeat runs of the cycle under candidate launch settings --
+ * tile (64/128/256), cell_factor (1.5..4) and precision_mode (FP32 scan or
+ * FP16/BF16 distance tiles) -- and keeps the fastest in the cycle's parameters.
+ * Every candidate gives the same correspondences; only speed differs.  With a
+ * cache_path (nullable) the choice is looked up first and appended after
+ * tuning; entries are keyed by the device and the workload's shape
+ * (clouds, pairs, hypotheses, total points, radius, estimator). */
+typedef struct {
+  int32_t tile;
+  int32_t precision_mode;
+  double cell_factor;
+  double best_ms;      /* run time of the chosen setting (0 when from the cache) */
+  double default_ms;   /* run time of the default setting (0 when from the cache) */
+  int32_t candidates;  /* settings timed (0 when from the cache) */
+  int32_t from_cache;
+} lcr_tune_result;
+int lcr_cycle_autotune(lcr_cycle *c, const char *cache_path, lcr_tune_result *out);
+
 /* One-shot: create, run, results, destroy. */
 int lcr_icp_batch_align(const lcr_cycle_desc *d, lcr_cycle_result *r);
 

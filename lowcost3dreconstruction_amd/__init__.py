@@ -29,6 +29,8 @@ CONV_REL_MSE = 4
 CONV_NO_CORRESPONDENCES = 5
 
 PREC_FP32 = 0
+PREC_FP16 = 1  # matrix-core FP16 distance tiles + exact FP32 rerank
+PREC_BF16 = 2  # matrix-core BF16 distance tiles + exact FP32 rerank
 
 # symbols declared in include/lcr.h
 EXPORTS = (
@@ -42,7 +44,7 @@ EXPORTS = (
     "lcr_proposal_create", "lcr_proposal_destroy", "lcr_proposal_set_weights",
     "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
     "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
-    "lcr_proposal_predict",
+    "lcr_proposal_predict", "lcr_cycle_autotune",
 )
 
 
@@ -95,6 +97,18 @@ class CycleResult(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("tile", C.c_int32),
+        ("precision_mode", C.c_int32),
+        ("cell_factor", C.c_double),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
+        ("candidates", C.c_int32),
+        ("from_cache", C.c_int32),
+    ]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -142,6 +156,7 @@ def _load():
     L.lcr_proposal_synthesize.argtypes = [vp, fp, fp, dp, C.c_int32, C.c_int32, C.c_float,
                                           C.c_uint64, fp, fp]
     L.lcr_proposal_predict.argtypes = [vp, C.c_int32, C.c_int32, fp, fp]
+    L.lcr_cycle_autotune.argtypes = [vp, C.c_char_p, C.POINTER(TuneResult)]
     L.lcr_last_error.restype = C.c_char_p
     L.lcr_version.restype = C.c_char_p
     return L
@@ -187,11 +202,12 @@ class IterativeClosestPoint:
     IterativeClosestPointWithNormals-style point-to-plane ICP (point_to_plane=True),
     with PCL's defaults (max_iterations 10, no distance limit, epsilons off)."""
 
-    def __init__(self, point_to_plane=False, cell_factor=0.0, tile=0):
+    def __init__(self, point_to_plane=False, cell_factor=0.0, tile=0, precision_mode=PREC_FP32):
         self._p = default_params()
         self._p.point_to_plane = int(bool(point_to_plane))
         self._p.cell_factor = float(cell_factor)
         self._p.tile = int(tile)
+        self._p.precision_mode = int(precision_mode)
         h = C.c_void_p()
         _check(lib.lcr_icp_create(C.byref(self._p), C.byref(h)))
         self._h = h
@@ -323,7 +339,7 @@ class Cycle:
     def __init__(self, clouds, pairs, normals=None, guesses=None, n_hyp=1, max_corr_dist=0.05,
                  max_iterations=60, transformation_epsilon=1e-9, euclidean_fitness_epsilon=1e-3,
                  point_to_plane=True, normals_k=30, viewpoints=None, cell_factor=0.0, tile=0,
-                 proposal=None, proposal_points=8192):
+                 proposal=None, proposal_points=8192, precision_mode=0):
         self.clouds = [_f32(c) for c in clouds]
         self.normals = [_f32(n) for n in normals] if normals is not None else None
         pr = np.asarray(pairs, np.int32).reshape(-1, 2)
@@ -349,6 +365,7 @@ class Cycle:
         p.point_to_plane = int(bool(point_to_plane))
         p.cell_factor = float(cell_factor)
         p.tile = int(tile)
+        p.precision_mode = int(precision_mode)
         d = CycleDesc()
         d.n_clouds = len(self.clouds)
         d.xyz = self._xyz_arr
@@ -379,6 +396,14 @@ class Cycle:
 
     def run(self):
         _check(lib.lcr_cycle_run(self._h))
+
+    def autotune(self, cache_path=None):
+        """Times candidate (tile, cell_factor, precision_mode) settings on this cycle
+        and keeps the fastest (results are the same for all of them)."""
+        r = TuneResult()
+        path = None if cache_path is None else os.fsencode(cache_path)
+        _check(lib.lcr_cycle_autotune(self._h, path, C.byref(r)))
+        return {f: getattr(r, f) for f, _ in TuneResult._fields_}
 
     def results(self):
         P = self.n_pairs

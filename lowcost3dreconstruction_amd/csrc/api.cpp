@@ -4,7 +4,10 @@
 #include <math.h>
 #include <string.h>
 
+#include <stdio.h>
+
 #include <algorithm>
+#include <chrono>
 #include <functional>
 #include <memory>
 #include <string>
@@ -63,8 +66,13 @@ bool params_ok(const lcr_icp_params* p, std::string* why) {
     *why = "max_iterations above 100000";
     return false;
   }
-  if (p->precision_mode != LCR_PREC_FP32) {
+  if (p->precision_mode != LCR_PREC_FP32 && p->precision_mode != LCR_PREC_FP16 &&
+      p->precision_mode != LCR_PREC_BF16) {
     *why = "unsupported precision_mode";
+    return false;
+  }
+  if (p->precision_mode != LCR_PREC_FP32 && p->tile != 0 && p->tile != 64) {
+    *why = "distance tiles use tile 64";
     return false;
   }
   if (p->tile != 0 && p->tile != 64 && p->tile != 128 && p->tile != 256) {
@@ -75,13 +83,14 @@ bool params_ok(const lcr_icp_params* p, std::string* why) {
 }
 
 IcpConst make_const(const lcr_icp_params& p) {
-  IcpConst c;
+  IcpConst c{};
   c.max_iterations = p.max_iterations;
   c.point_to_plane = p.point_to_plane ? 1 : 0;
   c.rotation_threshold = 1.0 - p.transformation_epsilon;
   c.translation_threshold = p.transformation_epsilon;
   c.mse_rel = p.euclidean_fitness_epsilon;
   c.mse_abs = 1e-12;
+  c.precision = p.precision_mode;
   return c;
 }
 
@@ -269,13 +278,13 @@ static void prepare(lcr_icp* h) {
     h->cs.upload(h->stream.s, xyz, nrm_up, cnt, 3 * sizeof(float));
     h->cs.build(h->stream.s, h->params.cell_factor, nullptr);
     const double r = h->params.max_corr_dist;
-    h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 64);
+    h->eng.setup(h->cs, {0}, {1}, {r * r}, (h->params.tile && !h->params.precision_mode) ? h->params.tile : 64);
     h->dirty = false;
   } else {
     // parameters may have changed: refresh the instance radius
     const double r = h->params.max_corr_dist;
     if (h->eng.hp.size() == 1 && h->eng.hp[0].max_corr_d2 != r * r) {
-      h->eng.setup(h->cs, {0}, {1}, {r * r}, h->params.tile ? h->params.tile : 64);
+      h->eng.setup(h->cs, {0}, {1}, {r * r}, (h->params.tile && !h->params.precision_mode) ? h->params.tile : 64);
     }
   }
 }
@@ -552,8 +561,9 @@ int lcr_cycle_run(lcr_cycle* c) {
         tgt[p * H + hy] = c->tgt[p];
       }
     c->eng.c = make_const(d.params);
-    const int tile = d.params.tile ? d.params.tile : 64;
-    bool same = c->eng.npairs == ninst && c->eng.tile == tile && c->eng.hp.size() == (size_t)ninst;
+    const int tile = (d.params.tile && !d.params.precision_mode) ? d.params.tile : 64;
+    bool same = c->eng.npairs == ninst && c->eng.tile == tile && c->eng.hp.size() == (size_t)ninst &&
+                c->eng.setup_precision == c->eng.c.precision;
     if (same) {
       for (int i = 0; i < ninst && same; ++i)
         same = c->eng.hp[i].ntiles == (c->cs.grids[src[i]].npts + tile - 1) / tile;
@@ -610,6 +620,130 @@ int lcr_cycle_results(lcr_cycle* c, lcr_cycle_result* r) {
     if (r->best_hyp) r->best_hyp[p] = best;
   }
   return LCR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+std::string tune_key(const lcr_cycle* c) {
+  hipDeviceProp_t prop;
+  std::string dev = "unknown";
+  if (hipGetDeviceProperties(&prop, c->stream.device) == hipSuccess)
+    dev = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount);
+  int64_t total = 0;
+  for (const auto& x : c->xyz) total += (int64_t)(x.size() / 3);
+  char buf[512];
+  snprintf(buf, sizeof buf, "dev=%s clouds=%d pairs=%d hyp=%d points=%.2e r=%.6g p2pl=%d k=%d",
+           dev.c_str(), c->desc.n_clouds, c->desc.n_pairs, c->nhyp_total, (double)total,
+           c->desc.params.max_corr_dist, c->desc.params.point_to_plane ? 1 : 0, c->desc.normals_k);
+  return buf;
+}
+
+bool tune_lookup(const char* path, const std::string& key, lcr_tune_result* r) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  char line[1024];
+  bool found = false;
+  while (fgets(line, sizeof line, f)) {
+    char* tab = strchr(line, '\t');
+    if (!tab || std::string(line, tab - line) != key) continue;
+    int tile = 0, prec = 0;
+    double cf = 0.0;
+    if (sscanf(tab + 1, "%d %d %lf", &tile, &prec, &cf) == 3 &&
+        (tile == 64 || tile == 128 || tile == 256) && prec >= 0 && prec <= 2 && cf > 0.0) {
+      r->tile = tile;
+      r->precision_mode = prec;
+      r->cell_factor = cf;
+      found = true;  // the last entry for a key wins
+    }
+  }
+  fclose(f);
+  return found;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcr_cycle_autotune(lcr_cycle* c, const char* cache_path, lcr_tune_result* out) {
+  return guarded([&] {
+    if (!c) return fail(LCR_E_ARG, "handle is NULL");
+    lcr_tune_result r;
+    memset(&r, 0, sizeof r);
+    lcr_icp_params& P = c->desc.params;
+    const std::string key = tune_key(c);
+    if (cache_path && tune_lookup(cache_path, key, &r)) {
+      P.tile = r.tile;
+      P.precision_mode = r.precision_mode;
+      P.cell_factor = r.cell_factor;
+      r.from_cache = 1;
+      if (out) *out = r;
+      return LCR_OK;
+    }
+    struct Cand {
+      int tile, prec;
+      double cf;
+    };
+    std::vector<Cand> cands;
+    for (double cf : {1.5, 2.0, 2.5, 3.0, 4.0}) {
+      for (int tile : {64, 128, 256}) cands.push_back({tile, LCR_PREC_FP32, cf});
+      cands.push_back({64, LCR_PREC_FP16, cf});
+      cands.push_back({64, LCR_PREC_BF16, cf});
+    }
+    const lcr_icp_params saved = P;
+    int rc = LCR_OK;
+    auto time_run = [&](const Cand& k) -> double {
+      P.tile = k.tile;
+      P.precision_mode = k.prec;
+      P.cell_factor = k.cf;
+      rc = lcr_cycle_run(c);  // the first run of a setting captures its graphs
+      if (rc != LCR_OK) return -1.0;
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = lcr_cycle_run(c);
+      const auto t1 = std::chrono::steady_clock::now();
+      if (rc != LCR_OK) return -1.0;
+      return std::chrono::duration<double, std::milli>(t1 - t0).count();
+    };
+    const double dms = time_run({saved.tile ? saved.tile : 64, saved.precision_mode,
+                                 saved.cell_factor > 0.0 ? saved.cell_factor : 2.5});
+    if (dms < 0.0) {
+      P = saved;
+      return rc;
+    }
+    r.default_ms = dms;
+    double best = 1e300;
+    Cand bc{64, 0, 2.5};
+    for (const Cand& k : cands) {
+      const double ms = time_run(k);
+      if (ms < 0.0) {
+        P = saved;
+        return rc;
+      }
+      ++r.candidates;
+      if (ms < best) {
+        best = ms;
+        bc = k;
+      }
+    }
+    P = saved;
+    P.tile = bc.tile;
+    P.precision_mode = bc.prec;
+    P.cell_factor = bc.cf;
+    r.tile = bc.tile;
+    r.precision_mode = bc.prec;
+    r.cell_factor = bc.cf;
+    r.best_ms = best;
+    if (cache_path) {
+      FILE* f = fopen(cache_path, "a");
+      if (f) {
+        fprintf(f, "%s\t%d %d %.17g %.6f\n", key.c_str(), bc.tile, bc.prec, bc.cf, best);
+        fclose(f);
+      }
+    }
+    if (out) *out = r;
+    return LCR_OK;
+  });
 }
 
 void lcr_cycle_destroy(lcr_cycle* c) {

@@ -28,6 +28,7 @@
 #include "lcr_kernels.hpp"
 #include "lcr_math.hpp"
 #include "lcr_search.hpp"
+#include "tile_search.hpp"
 
 namespace lcr {
 
@@ -107,9 +108,12 @@ __device__ __forceinline__ void accumulate(double* v, float qx, float qy, float 
   }
 }
 
-template <bool P2PL, int TILE>
+// PREC 0: exact per-lane grid search; 1 / 2: FP16 / BF16 matrix-core distance
+// tiles with exact FP32 rerank (tile_search.hpp; TILE is then 64, one wavefront).
+template <bool P2PL, int TILE, int PREC>
 __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const IcpConst c) {
   constexpr int NV = P2PL ? kP2LVals : kP2PVals;
+  static_assert(PREC == 0 || TILE == 64, "distance tiles need one wavefront per tile");
   __shared__ double lds[NV * (TILE / 64)];
   __shared__ int s_last;
   const TileRange tr = xcd_tiles(b.ntiles);
@@ -130,15 +134,25 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
-    if (own) {
-      xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
-      const int seed = b.prev ? b.prev[slot] : -1;
-      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded, seed,
-                    2, true);
+    if (own) xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+    bool tiled = false;
+    if constexpr (PREC != 0) {
+      __shared__ TileLds tl;
+      const bool q_ok = own && isfinite(qx) && isfinite(qy) && isfinite(qz);
+      if (bounded)
+        tiled = tile_nn<PREC>(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r, tl,
+                              nn.d2, nn.id, nn.j);
     }
-    const float rlim = r * 1.00001f;
-    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
-                nn.id, nn.j);
+    if (!tiled) {
+      if (own) {
+        const int seed = b.prev ? b.prev[slot] : -1;
+        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded,
+                      seed, 2, true);
+      }
+      const float rlim = r * 1.00001f;
+      bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY,
+                  nn.d2, nn.id, nn.j);
+    }
     if (own) {
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
@@ -334,13 +348,23 @@ __global__ void k_transform_output(const float4* raw, const float4* raw_nrm, int
 
 template <bool P2PL, int TILE>
 void launch_iter_t(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((k_icp_iter<P2PL, TILE>), dim3(nblocks), dim3(TILE), 0, s, b, c);
+  hipLaunchKernelGGL((k_icp_iter<P2PL, TILE, 0>), dim3(nblocks), dim3(TILE), 0, s, b, c);
+}
+
+template <int PREC>
+void launch_iter_tiles(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+  if (c.point_to_plane)
+    hipLaunchKernelGGL((k_icp_iter<true, 64, PREC>), dim3(nblocks), dim3(64), 0, s, b, c);
+  else
+    hipLaunchKernelGGL((k_icp_iter<false, 64, PREC>), dim3(nblocks), dim3(64), 0, s, b, c);
 }
 
 }  // namespace
 
 void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
   const bool pl = c.point_to_plane != 0;
+  if (c.precision == 1) return launch_iter_tiles<1>(b, c, nblocks, s);
+  if (c.precision == 2) return launch_iter_tiles<2>(b, c, nblocks, s);
   switch (b.tile) {
     case 64:
       pl ? launch_iter_t<true, 64>(b, c, nblocks, s) : launch_iter_t<false, 64>(b, c, nblocks, s);
@@ -389,21 +413,31 @@ void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n
                      raw_nrm, n, st, pair, out_xyz, out_nrm);
 }
 
-int icp_max_blocks_per_cu(int tile, bool p2pl) {
+int icp_max_blocks_per_cu(int tile, bool p2pl, int precision) {
   int nb = 0;
   hipError_t e = hipSuccess;
+  if (precision == 1 || precision == 2) {
+    if (precision == 1)
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 1>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64, 1>, 64, 0);
+    else
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 2>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64, 2>, 64, 0);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb;
+  }
   switch (tile) {
     case 64:
-      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64>, 64, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64>, 64, 0);
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 0>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64, 0>, 64, 0);
       break;
     case 128:
-      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 128>, 128, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 128>, 128, 0);
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 128, 0>, 128, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 128, 0>, 128, 0);
       break;
     default:
-      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 256>, 256, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 256>, 256, 0);
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 256, 0>, 256, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 256, 0>, 256, 0);
       break;
   }
   if (e != hipSuccess || nb < 1) nb = 1;

@@ -76,6 +76,8 @@ struct IcpConst {
   double translation_threshold;  // transformation_epsilon
   double mse_rel;                // euclidean_fitness_epsilon
   double mse_abs;                // 1e-12
+  int precision;                 // correspondence search: 0 FP32, 1 FP16 / 2 BF16 tiles
+  int pad_;
 };
 
 // Flat FP32 weight layout of the pose-proposal network, row-major [in][out]:

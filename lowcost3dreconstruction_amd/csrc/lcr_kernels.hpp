@@ -68,7 +68,7 @@ void launch_correspondences(const IcpBuffers& b, int pair, int npts_src, int32_t
 void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n,
                              const PairState* st, int pair, float* out_xyz, float* out_nrm,
                              hipStream_t s);
-int icp_max_blocks_per_cu(int tile, bool p2pl);
+int icp_max_blocks_per_cu(int tile, bool p2pl, int precision);
 
 // ---- K7 normals (normals.hip)
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,

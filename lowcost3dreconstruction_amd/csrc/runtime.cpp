@@ -395,7 +395,8 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   if (ntiles > 0)
     LCR_HIP(hipMemcpy(d_tile_pair.p, tp.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
   const int cus = device_cu_count();
-  const int per_cu = icp_max_blocks_per_cu(tile, c.point_to_plane != 0);
+  const int per_cu = icp_max_blocks_per_cu(tile, c.point_to_plane != 0, c.precision);
+  setup_precision = c.precision;
   int nb = std::min(std::max(ntiles, 1), cus * per_cu);
   nb = std::max(8, (nb + 7) / 8 * 8);
   nblocks = nb;

@@ -172,6 +172,7 @@ struct Engine {
   // (src, tgt) cloud indices per instance; squared radius per instance.
   void setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
              const std::vector<double>& r2, int tile_pts);
+  int setup_precision = -1;  // c.precision the launch shape was sized for
   IcpBuffers buffers(const CloudSet& cs) const;
   // Runs state init (from guesses, 16 doubles per instance or null) and up to
   // max_iterations iterations plus the fitness pass; results land in h_state.

@@ -123,3 +123,34 @@ def test_cycle_point_to_point_and_empty_cloud(lcr):
     # empty source or target: nothing runs, not converged (Registration::initCompute)
     assert not r["converged"][1] and r["iterations"][1] == 0
     assert not r["converged"][2] and r["iterations"][2] == 0
+
+
+def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
+    """The tuner only changes the launch shape: transforms agree with the default
+    setting to rounding (the reduction order follows the tile size), and a second
+    tuner on the same workload reads its choice back from the cache."""
+    cyc, normals = small_cycle
+    print_case(test="autotune")
+    base = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=cyc.meta["prior"],
+                     point_to_plane=True, **KW)
+    base.run()
+    r0 = base.results()
+    cache = tmp_path / "tune.txt"
+    c = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=cyc.meta["prior"],
+                  point_to_plane=True, **KW)
+    t = c.autotune(str(cache))
+    assert t["from_cache"] == 0 and t["candidates"] == 25
+    assert t["tile"] in (64, 128, 256) and t["precision_mode"] in (0, 1, 2)
+    assert 0 < t["best_ms"] <= t["default_ms"] * 1.5
+    c.run()
+    r1 = c.results()
+    for a, b in zip(r0["T"], r1["T"]):
+        er, et = synth.transform_error(a, b)
+        assert er < 1e-9 and et < 1e-10
+    np.testing.assert_array_equal(r0["state"], r1["state"])
+    c2 = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=cyc.meta["prior"],
+                   point_to_plane=True, **KW)
+    t2 = c2.autotune(str(cache))
+    assert t2["from_cache"] == 1
+    assert (t2["tile"], t2["precision_mode"], t2["cell_factor"]) == \
+        (t["tile"], t["precision_mode"], t["cell_factor"])

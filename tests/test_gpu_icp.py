@@ -188,3 +188,57 @@ def test_tiles_and_cell_factors_agree(lcr, c1):
             Ts.append(icp.getFinalTransformation())
     for T in Ts[1:]:
         assert_close_T(T, Ts[0], rot=1e-9, tr=1e-10)
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("p2pl", [False, True])
+def test_distance_tiles_match_exact_search(lcr, c1, prec, p2pl):
+    """FP16/BF16 matrix-core distance tiles with the exact FP32 rerank pick the same
+    correspondences as the exact scan, so the whole ICP run is bitwise identical
+    (same tile size, so the same reduction order)."""
+    print_case(test="distance_tiles", prec=prec, p2pl=p2pl)
+    tn = oracle.normals(c1.tgt, 30)[0] if p2pl else None
+    out = []
+    for mode in (0, prec):
+        icp = make_icp(lcr, c1.src, c1.tgt, tn, p2pl=p2pl, r=0.05, iters=30, tile=64,
+                       precision_mode=mode)
+        icp.align()
+        out.append((icp.getFinalTransformation(), icp.nr_iterations(), icp.getConvergenceState(),
+                    icp.getFitnessScore()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_distance_tiles_fallback_and_small_radius(lcr, prec):
+    """A dense cloud with a radius spanning many cells overflows the staged block
+    (scalar fallback), a tiny radius stages almost nothing; both stay identical to
+    the exact search, as does a cloud in millimetre units (FP16 range check)."""
+    base = synth.pair_ellipsoids(3)
+    for scale, r in ((1.0, 0.2), (1.0, 0.002), (1000.0, 50.0)):
+        print_case(test="distance_tiles_edge", prec=prec, scale=scale, r=r)
+        src = (base.src * scale).astype(np.float32)
+        tgt = (base.tgt * scale).astype(np.float32)
+        out = []
+        for mode in (0, prec):
+            icp = make_icp(lcr, src, tgt, r=r, iters=10, tile=64, precision_mode=mode)
+            icp.align()
+            out.append((icp.getFinalTransformation(), icp.nr_iterations(), icp.getFitnessScore()))
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        assert out[0][1:] == out[1][1:]
+
+
+def test_distance_tiles_cycle_matches_exact(lcr):
+    cyc = synth.turntable_cycle(4, n_captures=4, yaw_step_deg=10.0, scale=1.5,
+                                centre=(0.0, 0.0, -0.8))
+    print_case(test="distance_tiles_cycle")
+    res = []
+    for mode in (0, 1, 2):
+        c = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=cyc.meta["prior"], point_to_plane=True,
+                      max_corr_dist=0.01, max_iterations=60, tile=64, precision_mode=mode)
+        c.run()
+        res.append(c.results())
+    for r in res[1:]:
+        np.testing.assert_array_equal(r["T"], res[0]["T"])
+        np.testing.assert_array_equal(r["iterations"], res[0]["iterations"])
+        np.testing.assert_array_equal(r["fitness"], res[0]["fitness"])
