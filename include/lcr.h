@@ -184,6 +184,14 @@ typedef struct {
 } lcr_tune_result;
 int lcr_cycle_autotune(lcr_cycle *c, const char *cache_path, lcr_tune_result *out);
 
+/* Rigid transform of a cloud on the device: xyz' = T xyz (FP64, rounded once to
+ * float) and normal' = R normal, non-finite entries passed through.  Replaces
+ * pcl::transformPointCloudWithNormals as the transform tool applies a 4x4
+ * (pcl_tools/transform.cpp:84-90) and as alignment.sh:109-110 chains a pair's
+ * result onto the later captures.  nrm / out_nrm nullable; T row-major. */
+int lcr_transform_points(const float *xyz, const float *nrm, int64_t n, int64_t stride_bytes,
+                         const double T[16], float *out_xyz, float *out_nrm);
+
 /* One-shot: create, run, results, destroy. */
 int lcr_icp_batch_align(const lcr_cycle_desc *d, lcr_cycle_result *r);
 

@@ -44,7 +44,7 @@ EXPORTS = (
     "lcr_proposal_create", "lcr_proposal_destroy", "lcr_proposal_set_weights",
     "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
     "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
-    "lcr_proposal_predict", "lcr_cycle_autotune",
+    "lcr_proposal_predict", "lcr_cycle_autotune", "lcr_transform_points",
 )
 
 
@@ -157,6 +157,7 @@ def _load():
                                           C.c_uint64, fp, fp]
     L.lcr_proposal_predict.argtypes = [vp, C.c_int32, C.c_int32, fp, fp]
     L.lcr_cycle_autotune.argtypes = [vp, C.c_char_p, C.POINTER(TuneResult)]
+    L.lcr_transform_points.argtypes = [fp, fp, C.c_int64, C.c_int64, dp, fp, fp]
     L.lcr_last_error.restype = C.c_char_p
     L.lcr_version.restype = C.c_char_p
     return L
@@ -305,6 +306,20 @@ def normals(xyz, k, viewpoint=(0.0, 0.0, 0.0), flip_all=False):
     _check(lib.lcr_normals(_fp(x), len(x), 12, int(k), _fp(vp), int(bool(flip_all)), _fp(nrm),
                            _fp(curv)))
     return nrm, curv
+
+
+def transform_points(xyz, T, nrm=None):
+    """xyz' = T xyz and normals' = R normals on the device (FP64 transform rounded
+    once), as pcl::transformPointCloudWithNormals in the transform tool
+    (transform.cpp:84-90).  Returns (xyz', normals' or None)."""
+    x = _f32(xyz)
+    M = np.ascontiguousarray(np.asarray(T, np.float64).reshape(16))
+    out = np.empty_like(x)
+    n = _f32(nrm) if nrm is not None else None
+    out_n = np.empty_like(x) if n is not None else None
+    _check(lib.lcr_transform_points(_fp(x), _fp(n) if n is not None else None, len(x), 12, _dp(M),
+                                    _fp(out), _fp(out_n) if out_n is not None else None))
+    return out, out_n
 
 
 class NormalEstimation:

@@ -437,27 +437,16 @@ int lcr_normals(const float* xyz, int64_t n, int64_t stride_bytes, int32_t k, co
     cs.build(st.s, factor, nullptr);
     const float v[3] = {vp ? vp[0] : 0.f, vp ? vp[1] : 0.f, vp ? vp[2] : 0.f};
     cs.compute_normals(st.s, k, v, flip_all, nullptr);
-    std::vector<float4> sorted((size_t)std::max<int64_t>(cs.total_pts, 1));
-    if (cs.total_pts > 0) {
-      LCR_HIP(hipMemcpyAsync(sorted.data(), cs.d_nrm.p, cs.total_pts * sizeof(float4),
-                             hipMemcpyDeviceToHost, st.s));
-    }
-    std::vector<float4> pts((size_t)std::max<int64_t>(cs.total_pts, 1));
-    if (cs.total_pts > 0) {
-      LCR_HIP(hipMemcpyAsync(pts.data(), cs.d_pts.p, cs.total_pts * sizeof(float4),
-                             hipMemcpyDeviceToHost, st.s));
-    }
+    cs.normals_to_raw(st.s);  // input order; NaN for non-finite points
+    std::vector<float4> out((size_t)n);
+    LCR_HIP(hipMemcpyAsync(out.data(), cs.d_raw_nrm.p, n * sizeof(float4), hipMemcpyDeviceToHost,
+                           st.s));
     LCR_HIP(hipStreamSynchronize(st.s));
-    for (int64_t i = 0; i < 3 * n; ++i) nrm[i] = NAN;
-    if (curv)
-      for (int64_t i = 0; i < n; ++i) curv[i] = NAN;
-    for (int64_t j = 0; j < cs.total_pts; ++j) {
-      int o;
-      memcpy(&o, &pts[j].w, sizeof o);
-      nrm[3 * (int64_t)o] = sorted[j].x;
-      nrm[3 * (int64_t)o + 1] = sorted[j].y;
-      nrm[3 * (int64_t)o + 2] = sorted[j].z;
-      if (curv) curv[o] = sorted[j].w;
+    for (int64_t i = 0; i < n; ++i) {
+      nrm[3 * i] = out[i].x;
+      nrm[3 * i + 1] = out[i].y;
+      nrm[3 * i + 2] = out[i].z;
+      if (curv) curv[i] = out[i].w;
     }
     return LCR_OK;
   });
@@ -620,6 +609,31 @@ int lcr_cycle_results(lcr_cycle* c, lcr_cycle_result* r) {
     if (r->best_hyp) r->best_hyp[p] = best;
   }
   return LCR_OK;
+}
+
+int lcr_transform_points(const float* xyz, const float* nrm, int64_t n, int64_t stride_bytes,
+                         const double T[16], float* out_xyz, float* out_nrm) {
+  return guarded([&] {
+    if (n < 0 || !T || (n > 0 && (!xyz || !out_xyz))) return fail(LCR_E_ARG, "invalid arguments");
+    if (n > kMaxCloudPoints) return fail(LCR_E_LIMIT, "cloud larger than 2^24 points");
+    if (stride_bytes == 0) stride_bytes = 3 * sizeof(float);
+    if (stride_bytes < 12 || stride_bytes % 4 != 0) return fail(LCR_E_ARG, "bad stride_bytes");
+    if (n == 0) return LCR_OK;
+    Stream st;
+    CloudSet cs;
+    cs.upload(st.s, {xyz}, {nrm}, {n}, stride_bytes);
+    DevBuf d_out;
+    d_out.reserve((size_t)n * 6 * sizeof(float));
+    float* dx = d_out.as<float>();
+    float* dn = dx + 3 * n;
+    launch_transform_points(cs.d_raw.as<float4>(), (nrm && cs.has_nrm) ? cs.d_raw_nrm.as<float4>() : nullptr,
+                            n, T, dx, (nrm && out_nrm) ? dn : nullptr, st.s);
+    LCR_HIP(hipMemcpyAsync(out_xyz, dx, n * 3 * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    if (nrm && out_nrm)
+      LCR_HIP(hipMemcpyAsync(out_nrm, dn, n * 3 * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    LCR_HIP(hipStreamSynchronize(st.s));
+    return LCR_OK;
+  });
 }
 
 }  // extern "C"

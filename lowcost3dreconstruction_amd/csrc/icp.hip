@@ -346,6 +346,43 @@ __global__ void k_transform_output(const float4* raw, const float4* raw_nrm, int
   }
 }
 
+// Same arithmetic as k_transform_output for a caller-given transform (pose
+// chaining of a cycle's clouds).
+struct Mat16 {
+  double m[16];
+};
+
+__global__ void k_transform_points(const float4* raw, const float4* raw_nrm, int64_t n, Mat16 T,
+                                   float* out_xyz, float* out_nrm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = raw[i];
+  if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+    float x, y, z;
+    xform(T.m, p, x, y, z);
+    out_xyz[3 * i] = x;
+    out_xyz[3 * i + 1] = y;
+    out_xyz[3 * i + 2] = z;
+  } else {
+    out_xyz[3 * i] = p.x;
+    out_xyz[3 * i + 1] = p.y;
+    out_xyz[3 * i + 2] = p.z;
+  }
+  if (out_nrm && raw_nrm) {
+    const float4 m = raw_nrm[i];
+    if (isfinite(m.x) && isfinite(m.y) && isfinite(m.z)) {
+      const double nx = m.x, ny = m.y, nz = m.z;
+      out_nrm[3 * i] = (float)((T.m[0] * nx + T.m[1] * ny) + T.m[2] * nz);
+      out_nrm[3 * i + 1] = (float)((T.m[4] * nx + T.m[5] * ny) + T.m[6] * nz);
+      out_nrm[3 * i + 2] = (float)((T.m[8] * nx + T.m[9] * ny) + T.m[10] * nz);
+    } else {
+      out_nrm[3 * i] = m.x;
+      out_nrm[3 * i + 1] = m.y;
+      out_nrm[3 * i + 2] = m.z;
+    }
+  }
+}
+
 template <bool P2PL, int TILE>
 void launch_iter_t(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL((k_icp_iter<P2PL, TILE, 0>), dim3(nblocks), dim3(TILE), 0, s, b, c);
@@ -411,6 +448,15 @@ void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n
   if (n <= 0) return;
   hipLaunchKernelGGL(k_transform_output, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, raw,
                      raw_nrm, n, st, pair, out_xyz, out_nrm);
+}
+
+void launch_transform_points(const float4* raw, const float4* raw_nrm, int64_t n, const double* T,
+                             float* out_xyz, float* out_nrm, hipStream_t s) {
+  if (n <= 0) return;
+  Mat16 M;
+  for (int i = 0; i < 16; ++i) M.m[i] = T[i];
+  hipLaunchKernelGGL(k_transform_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, raw,
+                     raw_nrm, n, M, out_xyz, out_nrm);
 }
 
 int icp_max_blocks_per_cu(int tile, bool p2pl, int precision) {

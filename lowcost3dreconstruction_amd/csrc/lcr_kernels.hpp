@@ -68,6 +68,9 @@ void launch_correspondences(const IcpBuffers& b, int pair, int npts_src, int32_t
 void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n,
                              const PairState* st, int pair, float* out_xyz, float* out_nrm,
                              hipStream_t s);
+// x' = T x (xyz, FP64, rounded once) and n' = R n for a host-given row-major T.
+void launch_transform_points(const float4* raw, const float4* raw_nrm, int64_t n, const double* T,
+                             float* out_xyz, float* out_nrm, hipStream_t s);
 int icp_max_blocks_per_cu(int tile, bool p2pl, int precision);
 
 // ---- K7 normals (normals.hip)

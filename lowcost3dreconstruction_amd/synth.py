@@ -278,12 +278,23 @@ class Cycle:
     meta: dict = field(default_factory=dict)
 
 
+def tilt_prior(tilt_deg, centre):
+    """Prior for a top (tilt > 0) or bottom (tilt < 0) view onto capture 0: the
+    inverse of the tilt about the horizontal camera axis through the centre."""
+    P = rigid(np.eye(3), np.asarray(centre, dtype=np.float64))
+    return P @ rigid(rot_axis_angle([1, 0, 0], np.deg2rad(-tilt_deg)), np.zeros(3)) @ np.linalg.inv(P)
+
+
 def turntable_cycle(seed, n_captures=36, yaw_step_deg=10.0, crop=(0.05, 0.60), jitter=0.005,
                     centre=(0.0, 0.0, -0.9), leaf=0.002, cam=KINECT_V1, scale=1.0,
-                    noise=lambda z: 0.0012 * z * z, closing_pair=True):
+                    noise=lambda z: 0.0012 * z * z, closing_pair=True, top_bottom=False,
+                    tilt_deg=45.0):
     """A turntable capture cycle: capture i at yaw i*step (+ translation jitter), each
     cropped by a random half-space removing a U[crop] fraction.  Pairs are (i, i-1)
-    for i = 1..N-1 (alignment.sh's order) plus (0, N-1) when closing_pair."""
+    for i = 1..N-1 (alignment.sh's order) plus (0, N-1) when closing_pair.  With
+    top_bottom, two more captures see the object tilted by +-tilt_deg about the
+    horizontal axis; each is paired onto capture 0 (centroid_align.cpp:104-130 aligns
+    the top and bottom views onto the first capture)."""
     rng = np.random.default_rng(seed)
     shape = make_shape(rng, scale=scale)
     poses, clouds = [], []
@@ -298,11 +309,39 @@ def turntable_cycle(seed, n_captures=36, yaw_step_deg=10.0, crop=(0.05, 0.60), j
     pairs = [(i, i - 1) for i in range(1, n_captures)]
     if closing_pair:
         pairs.append((0, n_captures - 1))
+    prior = turntable_prior(yaw_step_deg, centre)
+    priors = [prior] * len(pairs)
+    if top_bottom:
+        for tilt in (tilt_deg, -tilt_deg):
+            jit = jitter * random_unit(rng)
+            T = rigid(rot_axis_angle([1, 0, 0], np.deg2rad(tilt)), np.asarray(centre) + jit)
+            pts = depth_cloud(shape, T, cam, noise=noise, leaf=leaf, rng=rng)
+            if crop is not None:
+                pts = occlusion_crop(pts, rng.uniform(*crop), rng)
+            poses.append(T)
+            clouds.append(pts.astype(np.float32))
+            pairs.append((len(clouds) - 1, 0))
+            priors.append(tilt_prior(tilt, centre))
     pairs = np.asarray(pairs, dtype=np.int32)
     T_gt = np.stack([poses[t] @ np.linalg.inv(poses[s]) for s, t in pairs])
-    prior = turntable_prior(yaw_step_deg, centre)
     return Cycle(clouds, pairs, T_gt, np.stack(poses),
-                 meta=dict(seed=seed, prior=np.stack([prior] * len(pairs))))
+                 meta=dict(seed=seed, prior=np.stack(priors)))
+
+
+def hypotheses(cycle, n_hyp, max_angle_deg, max_trans, seed):
+    """(P * n_hyp, 4, 4) initial poses, hypothesis-major within a pair: the pair's
+    prior, then n_hyp - 1 copies moved by a random rotation of up to max_angle_deg
+    about a uniform axis through the prior-mapped source centroid, and a
+    translation of up to max_trans."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for (s, _), P in zip(cycle.pairs, cycle.meta["prior"]):
+        c = P[:3, :3] @ cycle.clouds[s].astype(np.float64).mean(axis=0) + P[:3, 3]
+        C = rigid(np.eye(3), c)
+        out.append(P)
+        for _ in range(n_hyp - 1):
+            out.append(C @ perturb(rng, max_angle_deg, max_trans) @ np.linalg.inv(C) @ P)
+    return np.stack(out)
 
 
 def perturb(rng, max_angle_deg, max_trans):

@@ -110,3 +110,52 @@ def test_normal_estimation_cli(tmp_path, lcr, flags, flip):
         rn = -rn
     dots = np.sum(got[:, 3:6] * rn, axis=1)
     assert np.mean(dots > 1 - 1e-6) > 0.999
+
+
+def test_transform_points_matches_oracle(lcr):
+    rng = np.random.default_rng(2)
+    xyz = rng.normal(size=(5000, 3)).astype(np.float32)
+    xyz[7] = np.nan
+    nrm = rng.normal(size=(5000, 3)).astype(np.float32)
+    T = synth.rigid(synth.random_rotation(rng), rng.normal(size=3))
+    print_case(test="transform_points")
+    out, out_n = lcr.transform_points(xyz, T, nrm)
+    ref = oracle.transform_points(xyz, T)
+    np.testing.assert_array_equal(out[np.isfinite(ref)], ref[np.isfinite(ref)])
+    assert np.all(np.isnan(out[7]))
+    np.testing.assert_allclose(out_n, nrm.astype(np.float64) @ T[:3, :3].T, rtol=0, atol=2e-6)
+
+
+def test_align_cycle_cli_chains_poses(lcr, tmp_path):
+    """alignment.sh's naming (0.ply .. N-1.ply, top.ply, bottom.ply): one batched
+    run, every pair equal to the Cycle binding, poses chained into capture 0's frame."""
+    cyc = synth.turntable_cycle(8, n_captures=4, yaw_step_deg=10.0, scale=1.5,
+                                centre=(0.0, 0.0, -0.8), top_bottom=True, closing_pair=False)
+    names = ["0", "1", "2", "3", "top", "bottom"]
+    for name, c in zip(names, cyc.clouds):
+        write_ascii_ply(tmp_path / f"{name}.ply", c)
+    out = tmp_path / "out"
+    out.mkdir()
+    print_case(test="cli_align_cycle")
+    r = subprocess.run([os.path.join(BIN, "align_cycle"), "-d", str(tmp_path), "-n", "4", "-o",
+                        str(out), "--top_bottom", "--point_to_plane", "--distance_threshold",
+                        "0.01", "--max_iterations", "60", "--yaw_step", "10", "--centre",
+                        "0,0,-0.8"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("Has converged:") == len(cyc.pairs)
+    guesses = np.stack([synth.turntable_prior(10.0, (0.0, 0.0, -0.8))] * 3 +
+                       [np.eye(4), np.eye(4)])
+    ref = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses, point_to_plane=True,
+                    max_corr_dist=0.01, max_iterations=60, normals_k=30)
+    ref.run()
+    T = ref.results()["T"]
+    pose = {"0": np.eye(4)}
+    for i in (1, 2, 3):
+        pose[str(i)] = pose[str(i - 1)] @ T[i - 1]
+    pose["top"], pose["bottom"] = T[3], T[4]
+    for name, c in zip(names, cyc.clouds):
+        M = np.loadtxt(out / f"{name}.txt")
+        np.testing.assert_allclose(M, pose[name], rtol=0, atol=1e-12)
+        got, _ = read_binary_ply(out / f"{name}.ply")
+        exp = oracle.transform_points(c, M)
+        np.testing.assert_array_equal(got[:, :3].astype(np.float32), exp.astype(np.float32))
