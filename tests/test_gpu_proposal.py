@@ -6,11 +6,13 @@ of the per-point layers within 3e-2 relative (Frobenius, per weight block) -- th
 device sums in FP32 and rounds h1/h2 to bf16 from FP32 values, so a few
 activations differ by one bf16 ulp and a near-tied max-pool channel can pick a
 different point."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle
-from conftest import print_case
+from conftest import ROOT, print_case
 
 pytestmark = pytest.mark.gpu
 
@@ -94,3 +96,25 @@ def test_training_reduces_loss(lcr):
     losses = p.train(steps=300, batch=64, lr=1e-3, seed=1)
     assert np.all(np.isfinite(losses))
     assert losses[-20:].mean() < 0.7 * losses[:20].mean(), (losses[:20].mean(), losses[-20:].mean())
+
+
+WEIGHTS = os.path.join(ROOT, "lowcost3dreconstruction_amd", "models", "proposal_c4.npy")
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained proposal weights committed")
+def test_learned_proposal_gains_over_plain_icp(lcr):
+    """Held-out pairs (new shapes and motions, 20-45 deg / up to 10 cm initial
+    error): point-to-plane ICP from the identity versus the same batched ICP with
+    the trained network's hypothesis added (best by fitness).  The learned
+    hypothesis must align clearly more pairs, and the fitness fallback must keep
+    almost every pair that plain ICP already aligned."""
+    from lowcost3dreconstruction_amd import proposal_train as pt
+    prop = lcr.Proposal(np.load(WEIGHTS))
+    print_case(test="proposal_ablation")
+    src, tgt, T_gt = pt.make_eval_pairs(lcr, prop, 128, seed=4242, min_angle_deg=20.0,
+                                        max_angle_deg=45.0)
+    res = pt.ablation(lcr, prop, src, tgt, T_gt)
+    plain, learned = res["plain"], res["learned"]
+    print(f"success plain {plain['success']:.3f} learned {learned['success']:.3f}")
+    assert learned["success"] >= plain["success"] + 0.15
+    assert np.mean(plain["ok"] & ~learned["ok"]) <= 0.03
