@@ -106,7 +106,9 @@ int lcr_icp_convergence_state(const lcr_icp *h);
 /* CorrespondenceEstimation::determineCorrespondences for the source transformed by
  * T (row-major, NULL = the current final transform): per source point the target
  * index of its nearest neighbour, or -1 when d2 > max_corr_dist^2 or the point is
- * not finite, and that nearest d2 (FLANN float form; +inf if none was searched). */
+ * not finite; d2 is that nearest distance (FLANN float form) for kept points, and
+ * for rejected ones either the nearest distance found or +inf when no target
+ * point lies within max_corr_dist of the query's grid cell (nothing searched). */
 int lcr_icp_correspondences(lcr_icp *h, const double *T, int32_t *idx, float *d2);
 
 /* pcl::NormalEstimation with setKSearch(k) and a viewpoint, then the CLI's flip

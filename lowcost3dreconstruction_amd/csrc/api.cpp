@@ -278,12 +278,14 @@ static void prepare(lcr_icp* h) {
     h->cs.upload(h->stream.s, xyz, nrm_up, cnt, 3 * sizeof(float));
     h->cs.build(h->stream.s, h->params.cell_factor, nullptr);
     const double r = h->params.max_corr_dist;
+    h->cs.build_near_mask(h->stream.s, r);
     h->eng.setup(h->cs, {0}, {1}, {r * r}, (h->params.tile && !h->params.precision_mode) ? h->params.tile : 64);
     h->dirty = false;
   } else {
     // parameters may have changed: refresh the instance radius
     const double r = h->params.max_corr_dist;
     if (h->eng.hp.size() == 1 && h->eng.hp[0].max_corr_d2 != r * r) {
+      h->cs.build_near_mask(h->stream.s, r);
       h->eng.setup(h->cs, {0}, {1}, {r * r}, (h->params.tile && !h->params.precision_mode) ? h->params.tile : 64);
     }
   }
@@ -538,6 +540,7 @@ int lcr_cycle_run(lcr_cycle* c) {
       c->cs.normals_to_raw(c->stream.s);
     }
     c->cs.build(c->stream.s, d.params.cell_factor, &spans);
+    c->cs.build_near_mask(c->stream.s, d.params.max_corr_dist);
     if (d.params.point_to_plane && !c->cs.has_nrm)
       return fail(LCR_E_NORMALS, "point-to-plane ICP needs normals");
     const int H = c->nhyp_total;

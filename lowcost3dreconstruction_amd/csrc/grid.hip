@@ -290,6 +290,53 @@ __global__ void k_bvh_up(const GridDesc* __restrict__ grids, int step, float4* _
   N[2 * node + 1] = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
 }
 
+// grid owning a global cell_start entry (grids in cell_off order)
+__device__ __forceinline__ int grid_of_cell(const GridDesc* __restrict__ g, int n, int64_t c) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].cell_off <= c)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// One axis of the dilation: out[c] = OR of in over the cells within kd along
+// `axis` (pass 0 reads occupancy from cell_start instead of `in`).
+__global__ void k_near_pass(const GridDesc* __restrict__ grids, int ngrids,
+                            const int* __restrict__ cs, const uint8_t* __restrict__ in,
+                            uint8_t* __restrict__ out, int64_t total, int axis, float r) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  const GridDesc g = grids[grid_of_cell(grids, ngrids, c)];
+  const int64_t local = c - g.cell_off;
+  const int64_t ncells = (int64_t)g.nx * g.ny * g.nz;
+  if (g.npts == 0 || local >= ncells) {
+    out[c] = 0;
+    return;
+  }
+  const int x = (int)(local % g.nx), y = (int)((local / g.nx) % g.ny),
+            z = (int)(local / ((int64_t)g.nx * g.ny));
+  const int kd = (int)ceilf(r * 1.0001f * g.inv_h) + 1;
+  const int pos = axis == 0 ? x : (axis == 1 ? y : z);
+  const int dim = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)g.nx : (int64_t)g.nx * g.ny);
+  const int lo = max(pos - kd, 0), hi = min(pos + kd, dim - 1);
+  uint8_t v = 0;
+  for (int p = lo; p <= hi && !v; ++p) {
+    const int64_t idx = c + (int64_t)(p - pos) * stride;
+    v = in ? in[idx] : (uint8_t)(cs[idx + 1] > cs[idx]);
+  }
+  out[c] = v;
+}
+
+__global__ void k_set_near_r(GridDesc* grids, int n, float r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) grids[i].near_r = r;
+}
+
 // sorted-order normals back to input order: raw[cloud_off[c] + id] = nrm[j]
 __global__ void k_unsort_normals(const float4* __restrict__ pts, const float4* __restrict__ nrm,
                                  const int* __restrict__ cloud_of_sorted,
@@ -346,6 +393,23 @@ void launch_unsort_normals(const float4* pts, const float4* nrm, const int* clou
   if (n_sorted > 0)
     hipLaunchKernelGGL(k_unsort_normals, dim3(blocks_for(n_sorted, 256)), dim3(256), 0, s, pts, nrm,
                        cloud_of_sorted, cloud_off, n_sorted, raw_nrm);
+}
+
+void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_cells,
+                      float r, uint8_t* out, uint8_t* tmp, hipStream_t s) {
+  if (total_cells <= 0 || ngrids <= 0) return;
+  const unsigned b = blocks_for(total_cells, 256);
+  hipLaunchKernelGGL(k_near_pass, dim3(b), dim3(256), 0, s, grids, ngrids, cell_start,
+                     (const uint8_t*)nullptr, out, total_cells, 0, r);
+  hipLaunchKernelGGL(k_near_pass, dim3(b), dim3(256), 0, s, grids, ngrids, cell_start,
+                     (const uint8_t*)out, tmp, total_cells, 1, r);
+  hipLaunchKernelGGL(k_near_pass, dim3(b), dim3(256), 0, s, grids, ngrids, cell_start,
+                     (const uint8_t*)tmp, out, total_cells, 2, r);
+}
+
+void launch_set_near_r(GridDesc* grids, int ngrids, float r, hipStream_t s) {
+  if (ngrids <= 0) return;
+  hipLaunchKernelGGL(k_set_near_r, dim3((ngrids + 63) / 64), dim3(64), 0, s, grids, ngrids, r);
 }
 
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {

@@ -59,6 +59,16 @@ __device__ __forceinline__ void xform(const double* T, const float4& s, float& q
   qz = (float)v;
 }
 
+// False only when the query lies in a grid cell that the near mask marks as
+// holding no point within GridDesc::near_r; queries outside the grid return true
+// (the search handles them).
+__device__ __forceinline__ bool near_cell(const GridDesc& g, const uint8_t* __restrict__ near,
+                                          float qx, float qy, float qz) {
+  const CellPos c = cell_pos(g, qx, qy, qz);
+  if (c.cx < 0 || c.cy < 0 || c.cz < 0 || c.cx >= g.nx || c.cy >= g.ny || c.cz >= g.nz) return true;
+  return near[g.cell_off + ((int64_t)c.cz * g.ny + c.cy) * g.nx + c.cx] != 0;
+}
+
 struct TileRange {
   int first, end, step;
 };
@@ -135,16 +145,19 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
     if (own) xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+    // no target point within r of the query's cell: no correspondence, no search
+    const bool empty = own && bounded && b.near && gt.near_r >= r &&
+                       !near_cell(gt, b.near, qx, qy, qz);
     bool tiled = false;
     if constexpr (PREC != 0) {
       __shared__ TileLds tl;
-      const bool q_ok = own && isfinite(qx) && isfinite(qy) && isfinite(qz);
+      const bool q_ok = own && !empty && isfinite(qx) && isfinite(qy) && isfinite(qz);
       if (bounded)
         tiled = tile_nn<PREC>(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r, tl,
                               nn.d2, nn.id, nn.j);
     }
     if (!tiled) {
-      if (own) {
+      if (own && !empty) {
         const int seed = b.prev ? b.prev[slot] : -1;
         nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded,
                       seed, 2, true);
@@ -304,9 +317,14 @@ __global__ void k_correspondences(const IcpBuffers b, int pair, int32_t* idx, fl
   float qx, qy, qz;
   xform(b.state[pair].T, s, qx, qy, qz);
   const float r = (float)sqrt(pd.max_corr_d2);
+  const int o = __float_as_int(s.w);
+  if (isfinite(r) && b.near && gt.near_r >= r && !near_cell(gt, b.near, qx, qy, qz)) {
+    idx[o] = -1;
+    d2[o] = INFINITY;
+    return;
+  }
   const NNResult nn =
       nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, isfinite(r));
-  const int o = __float_as_int(s.w);
   const bool keep = nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2;
   idx[o] = keep ? nn.id : -1;
   d2[o] = nn.d2;

@@ -36,7 +36,7 @@ struct GridDesc {
   int nraw;           // points handed in (including non-finite ones)
   int64_t node_off;   // first node of this cloud's bounding-volume tree (2 float4 each)
   int depth;          // tree depth: leaves are nodes [2^depth, 2^(depth+1)), 1-based
-  int pad_;
+  float near_r;       // radius of the cell mask "some point within near_r" (0 = no mask)
 };
 
 // Points per leaf of the bounding-volume tree over a cloud's cell-sorted points.

@@ -25,6 +25,11 @@ void launch_place_points(const float4* raw, const float4* raw_nrm, const int* ci
                          int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
                          int* slot_cell, const int64_t* cloud_off, const int* point_cloud,
                          float4* pts, float4* nrm, hipStream_t s);
+// Dilated occupancy per cell (1 = some point within r of the cell), three
+// separable passes; tmp and out hold total_cells bytes.
+void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_cells,
+                      float r, uint8_t* out, uint8_t* tmp, hipStream_t s);
+void launch_set_near_r(GridDesc* grids, int ngrids, float r, hipStream_t s);
 void launch_unsort_normals(const float4* pts, const float4* nrm, const int* cloud_of_sorted,
                            const int64_t* cloud_off, int64_t n_sorted, float4* raw_nrm,
                            int64_t n_raw, hipStream_t s);
@@ -45,6 +50,7 @@ struct IcpBuffers {
   double* partials;       // ntiles * kMaxVals
   int* prev;              // ntiles * tile: last match (sorted target position) per source
                           // point, -1 = none; seeds the next search (nullable)
+  const uint8_t* near;    // per target cell: some point within GridDesc::near_r (nullable)
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)

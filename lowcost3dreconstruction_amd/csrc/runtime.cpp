@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 
 namespace lcr {
 
@@ -222,6 +223,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
       g.nz = (int)std::min<double>(floor(((double)b[5] - b[2]) / h0) + 1, 65);
     }
     g.cell_off = cells;
+    g.near_r = 0.0f;
     cells += (int64_t)g.nx * g.ny * g.nz + 1;
   }
   d_grids.reserve((size_t)n * sizeof(GridDesc));
@@ -266,6 +268,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
       g.nz = (int)(floor(ez * (double)g.inv_h) + 1);
     }
     g.cell_off = cells;
+    g.near_r = 0.0f;
     cells += (int64_t)g.nx * g.ny * g.nz + 1;
   }
   if (cells > (int64_t)1 << 30) throw Error{-5, "grid too large"};
@@ -339,6 +342,24 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
   prof.end(span, s, spans);
   has_nrm = true;
   LCR_HIP(hipStreamSynchronize(s));
+}
+
+void CloudSet::build_near_mask(hipStream_t s, double r) {
+  // dilation in cells: a point within r of a query lies at most ceil(r / h) + 1
+  // cells away along each axis (the +1 covers positions inside the cells and
+  // binning rounding)
+  int kd_max = 0;
+  for (const GridDesc& g : grids)
+    if (g.npts > 0) kd_max = std::max(kd_max, (int)ceil(r * 1.0001 * (double)g.inv_h) + 1);
+  const bool use = std::isfinite(r) && r > 0.0 && kd_max <= kMaxNearCells && total_cells > 0;
+  for (GridDesc& g : grids) g.near_r = use ? (float)r : 0.0f;
+  if (use) {
+    d_near.reserve((size_t)total_cells);
+    d_near_tmp.reserve((size_t)total_cells);
+    launch_near_mask(d_grids.as<GridDesc>(), n, d_cell_start.as<int>(), total_cells, (float)r,
+                     d_near.as<uint8_t>(), d_near_tmp.as<uint8_t>(), s);
+  }
+  launch_set_near_r(d_grids.as<GridDesc>(), n, use ? (float)r : 0.0f, s);
 }
 
 void CloudSet::normals_to_raw(hipStream_t s) {
@@ -415,6 +436,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.state = d_state.as<PairState>();
   b.partials = d_partials.as<double>();
   b.prev = d_prev.as<int>();
+  b.near = cs.d_near.p ? cs.d_near.as<uint8_t>() : nullptr;
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;

@@ -122,7 +122,7 @@ struct CloudSet {
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
   DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
   DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_mom, d_nsel, d_defer;
-  DevBuf d_slot_raw, d_slot_cell;
+  DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
 
   // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;
@@ -138,6 +138,11 @@ struct CloudSet {
   // Copies the cell-sorted normals back to input order (d_raw_nrm), so the next
   // build() carries them into its own order.
   void normals_to_raw(hipStream_t s);
+  // Per grid cell: 1 if some point of the cloud may lie within r of a position in
+  // the cell (d_near; GridDesc::near_r = r).  Skipped (near_r = 0) for an
+  // infinite r or one spanning more than kMaxNearCells cells.
+  void build_near_mask(hipStream_t s, double r);
+  static constexpr int kMaxNearCells = 24;
 };
 
 // A batch of ICP instances over a CloudSet.

@@ -82,9 +82,12 @@ def random_motions(rng, n, max_angle_deg=45.0, max_trans=0.1, min_angle_deg=0.0)
 
 
 def train(lcr, pairs=200_000, npts=8192, epochs=4, batch=256, lr=1e-3, lr_final=1e-4, seed=0,
-          weights=None, log=print):
+          weights=None, log=print, time_budget=None):
     """Synthesise `pairs` training pairs on the device and train with Adam; the rate
-    decays geometrically from lr to lr_final.  Returns (proposal, losses)."""
+    decays geometrically from lr to lr_final over 20 segments.  With time_budget
+    (seconds of training), the steps of the remaining segments are scaled down
+    after the first segment if the projected time exceeds it.  Returns (proposal,
+    losses)."""
     rng = np.random.default_rng(seed)
     shapes, shares = random_shapes(rng, pairs)
     motions = random_motions(rng, pairs)
@@ -96,14 +99,23 @@ def train(lcr, pairs=200_000, npts=8192, epochs=4, batch=256, lr=1e-3, lr_final=
     segments = 20
     losses = []
     t0 = time.time()
+    scale = 1.0
     for k in range(segments):
         n = steps // segments + (1 if k < steps % segments else 0)
+        n = max(1, int(n * scale)) if k > 0 else n
         if n == 0:
             continue
         rate = lr * (lr_final / lr) ** (k / max(segments - 1, 1))
         losses.append(prop.train(n, batch, lr=rate, seed=seed * 1000 + k))
-        log(f"segment {k}: lr {rate:.2e}, loss {losses[-1][-50:].mean():.4f}, "
+        log(f"segment {k}: {n} steps, lr {rate:.2e}, loss {losses[-1][-50:].mean():.4f}, "
             f"{time.time() - t0:.1f} s")
+        if k == 0 and time_budget:
+            per_step = (time.time() - t0) / n
+            projected = per_step * steps
+            if projected > time_budget:
+                scale = time_budget / projected
+                log(f"projected {projected:.0f} s > budget {time_budget:.0f} s: "
+                    f"later segments scaled by {scale:.3f}")
     return prop, np.concatenate(losses)
 
 
@@ -151,17 +163,21 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--eval-pairs", type=int, default=256)
+    ap.add_argument("--time-budget", type=float, default=0.0,
+                    help="seconds of training; later segments shrink to fit (0 = no limit)")
     args = ap.parse_args()
     import lowcost3dreconstruction_amd as lcr
     lcr.init(0)
     prop, losses = train(lcr, args.pairs, args.points, args.epochs, args.batch, args.lr,
-                         seed=args.seed, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+                         seed=args.seed, log=lambda *a: print(*a, file=sys.stderr, flush=True),
+                         time_budget=args.time_budget or None)
     w = prop.get_weights()
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     np.save(args.out, w)
     src, tgt, T_gt = make_eval_pairs(lcr, prop, args.eval_pairs, args.points, seed=args.seed + 777)
     res = ablation(lcr, prop, src, tgt, T_gt)
     summary = {"pairs": args.pairs, "points": args.points, "epochs": args.epochs,
+               "steps": int(len(losses)),
                "final_loss": float(losses[-100:].mean()), "first_loss": float(losses[:100].mean()),
                "success_plain": res["plain"]["success"], "success_learned": res["learned"]["success"],
                "iters_plain": float(res["plain"]["iterations"].mean()),
