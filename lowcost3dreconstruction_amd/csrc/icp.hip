@@ -148,24 +148,24 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     // no target point within r of the query's cell: no correspondence, no search
     const bool empty = own && bounded && b.near && gt.near_r >= r &&
                        !near_cell(gt, b.near, qx, qy, qz);
-    bool tiled = false;
+    bool search = own && !empty;  // lanes that still need the scalar search
+    int seed = (search && b.prev) ? b.prev[slot] : -1;
     if constexpr (PREC != 0) {
       __shared__ TileLds tl;
-      const bool q_ok = own && !empty && isfinite(qx) && isfinite(qy) && isfinite(qz);
-      if (bounded)
-        tiled = tile_nn<PREC>(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r, tl,
-                              nn.d2, nn.id, nn.j);
-    }
-    if (!tiled) {
-      if (own && !empty) {
-        const int seed = b.prev ? b.prev[slot] : -1;
-        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded,
-                      seed, 2, true);
+      const bool q_ok = search && isfinite(qx) && isfinite(qy) && isfinite(qz);
+      bool cert = false;
+      if (bounded && tile_nn<PREC>(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r,
+                                   0.75f * gt.h, tl, nn.d2, nn.id, nn.j, cert)) {
+        if (q_ok && cert) search = false;
+        if (nn.j >= 0) seed = nn.j;
       }
-      const float rlim = r * 1.00001f;
-      bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY,
-                  nn.d2, nn.id, nn.j);
     }
+    if (search)
+      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded, seed,
+                    2, true);
+    const float rlim = r * 1.00001f;
+    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
+                nn.id, nn.j);
     if (own) {
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {

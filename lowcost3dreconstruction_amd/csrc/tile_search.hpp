@@ -5,10 +5,13 @@
 // (CorrespondenceEstimation, fine_registration.cpp:121) and returns the same
 // match: the FLANN-form float distance, ties to the smallest original index.
 //
-//  1. The wavefront's queries span a box; every target point within r of any
-//     query lies in the grid cells that overlap the box grown by r (plus one cell
-//     of binning slack).  Those cells' points are staged in LDS: one lane per row
-//     of cells finds the runs, a prefix sum places them, then a flat copy.
+//  1. The wavefront's queries span a box; the grid cells overlapping the box
+//     grown by rho = min(r, 0.75 cells) (plus a binning-rounding slack) are
+//     staged in LDS: one lane per row of cells finds the runs, a prefix sum
+//     places them, then a flat copy.  A query whose nearest staged point is
+//     closer than its distance to the grown box (or whose distance to the box
+//     exceeds r) is final; the few others continue in the exact scalar search,
+//     seeded with the staged answer.
 //  2. Coordinates relative to the box centre c are rounded to FP16 or BF16 and
 //     d~^2 = |q~|^2 + |p~|^2 - 2 q~.p~ is formed with the dot products on
 //     v_mfma_f32_16x16x32_{f16,bf16} (16 queries x 16 candidates per
@@ -107,37 +110,48 @@ __device__ __forceinline__ void tile_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Grid cell range [c0, c1] along one axis covering [lo, hi] (world units), with
-// one cell of slack on each side for binning rounding; returns false if empty.
+// Grid cell range [c0, c1] along one axis covering [lo, hi] (world units).  Points
+// are binned with the same (p - o) * inv_h arithmetic; the slack (1e-3 + 1e-6 |f|
+// cells, as in cell_pos) covers its rounding.  Returns false if empty.
 __device__ __forceinline__ bool axis_cells(float lo, float hi, float o, float inv_h, int n, int& c0,
                                            int& c1) {
   const float f0 = fminf(fmaxf((lo - o) * inv_h, -4.0e6f), 4.0e6f);
   const float f1 = fminf(fmaxf((hi - o) * inv_h, -4.0e6f), 4.0e6f);
-  c0 = max((int)floorf(f0) - 1, 0);
-  c1 = min((int)floorf(f1) + 1, n - 1);
+  const float s0 = 1.0e-3f + 1.0e-6f * fabsf(f0), s1 = 1.0e-3f + 1.0e-6f * fabsf(f1);
+  c0 = max((int)floorf(f0 - s0), 0);
+  c1 = min((int)floorf(f1 + s1), n - 1);
   return c0 <= c1;
 }
 
 // Must be called by all 64 lanes.  `own` lanes hold a finite query (qx, qy, qz).
-// On success the lane's (best, bid, bj) is the FLANN-form nearest target point if
-// it lies within r (else it may be a farther point or none); returns false,
+// The staged block covers the queries' box grown by rho = min(r, margin).  On
+// success the lane's (best, bid, bj) is the FLANN-form nearest staged point, and
+// `cert` says whether that is final: the nearest point overall (its distance is
+// below the query's distance to the grown box), or, when that distance exceeds r,
+// no point outside the block can be within r.  An uncertified lane's result is a
+// valid candidate (an upper bound) for the caller's exact search.  Returns false,
 // leaving the outputs untouched, when the tile cannot be staged.
 template <int PREC>
 __device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
                         const float4* __restrict__ P, bool own, float qx, float qy, float qz,
-                        float r, TileLds& L, float& best, int& bid, int& bj) {
+                        float r, float margin, TileLds& L, float& best, int& bid, int& bj,
+                        bool& cert) {
   const int lane = threadIdx.x & 63;
+  cert = false;
   const uint64_t owners = __ballot(own);
   if (owners == 0 || g.npts == 0) return true;
   const float lox = wave_minf(own ? qx : INFINITY), hix = wave_maxf(own ? qx : -INFINITY);
   const float loy = wave_minf(own ? qy : INFINITY), hiy = wave_maxf(own ? qy : -INFINITY);
   const float loz = wave_minf(own ? qz : INFINITY), hiz = wave_maxf(own ? qz : -INFINITY);
-  const float rr = r * 1.00001f;
+  const float rr = fminf(r * 1.00001f, margin);
   int x0, x1, y0, y1, z0, z1;
   if (!axis_cells(lox - rr, hix + rr, g.ox, g.inv_h, g.nx, x0, x1) ||
       !axis_cells(loy - rr, hiy + rr, g.oy, g.inv_h, g.ny, y0, y1) ||
-      !axis_cells(loz - rr, hiz + rr, g.oz, g.inv_h, g.nz, z0, z1))
-    return true;  // no grid cell within r of any query: no match anywhere
+      !axis_cells(loz - rr, hiz + rr, g.oz, g.inv_h, g.nz, z0, z1)) {
+    // no grid cell within rho of any query
+    cert = rr >= r * 1.00001f;
+    return true;
+  }
   // ---- stage the block's points
   const int nyb = y1 - y0 + 1, nrows = (z1 - z0 + 1) * nyb;
   if (nrows > kTileCap) return false;
@@ -187,7 +201,10 @@ __device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
     }
     M += total;
   }
-  if (M == 0) return true;
+  if (M == 0) {
+    cert = rr >= r * 1.00001f;
+    return true;
+  }
   // ---- centre, rounding bounds
   const float cx = 0.5f * (lox + hix), cy = 0.5f * (loy + hiy), cz = 0.5f * (loz + hiz);
   // every staged point and query lies in the cell box [x0, x1+1] x ... (h units)
@@ -290,6 +307,12 @@ __device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
       bid = id;
       bj = L.rj[lane];
     }
+    // every point within distance D of the query lies in the grown box, whose
+    // cells were all staged
+    const float D = fminf(fminf(fminf(qx - (lox - rr), (hix + rr) - qx),
+                                fminf(qy - (loy - rr), (hiy + rr) - qy)),
+                          fminf(qz - (loz - rr), (hiz + rr) - qz));
+    cert = (D >= r * 1.00001f) || (D > 0.0f && best < D * D * 0.99998f);
   }
   return true;
 }
