@@ -9,13 +9,17 @@ per cloud); 36 adjacent pairs (i -> i-1, plus the closing pair) aligned
 point-to-plane from the turntable prior (the rig's yaw step, as
 rotate_align.cpp:224-235 applies it), r = 1 cm, <= 60 iterations, epsilons
 1e-9 / 1e-3, 30-NN normals computed on the device.  When trained proposal weights
-are present the learned hypothesis runs beside the prior in the same batch and
-the better by fitness is kept (batched + learned).
+are present the learned hypothesis joins the prior in the same batch (batched +
+learned).
 
 `--workload c5`: 72 captures at 5 deg plus a top and a bottom view (45 deg tilt,
 paired onto capture 0), 512x424 Kinect-v2 rendering, noise 0.001 z, 1.5 mm voxels;
 15 pose hypotheses per pair (the prior and 14 rotations of up to 20 deg about the
 source centroid), plus the learned one when weights are present.
+
+With several hypotheses per pair (the learned one, or C5's 15) each pair's
+initial poses are scored by their inliers within r and only the best is refined
+(`--no-gate` refines them all and keeps the best by final fitness).
 
 A step = normals + grids + all ICP iterations (one hipGraph) + fitness + selection
 for the whole cycle, with the clouds already resident in HBM.  `value` counts the
@@ -143,6 +147,9 @@ def main():
                          "FP16/BF16 matrix-core distance tiles with exact FP32 rerank")
     ap.add_argument("--proposal", default="auto",
                     help="'auto' (use the committed weights when present), 'none', or a path")
+    ap.add_argument("--no-gate", action="store_true",
+                    help="refine every hypothesis of a pair instead of only the one with the "
+                         "most inliers at its initial pose")
     ap.add_argument("--autotune", action="store_true",
                     help="pick tile / cell size / precision with the autotuner first")
     ap.add_argument("--compare-precisions", type=int, default=3, metavar="STEPS",
@@ -181,9 +188,10 @@ def main():
         raise SystemExit(f"proposal weights not found: {wpath}")
 
     def make_cycle(mode, with_prop=True):
+        p = prop if with_prop else None
+        gate = not args.no_gate and (n_hyp > 1 or p is not None)
         return lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses, n_hyp=n_hyp, point_to_plane=True,
-                         normals_k=30, precision_mode=mode, proposal=prop if with_prop else None,
-                         **icp_kw())
+                         normals_k=30, precision_mode=mode, proposal=p, hyp_gate=gate, **icp_kw())
 
     # ---------------- batched cycle (the measured path)
     cycle = make_cycle(PRECISION[args.precision])
@@ -211,6 +219,7 @@ def main():
     _, grid_ms = lcr.prof_read("grid_build")
     _, nrm_ms = lcr.prof_read("normals")
     _, prop_ms = lcr.prof_read("proposal")
+    _, gate_ms = lcr.prof_read("gate")
     log(f"batched step {step_s * 1e3:.3f} ms (profiled replay {prof_step_s * 1e3:.3f} ms); "
         f"per step: icp {iter_ms / args.steps:.3f} ms ({n_iter_launches // args.steps} launches), "
         f"fitness {fit_ms / args.steps:.3f} ms, grids {grid_ms / args.steps:.3f} ms, "
@@ -323,6 +332,7 @@ def main():
             "points_total": int(sum(sizes)), "pairs": int(len(cyc.pairs)),
             "hypotheses_per_pair": int(H),
             "learned_proposal": prop is not None,
+            "hypothesis_gate": (not args.no_gate and (n_hyp > 1 or prop is not None)),
             "max_iterations": 60, "max_corr_dist_m": 0.01, "normals_k": 30, "seed": args.seed,
             "initial_pose": "turntable prior (rig yaw step about its axis)",
             "correspondence_search": args.precision,
@@ -347,7 +357,7 @@ def main():
         "precision_modes": modes,
         "kernel_ms_per_step": {"icp_iteration": iter_ms / args.steps, "fitness": fit_ms / args.steps,
                                "grid_build": grid_ms / args.steps, "normals": nrm_ms / args.steps,
-                               "proposal": prop_ms / args.steps,
+                               "proposal": prop_ms / args.steps, "gate": gate_ms / args.steps,
                                "profiled_step": prof_step_s * 1e3},
         "cpu_baseline": cpu,
     }

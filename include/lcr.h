@@ -142,6 +142,11 @@ typedef struct {
                                    hypothesis by fitness wins, so a worse learned
                                    guess falls back to the plain one */
   int32_t proposal_points;      /* points per cloud fed to the network (default 8192) */
+  int32_t hyp_gate;             /* 0: refine every hypothesis, keep the best by final
+                                   fitness; 1: first score every hypothesis's initial
+                                   pose by its source points within max_corr_dist
+                                   (ties: smaller mean d2) and refine only the best
+                                   one per pair */
 } lcr_cycle_desc;
 
 typedef struct {

@@ -397,6 +397,7 @@ class Cycle:
         self._proposal = proposal  # kept alive while the cycle uses it
         d.proposal = proposal._h if proposal is not None else None
         d.proposal_points = int(proposal_points)
+        d.hyp_gate = int(bool(hyp_gate))
         self.n_total_hyp = self.n_hyp + (1 if proposal is not None else 0)
         self._desc = d
         h = C.c_void_p()

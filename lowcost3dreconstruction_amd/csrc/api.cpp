@@ -573,6 +573,7 @@ int lcr_cycle_run(lcr_cycle* c) {
         prof.end(sp, c->stream.s, &spans);
       };
     }
+    c->eng.gate_h = (d.hyp_gate && H > 1) ? H : 0;
     c->eng.run(c->stream.s, c->cs, c->guesses.data(), true, pre);
     if (!spans.empty()) prof.collect(spans, false);
     c->states.assign(c->eng.states(), c->eng.states() + ninst);
@@ -600,9 +601,13 @@ int lcr_cycle_results(lcr_cycle* c, lcr_cycle_result* r) {
   if (!c->ran) return fail(LCR_E_STATE, "lcr_cycle_run has not run");
   const int H = c->nhyp_total;
   for (int p = 0; p < c->desc.n_pairs; ++p) {
-    int best = 0;
-    for (int h = 1; h < H; ++h)
-      if (c->states[p * H + h].fitness < c->states[p * H + best].fitness) best = h;
+    int best = -1;
+    for (int h = 0; h < H; ++h) {
+      const PairState& s = c->states[p * H + h];
+      if (s.gated) continue;
+      if (best < 0 || s.fitness < c->states[p * H + best].fitness) best = h;
+    }
+    if (best < 0) best = 0;
     const PairState& s = c->states[p * H + best];
     if (r->T) memcpy(r->T + 16 * (size_t)p, s.T, 16 * sizeof(double));
     if (r->iterations) r->iterations[p] = s.iterations;

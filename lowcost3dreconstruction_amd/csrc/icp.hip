@@ -227,17 +227,25 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
   }
 }
 
+// per_instance: each instance's own max_corr_d2 is the range.  A finite range
+// bounds the search at sqrt(range): only points within it are counted.
 template <int TILE>
-__global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max_range) {
+__global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max_range_arg,
+                                                  int per_instance) {
   __shared__ double lds[2 * (TILE / 64)];
   __shared__ int s_last;
   const TileRange tr = xcd_tiles(b.ntiles);
   for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     const int p = b.tile_pair[tile];
     PairState* st = b.state + p;
+    if (st->gated) continue;  // uniform across the block
     const PairDesc pd = b.pairs[p];
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
+    const double max_range = per_instance ? pd.max_corr_d2 : max_range_arg;
+    const bool bounded = max_range < DBL_MAX && max_range >= 0.0;
+    const float rq = bounded ? (float)sqrt(max_range) : INFINITY;
+    const float r = rq * 1.00001f;
     const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
     double v[2] = {0.0, 0.0};
     const bool own = local < gs.npts;
@@ -245,11 +253,14 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
     if (own) {
       xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+      const bool empty = bounded && b.near && gt.near_r >= rq && !near_cell(gt, b.near, qx, qy, qz);
       const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
-      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, INFINITY, false,
-                    seed, 2, true);
+      if (!empty)
+        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded, seed,
+                      2, true);
     }
-    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, INFINITY, nn.d2, nn.id, nn.j);
+    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? r * r : INFINITY, nn.d2, nn.id,
+                nn.j);
     if (own && nn.j >= 0 && (double)nn.d2 <= max_range) {
       v[0] = nn.d2;
       v[1] = 1.0;
@@ -282,6 +293,34 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
   }
 }
 
+// Per pair: keep the hypothesis with the most source points within its radius
+// (ties: the smaller mean d2 of those, then the lower index), set the others aside.
+__global__ void k_gate(PairState* st, int npairs, int H) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  int best = -1;
+  for (int h = 0; h < H; ++h) {
+    const PairState& s = st[p * H + h];
+    if (!s.active) continue;
+    if (best < 0) {
+      best = h;
+      continue;
+    }
+    const PairState& b = st[p * H + best];
+    if (s.fit_count > b.fit_count || (s.fit_count == b.fit_count && s.fitness < b.fitness))
+      best = h;
+  }
+  for (int h = 0; h < H; ++h) {
+    PairState& s = st[p * H + h];
+    s.fitness = DBL_MAX;
+    s.fit_count = 0;
+    if (h != best && best >= 0) {
+      s.active = 0;
+      s.gated = 1;
+    }
+  }
+}
+
 __global__ void k_state_init(PairState* st, const double* guesses, const PairDesc* pairs,
                              const GridDesc* grids, int npairs) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,7 +342,7 @@ __global__ void k_state_init(PairState* st, const double* guesses, const PairDes
   s.solve_failed = 0;
   s.ticket = 0u;
   s.fit_count = 0;
-  s.pad_ = 0;
+  s.gated = 0;
   st[p] = s;
 }
 
@@ -433,18 +472,25 @@ void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, h
   }
 }
 
-void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s) {
+void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,
+                    bool per_instance) {
+  const int pi = per_instance ? 1 : 0;
   switch (b.tile) {
     case 64:
-      hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range);
+      hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range, pi);
       break;
     case 128:
-      hipLaunchKernelGGL(k_fitness<128>, dim3(nblocks), dim3(128), 0, s, b, max_range);
+      hipLaunchKernelGGL(k_fitness<128>, dim3(nblocks), dim3(128), 0, s, b, max_range, pi);
       break;
     default:
-      hipLaunchKernelGGL(k_fitness<256>, dim3(nblocks), dim3(256), 0, s, b, max_range);
+      hipLaunchKernelGGL(k_fitness<256>, dim3(nblocks), dim3(256), 0, s, b, max_range, pi);
       break;
   }
+}
+
+void launch_gate(PairState* st, int npairs, int H, hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_gate, dim3((npairs + 63) / 64), dim3(64), 0, s, st, npairs, H);
 }
 
 void launch_state_init(PairState* st, const double* guesses, const PairDesc* pairs,

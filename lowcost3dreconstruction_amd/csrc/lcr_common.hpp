@@ -66,7 +66,7 @@ struct PairState {
   int solve_failed;    // 6x6 system not solvable in the last iteration
   unsigned int ticket; // blocks of the current launch that finished this instance
   int fit_count;       // points counted by the fitness pass
-  int pad_;
+  int gated;           // 1: set aside by the hypothesis gate (never refined or chosen)
 };
 
 struct IcpConst {

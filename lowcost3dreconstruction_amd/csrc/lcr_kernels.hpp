@@ -60,7 +60,13 @@ struct IcpBuffers {
 void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s);
 // getFitnessScore(max_range) for every instance with a nonzero iteration count or
 // all instances when `all` is set.
-void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s);
+// getFitnessScore(max_range) of every instance that is not gated; with
+// per_instance, each instance's own max_corr_d2 replaces max_range.
+void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,
+                    bool per_instance = false);
+// Hypothesis gate over n pairs of H consecutive instances, after a fitness pass
+// with each instance's radius: the best keeps running, the others are set aside.
+void launch_gate(PairState* st, int npairs, int H, hipStream_t s);
 // Initialise instance states from row-major 4x4 guesses (nullable = identity);
 // instances with an empty source or target start inactive.
 void launch_state_init(PairState* st, const double* guesses, const PairDesc* pairs,

@@ -487,6 +487,12 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   h_active.reserve(sizeof(int));
   const bool eager = !use_graph || prof.on();
   std::vector<Profiler::Span> spans;
+  if (gate_h > 1 && ntiles > 0) {
+    const int k = prof.begin("gate", s, &spans);
+    launch_fitness(b, 0.0, nblocks, s, true);
+    launch_gate(d_state.as<PairState>(), npairs / gate_h, gate_h, s);
+    prof.end(k, s, &spans);
+  }
   auto chunk = [&](std::vector<Profiler::Span>* sp) {
     for (int it = 0; it < kChunk; ++it) {
       const int k = prof.begin("icp_iteration", s, sp);

@@ -178,6 +178,9 @@ struct Engine {
   void setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
              const std::vector<double>& r2, int tile_pts);
   int setup_precision = -1;  // c.precision the launch shape was sized for
+  // > 1: instances come in groups of gate_h hypotheses of one pair; before the
+  // iterations, each group keeps only its best hypothesis by inliers within r.
+  int gate_h = 0;
   IcpBuffers buffers(const CloudSet& cs) const;
   // Runs state init (from guesses, 16 doubles per instance or null) and up to
   // max_iterations iterations plus the fitness pass; results land in h_state.

@@ -154,3 +154,33 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
     assert t2["from_cache"] == 1
     assert (t2["tile"], t2["precision_mode"], t2["cell_factor"]) == \
         (t["tile"], t["precision_mode"], t["cell_factor"])
+
+
+def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
+    """With hyp_gate, each pair's hypotheses are scored by their source points
+    within r at the initial pose (the oracle's correspondence count; ties by mean
+    d2, then index) and only that one is refined: bitwise the single-pair ICP from
+    that pose."""
+    cyc, normals = small_cycle
+    H = 4
+    guesses = synth.hypotheses(cyc, H, 20.0, 0.005, 3)
+    print_case(test="gate", pairs=len(cyc.pairs), hyp=H)
+    c = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=guesses, n_hyp=H,
+                  point_to_plane=True, hyp_gate=True, **KW)
+    c.run()
+    r = c.results()
+    inst = c.instances()
+    for i, (s, t) in enumerate(cyc.pairs):
+        score = []
+        for h in range(H):
+            idx, d2 = oracle.correspondences(cyc.clouds[s], cyc.clouds[t], guesses[i * H + h],
+                                             KW["max_corr_dist"])
+            ok = idx >= 0
+            score.append((-int(ok.sum()), float(d2[ok].astype(np.float64).mean()) if ok.any() else 0.0, h))
+        want = min(score)[2]
+        assert r["best_hyp"][i] == want, (i, score)
+        refined = [h for h in range(H) if inst["iterations"][i * H + h] > 0]
+        assert refined == [want]
+        icp = single_pair(lcr, cyc.clouds[s], cyc.clouds[t], normals[t], guesses[i * H + want])
+        np.testing.assert_array_equal(r["T"][i], icp.getFinalTransformation())
+        assert r["fitness"][i] == icp.getFitnessScore()
