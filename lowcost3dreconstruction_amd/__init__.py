@@ -83,6 +83,7 @@ class CycleDesc(C.Structure):
         ("viewpoints", C.POINTER(C.c_float)),
         ("proposal", C.c_void_p),
         ("proposal_points", C.c_int32),
+        ("hyp_gate", C.c_int32),
     ]
 
 
@@ -354,7 +355,7 @@ class Cycle:
     def __init__(self, clouds, pairs, normals=None, guesses=None, n_hyp=1, max_corr_dist=0.05,
                  max_iterations=60, transformation_epsilon=1e-9, euclidean_fitness_epsilon=1e-3,
                  point_to_plane=True, normals_k=30, viewpoints=None, cell_factor=0.0, tile=0,
-                 proposal=None, proposal_points=8192, precision_mode=0):
+                 proposal=None, proposal_points=8192, precision_mode=0, hyp_gate=False):
         self.clouds = [_f32(c) for c in clouds]
         self.normals = [_f32(n) for n in normals] if normals is not None else None
         pr = np.asarray(pairs, np.int32).reshape(-1, 2)

@@ -53,3 +53,37 @@ def test_invalid_params_rejected_before_any_device_call():
     rc = m.lib.lcr_icp_create(C.byref(p), C.byref(h))
     assert rc == -1
     assert b"greater than zero" in m.lib.lcr_last_error()
+
+
+def header_struct_fields(name):
+    """Field names, in order, of `typedef struct { ... } name;` in lcr.h."""
+    text = open(os.path.join(ROOT, "include", "lcr.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    m = re.search(r"typedef struct\s*\{([^{}]*)\}\s*" + name + r"\s*;", text)
+    assert m, name
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = decl.strip()
+        if decl:
+            fields.append(re.findall(r"([A-Za-z_][A-Za-z0-9_]*)\s*(?:\[[^\]]*\])?\s*$", decl)[0])
+    return fields
+
+
+def test_python_structs_mirror_the_header():
+    import lowcost3dreconstruction_amd as lcr
+    for cname, py in (("lcr_icp_params", lcr.IcpParams), ("lcr_cycle_desc", lcr.CycleDesc),
+                      ("lcr_cycle_result", lcr.CycleResult), ("lcr_tune_result", lcr.TuneResult)):
+        assert [f for f, _ in py._fields_] == header_struct_fields(cname), cname
+
+
+def test_cycle_wrapper_fills_every_descriptor_field():
+    """Every lcr_cycle_desc field is set by Cycle.__init__ from a name it defines."""
+    import inspect
+    import lowcost3dreconstruction_amd as lcr
+    src = inspect.getsource(lcr.Cycle.__init__)
+    params = set(inspect.signature(lcr.Cycle.__init__).parameters)
+    for f, _ in lcr.CycleDesc._fields_:
+        assert re.search(r"\bd\." + f + r"\s*=", src), f
+    for name in re.findall(r"int\(bool\((\w+)\)\)", src):
+        assert name in params, name
+    assert {"hyp_gate", "precision_mode", "proposal", "n_hyp", "guesses"} <= params
