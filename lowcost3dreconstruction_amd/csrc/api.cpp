@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <chrono>
@@ -82,8 +83,19 @@ bool params_ok(const lcr_icp_params* p, std::string* why) {
   return true;
 }
 
+// Tile order of the persistent kernels: grid stride unless LCR_TILE_ORDER=xcd
+// asks for XCD-contiguous ranges (read once).
+int tile_order_interleave() {
+  static const int v = [] {
+    const char* e = getenv("LCR_TILE_ORDER");
+    return (e && strcmp(e, "xcd") == 0) ? 0 : 1;
+  }();
+  return v;
+}
+
 IcpConst make_const(const lcr_icp_params& p) {
   IcpConst c{};
+  c.interleave = tile_order_interleave();
   c.max_iterations = p.max_iterations;
   c.point_to_plane = p.point_to_plane ? 1 : 0;
   c.rotation_threshold = 1.0 - p.transformation_epsilon;

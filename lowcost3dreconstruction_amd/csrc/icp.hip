@@ -73,8 +73,13 @@ struct TileRange {
   int first, end, step;
 };
 
-// XCD-aware persistent mapping (gridDim.x is a multiple of 8).
-__device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
+// Persistent mapping (gridDim.x is a multiple of 8).  XCD-contiguous: hardware
+// places block b on XCD b % 8, and XCD x takes the x-th eighth of the tile range,
+// so an instance's tiles (and its target grid) stay in one L2.  Interleaved:
+// plain grid stride, so the tiles of the few instances still active late in a
+// batch spread over the whole device.
+__device__ __forceinline__ TileRange xcd_tiles(int ntiles, int interleave) {
+  if (interleave) return TileRange{(int)blockIdx.x, ntiles, (int)gridDim.x};
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
   const int t0 = (int)(((int64_t)ntiles * xcd) >> 3);
   const int t1 = (int)(((int64_t)ntiles * (xcd + 1)) >> 3);
@@ -126,7 +131,7 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
   static_assert(PREC == 0 || TILE == 64, "distance tiles need one wavefront per tile");
   __shared__ double lds[NV * (TILE / 64)];
   __shared__ int s_last;
-  const TileRange tr = xcd_tiles(b.ntiles);
+  const TileRange tr = xcd_tiles(b.ntiles, c.interleave);
   for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     const int p = b.tile_pair[tile];
     PairState* st = b.state + p;
@@ -231,15 +236,18 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
 // bounds the search at sqrt(range): only points within it are counted.
 template <int TILE>
 __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max_range_arg,
-                                                  int per_instance) {
+                                                  int per_instance, int interleave) {
   __shared__ double lds[2 * (TILE / 64)];
   __shared__ int s_last;
-  const TileRange tr = xcd_tiles(b.ntiles);
+  const TileRange tr = xcd_tiles(b.ntiles, interleave);
   for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     const int p = b.tile_pair[tile];
     PairState* st = b.state + p;
     if (st->gated) continue;  // uniform across the block
     const PairDesc pd = b.pairs[p];
+    // the gate's score (per_instance) counts every kGateStride-th tile of the
+    // source: the same source points for every hypothesis of a pair
+    const bool sampled = !per_instance || (tile - pd.tile_begin) % kGateStride == 0;
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
     const double max_range = per_instance ? pd.max_corr_d2 : max_range_arg;
@@ -248,7 +256,7 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
     const float r = rq * 1.00001f;
     const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
     double v[2] = {0.0, 0.0};
-    const bool own = local < gs.npts;
+    const bool own = sampled && local < gs.npts;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
     if (own) {
@@ -473,17 +481,20 @@ void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, h
 }
 
 void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,
-                    bool per_instance) {
+                    bool per_instance, int interleave) {
   const int pi = per_instance ? 1 : 0;
   switch (b.tile) {
     case 64:
-      hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range, pi);
+      hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range, pi,
+                         interleave);
       break;
     case 128:
-      hipLaunchKernelGGL(k_fitness<128>, dim3(nblocks), dim3(128), 0, s, b, max_range, pi);
+      hipLaunchKernelGGL(k_fitness<128>, dim3(nblocks), dim3(128), 0, s, b, max_range, pi,
+                         interleave);
       break;
     default:
-      hipLaunchKernelGGL(k_fitness<256>, dim3(nblocks), dim3(256), 0, s, b, max_range, pi);
+      hipLaunchKernelGGL(k_fitness<256>, dim3(nblocks), dim3(256), 0, s, b, max_range, pi,
+                         interleave);
       break;
   }
 }

@@ -77,8 +77,11 @@ struct IcpConst {
   double mse_rel;                // euclidean_fitness_epsilon
   double mse_abs;                // 1e-12
   int precision;                 // correspondence search: 0 FP32, 1 FP16 / 2 BF16 tiles
-  int pad_;
+  int interleave;                // tile order: 0 XCD-contiguous, 1 grid stride
 };
+
+// The hypothesis gate scores each pose on every kGateStride-th work tile.
+constexpr int kGateStride = 8;
 
 // Flat FP32 weight layout of the pose-proposal network, row-major [in][out]:
 // W1 3x64, b1, W2 64x128, b2, W3 128x256, b3, Hw1 512x256, hb1, Hw2 256x6, hb2.

@@ -63,7 +63,7 @@ void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, h
 // getFitnessScore(max_range) of every instance that is not gated; with
 // per_instance, each instance's own max_corr_d2 replaces max_range.
 void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,
-                    bool per_instance = false);
+                    bool per_instance = false, int interleave = 0);
 // Hypothesis gate over n pairs of H consecutive instances, after a fitness pass
 // with each instance's radius: the best keeps running, the others are set aside.
 void launch_gate(PairState* st, int npairs, int H, hipStream_t s);

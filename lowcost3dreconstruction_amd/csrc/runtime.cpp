@@ -489,7 +489,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   std::vector<Profiler::Span> spans;
   if (gate_h > 1 && ntiles > 0) {
     const int k = prof.begin("gate", s, &spans);
-    launch_fitness(b, 0.0, nblocks, s, true);
+    launch_fitness(b, 0.0, nblocks, s, true, c.interleave);
     launch_gate(d_state.as<PairState>(), npairs / gate_h, gate_h, s);
     prof.end(k, s, &spans);
   }
@@ -505,7 +505,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   };
   auto tail = [&](std::vector<Profiler::Span>* sp) {
     const int k = prof.begin("fitness", s, sp);
-    launch_fitness(b, DBL_MAX, nblocks, s);
+    launch_fitness(b, DBL_MAX, nblocks, s, false, c.interleave);
     prof.end(k, s, sp);
     LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
                            hipMemcpyDeviceToHost, s));
@@ -577,7 +577,7 @@ void Engine::run_fitness(hipStream_t s, const CloudSet& cs, double max_range) {
   const IcpBuffers b = buffers(cs);
   if (ntiles > 0) {
     const int sp = prof.begin("fitness", s, &spans);
-    launch_fitness(b, max_range, nblocks, s);
+    launch_fitness(b, max_range, nblocks, s, false, c.interleave);
     prof.end(sp, s, &spans);
   }
   LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),

@@ -158,12 +158,27 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
 
 def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
     """With hyp_gate, each pair's hypotheses are scored by their source points
-    within r at the initial pose (the oracle's correspondence count; ties by mean
-    d2, then index) and only that one is refined: bitwise the single-pair ICP from
-    that pose."""
+    within r at the initial pose (on a fixed subsample of the source) and only the
+    best is refined: bitwise the single-pair ICP from that pose.  Here the prior
+    (hypothesis 1) has far more inliers than three poses rotated by 15-25 deg, so
+    it must be the one kept."""
+    from lowcost3dreconstruction_amd.synth import perturb, rigid
     cyc, normals = small_cycle
     H = 4
-    guesses = synth.hypotheses(cyc, H, 20.0, 0.005, 3)
+    rng = np.random.default_rng(3)
+    guesses = []
+    for (s, _), P in zip(cyc.pairs, cyc.meta["prior"]):
+        c = P[:3, :3] @ cyc.clouds[s].astype(np.float64).mean(axis=0) + P[:3, 3]
+        C = rigid(np.eye(3), c)
+        for h, ang in enumerate((20.0, None, 15.0, 25.0)):
+            if ang is None:
+                guesses.append(P)
+            else:
+                R = perturb(rng, ang, 0.0)
+                while synth.rotation_error(R[:3, :3], np.eye(3)) < np.deg2rad(0.8 * ang):
+                    R = perturb(rng, ang, 0.0)
+                guesses.append(C @ R @ np.linalg.inv(C) @ P)
+    guesses = np.stack(guesses)
     print_case(test="gate", pairs=len(cyc.pairs), hyp=H)
     c = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=guesses, n_hyp=H,
                   point_to_plane=True, hyp_gate=True, **KW)
@@ -171,16 +186,13 @@ def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
     r = c.results()
     inst = c.instances()
     for i, (s, t) in enumerate(cyc.pairs):
-        score = []
-        for h in range(H):
-            idx, d2 = oracle.correspondences(cyc.clouds[s], cyc.clouds[t], guesses[i * H + h],
-                                             KW["max_corr_dist"])
-            ok = idx >= 0
-            score.append((-int(ok.sum()), float(d2[ok].astype(np.float64).mean()) if ok.any() else 0.0, h))
-        want = min(score)[2]
-        assert r["best_hyp"][i] == want, (i, score)
+        counts = [int((oracle.correspondences(cyc.clouds[s], cyc.clouds[t], guesses[i * H + h],
+                                              KW["max_corr_dist"])[0] >= 0).sum())
+                  for h in range(H)]
+        assert counts[1] > 1.2 * max(counts[0], counts[2], counts[3]), counts
+        assert r["best_hyp"][i] == 1, (i, counts)
         refined = [h for h in range(H) if inst["iterations"][i * H + h] > 0]
-        assert refined == [want]
-        icp = single_pair(lcr, cyc.clouds[s], cyc.clouds[t], normals[t], guesses[i * H + want])
+        assert refined == [1]
+        icp = single_pair(lcr, cyc.clouds[s], cyc.clouds[t], normals[t], guesses[i * H + 1])
         np.testing.assert_array_equal(r["T"][i], icp.getFinalTransformation())
         assert r["fitness"][i] == icp.getFitnessScore()
