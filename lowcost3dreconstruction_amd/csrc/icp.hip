@@ -247,7 +247,9 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
     const PairDesc pd = b.pairs[p];
     // the gate's score (per_instance) counts every kGateStride-th tile of the
     // source: the same source points for every hypothesis of a pair
-    const bool sampled = !per_instance || (tile - pd.tile_begin) % kGateStride == 0;
+    const int stride = per_instance ? kGateStride : 1;
+    if ((tile - pd.tile_begin) % stride != 0) continue;  // uniform across the block
+    const int nparts = (pd.ntiles + stride - 1) / stride;  // tiles taking part
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
     const double max_range = per_instance ? pd.max_corr_d2 : max_range_arg;
@@ -256,7 +258,7 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
     const float r = rq * 1.00001f;
     const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
     double v[2] = {0.0, 0.0};
-    const bool own = sampled && local < gs.npts;
+    const bool own = local < gs.npts;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
     if (own) {
@@ -280,14 +282,14 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
       out[1] = v[1];
       __threadfence();
       const unsigned t = atomicAdd(&st->ticket, 1u);
-      s_last = (t == (unsigned)(pd.ntiles - 1));
+      s_last = (t == (unsigned)(nparts - 1));
     }
     __syncthreads();
     if (s_last) {
       __threadfence();
       v[0] = v[1] = 0.0;
-      for (int t = threadIdx.x; t < pd.ntiles; t += TILE) {
-        const double* in = b.partials + (int64_t)(pd.tile_begin + t) * kMaxVals;
+      for (int t = threadIdx.x; t < nparts; t += TILE) {
+        const double* in = b.partials + (int64_t)(pd.tile_begin + t * stride) * kMaxVals;
         v[0] += in[0];
         v[1] += in[1];
       }

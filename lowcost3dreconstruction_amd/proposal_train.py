@@ -131,9 +131,11 @@ def make_eval_pairs(lcr, prop, n, npts=8192, seed=1234, min_angle_deg=0.0, max_a
 
 
 def ablation(lcr, prop, src, tgt, T_gt, max_corr_dist=0.1, max_iterations=60,
-             ok_rot_deg=1.0, ok_trans=0.005):
+             ok_rot_deg=1.0, ok_trans=0.005, hyp_gate=False):
     """Plain point-to-plane ICP from the identity vs the same ICP with the learned
-    hypothesis added (best by fitness).  Returns per-pair errors and iterations."""
+    hypothesis added (both refined and the best by final fitness kept, or with
+    hyp_gate only the one with more inliers at its initial pose refined).  Returns
+    per-pair success, errors and iterations."""
     n = len(src)
     clouds = [c for pair in zip(src, tgt) for c in pair]
     pairs = [(2 * i, 2 * i + 1) for i in range(n)]
@@ -142,7 +144,7 @@ def ablation(lcr, prop, src, tgt, T_gt, max_corr_dist=0.1, max_iterations=60,
               normals_k=30)
     out = {}
     for name, p in (("plain", None), ("learned", prop)):
-        c = lcr.Cycle(clouds, pairs, proposal=p, **kw)
+        c = lcr.Cycle(clouds, pairs, proposal=p, hyp_gate=hyp_gate and p is not None, **kw)
         c.run()
         r = c.results()
         err = np.array([synth.transform_error(r["T"][i], T_gt[i]) for i in range(n)])

@@ -158,10 +158,10 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
 
 def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
     """With hyp_gate, each pair's hypotheses are scored by their source points
-    within r at the initial pose (on a fixed subsample of the source) and only the
-    best is refined: bitwise the single-pair ICP from that pose.  Here the prior
-    (hypothesis 1) has far more inliers than three poses rotated by 15-25 deg, so
-    it must be the one kept."""
+    within r at the initial pose (on a fixed 1/8 subsample of the source) and only
+    the best is refined: bitwise the single-pair ICP from that pose.  Against the
+    oracle's full inlier counts: a clear winner (1.5x the runner-up) must be the
+    one kept, otherwise the kept pose has at least 2/3 of the best count."""
     from lowcost3dreconstruction_amd.synth import perturb, rigid
     cyc, normals = small_cycle
     H = 4
@@ -189,10 +189,14 @@ def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
         counts = [int((oracle.correspondences(cyc.clouds[s], cyc.clouds[t], guesses[i * H + h],
                                               KW["max_corr_dist"])[0] >= 0).sum())
                   for h in range(H)]
-        assert counts[1] > 1.2 * max(counts[0], counts[2], counts[3]), counts
-        assert r["best_hyp"][i] == 1, (i, counts)
+        order = np.argsort(counts)[::-1]
+        kept = int(r["best_hyp"][i])
+        if counts[order[0]] > 1.5 * counts[order[1]]:
+            assert kept == order[0], (i, counts, kept)
+        else:
+            assert counts[kept] >= (2 / 3) * counts[order[0]], (i, counts, kept)
         refined = [h for h in range(H) if inst["iterations"][i * H + h] > 0]
-        assert refined == [1]
-        icp = single_pair(lcr, cyc.clouds[s], cyc.clouds[t], normals[t], guesses[i * H + 1])
+        assert refined == [kept]
+        icp = single_pair(lcr, cyc.clouds[s], cyc.clouds[t], normals[t], guesses[i * H + kept])
         np.testing.assert_array_equal(r["T"][i], icp.getFinalTransformation())
         assert r["fitness"][i] == icp.getFitnessScore()
