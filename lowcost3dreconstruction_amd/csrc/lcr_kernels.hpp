@@ -91,6 +91,56 @@ void launch_normals(const GridDesc* grids, const int* cell_start, const float4* 
                     int flip_all, double* mom, int* nsel, int* deferred, float4* nrm_sorted,
                     hipStream_t s);
 
+// ---- K11-K13 filters before ICP (filters.hip)
+// VoxelGrid lattice: voxel of p = floor(p * inv) - minb (FP32, per axis); index
+// x + y * div[0] + z * div[0] * div[1].
+struct VoxLattice {
+  float inv;
+  int minb[3];
+  int64_t div[3];
+};
+// device scratch of one VoxelGrid pass: key (n int64), slot_raw / sorted_raw
+// (finite count ints), counts / start / flags / outidx (nvox ints), scan_tmp
+// (scan_tmp_ints(nvox) ints)
+struct VoxScratch {
+  int64_t* key;
+  int* slot_raw;
+  int* sorted_raw;
+  int* counts;
+  int* start;
+  int* flags;
+  int* outidx;
+  int* scan_tmp;
+};
+void launch_voxel_grid(const float4* raw, const float4* raw_nrm, const float* curv,
+                       const uint8_t* rgb, int64_t n, const VoxLattice& L, int64_t nvox,
+                       int64_t nfin, VoxScratch w, float* out_xyz, float* out_nrm, float* out_curv,
+                       uint8_t* out_rgb, hipStream_t s);
+// StatisticalOutlierRemoval statistics and keep flags from per-point mean
+// distances (finite[i] = 1 for points that count); part holds 3 doubles per block
+// of sor_stat_blocks(n), stats 3 doubles (mean, stddev, threshold).
+int sor_stat_blocks(int64_t n);
+void launch_sor_stats(const float* dist, const uint8_t* finite, int64_t n, double mul, double* part,
+                      double* stats, uint8_t* keep, hipStream_t s);
+// depth back-projection (16-bit millimetre depth, row-major w x h)
+struct DepthParams {
+  double fx, fy, cx, cy;
+  int depth_min, depth_max, top, bottom, left, right;
+};
+void launch_depth_flags(const uint16_t* depth, int w, int h, const DepthParams& P, int* flags,
+                        hipStream_t s);
+void launch_depth_points(const uint16_t* depth, int w, int h, const uint8_t* color,
+                         const DepthParams& P, const int* pos, float* out_xyz, float* out_nrm,
+                         uint8_t* out_rgb, hipStream_t s);
+
+// StatisticalOutlierRemoval distances (normals.hip's selection with mean_k + 1
+// neighbours): dist / finite per input point (the caller zeroes both first);
+// sums holds one double per sorted point, deferred npts_total + 1 ints.
+void launch_sor_distances(const GridDesc* grids, const int* cell_start, const float4* pts,
+                          const int* cloud_of_sorted, const int64_t* cloud_off, int64_t npts_total,
+                          int mean_k, double* sums, int* nsel, int* deferred, float* dist,
+                          uint8_t* finite, hipStream_t s);
+
 // ---- K9 pose-proposal network (encoder.hip)
 struct EncBuffers {
   const float4* X;       // 2 * npairs clouds of npts normalised points (src, tgt, ...)

@@ -95,6 +95,20 @@ __device__ __forceinline__ void add_moment(Moments& m, const float4& q, const fl
   m.v[8] += dz * dz;
 }
 
+// What a selection accumulates: kMoments -- FP64 moments of the neighbours
+// relative to the query (normals); kSqrtDist -- the sum of sqrt(d2) over the
+// neighbours in FP64 (StatisticalOutlierRemoval's mean distance).
+constexpr int kMoments = 0;
+constexpr int kSqrtDist = 1;
+
+template <int MODE>
+__device__ __forceinline__ void take(Moments& m, const float4& q, const float4& p, float d2) {
+  if constexpr (MODE == kMoments)
+    add_moment(m, q, p);
+  else
+    m.v[0] += sqrt((double)d2);
+}
+
 // Calls f(j) for this lane's share of the points of rings 0..K.
 template <class F>
 __device__ __forceinline__ void for_block_points(const GridDesc& g, const int* cs, const CellPos& cp,
@@ -116,6 +130,7 @@ __device__ __forceinline__ int block_count(const GridDesc& g, const int* cs, con
 }
 
 // Exact path without an LDS buffer: every count rescans the block from memory.
+template <int MODE>
 __device__ void select_rescan(const GridDesc& g, const int* cs, const float4* pts, const float4& q,
                               const CellPos& cp, int K, int kk, Moments& acc) {
   unsigned T = 0;
@@ -172,7 +187,7 @@ __device__ void select_rescan(const GridDesc& g, const int* cs, const float4* pt
   for_block_points(g, cs, cp, K, [&](int j) {
     const float4 p = pts[j];
     const unsigned d = __float_as_uint(flann_d2(q.x, q.y, q.z, p));
-    if (d < T || (d == T && __float_as_int(p.w) <= idcut)) add_moment(acc, q, p);
+    if (d < T || (d == T && __float_as_int(p.w) <= idcut)) take<MODE>(acc, q, p, __uint_as_float(d));
   });
 }
 
@@ -251,6 +266,7 @@ __device__ __forceinline__ void stage_block(const GridDesc& g, const int* cs, co
 
 // Returns false (acc untouched) when the candidates overflow LDS; the query is
 // then left to the rescan pass.
+template <int MODE>
 __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, const float4* pts,
                                               const float4& q, int kk, float4* sp, Staged& st,
                                               Moments& acc) {
@@ -331,19 +347,22 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
     const int i = lane + 64 * m;
     if (i < st.cnt && slot[m] <= T) {
       const float4 p = sp[i];
-      if (slot[m] < T || __float_as_int(p.w) <= idcut) add_moment(acc, q, p);
+      if (slot[m] < T || __float_as_int(p.w) <= idcut) take<MODE>(acc, q, p, __uint_as_float(slot[m]));
     }
   }
   return true;
 }
 
-// Search pass: the FP64 moment sums of each point's K-neighbourhood, 9 doubles
-// per point (and the neighbour count), in cell-sorted order.
+// Search pass: per point of its K-neighbourhood either the 9 FP64 moment sums
+// (kMoments, mom holds 9 doubles per point) or the sum of distances (kSqrtDist,
+// mom holds 1), and the neighbour count, in cell-sorted order.
+template <int MODE>
 __global__ void __launch_bounds__(kWaves * 64)
 k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
                  const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
                  int64_t total, int k, double* __restrict__ mom, int* __restrict__ nsel,
                  int* __restrict__ deferred) {
+  constexpr int NV = MODE == kMoments ? 9 : 1;
   __shared__ float4 s_p[kWaves][kCap];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 64;
@@ -370,7 +389,8 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
     Moments acc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
-    if (kk >= 3 && !select_staged(g, cell_start + g.cell_off, pts, q, kk, s_p[w], st, acc)) {
+    if (kk >= (MODE == kMoments ? 3 : 1) &&
+        !select_staged<MODE>(g, cell_start + g.cell_off, pts, q, kk, s_p[w], st, acc)) {
       // deferred[0] counts the list deferred[1..]
       if (lane == 0) deferred[1 + atomicAdd(deferred, 1)] = (int)(base + qi);
       if (lane == qi) mine_n = kk;
@@ -378,11 +398,11 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
     }
     double mv = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < NV; ++i) {
       const double s = wave_sum_dpp(acc.v[i]);
       if (lane == i) mv = s;
     }
-    if (lane < 9) mom[(base + qi) * 9 + lane] = mv;
+    if (lane < NV) mom[(base + qi) * NV + lane] = mv;
     if (lane == qi) mine_n = kk;
   }
   if (own) nsel[jq] = mine_n;
@@ -390,10 +410,12 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
 
 // Rescan pass for the deferred queries (candidate blocks too large for LDS), one
 // wavefront per query, reading the block from memory for every count.
+template <int MODE>
 __global__ void __launch_bounds__(256)
 k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
                 const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted, int k,
                 const int* __restrict__ deferred, double* __restrict__ mom) {
+  constexpr int NV = MODE == kMoments ? 9 : 1;
   const int lane = threadIdx.x & 63;
   const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
   const int count = deferred[0];
@@ -406,14 +428,14 @@ k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell
     Moments acc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
-    select_rescan(g, cell_start + g.cell_off, pts, q, cell_pos(g, q.x, q.y, q.z), 1, kk, acc);
+    select_rescan<MODE>(g, cell_start + g.cell_off, pts, q, cell_pos(g, q.x, q.y, q.z), 1, kk, acc);
     double mv = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < NV; ++i) {
       const double s = wave_sum_dpp(acc.v[i]);
       if (lane == i) mv = s;
     }
-    if (lane < 9) mom[j * 9 + lane] = mv;
+    if (lane < NV) mom[j * NV + lane] = mv;
   }
 }
 
@@ -466,7 +488,36 @@ k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_
   out[jq] = make_float4((float)nx, (float)ny, (float)nz, (float)curv);
 }
 
+// StatisticalOutlierRemoval's value per input point: sum / mean_k in FP64 rounded
+// to float at the point's input position; finite flag for the statistics.
+__global__ void k_sor_dist(const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
+                           const int64_t* __restrict__ cloud_off, const double* __restrict__ sums,
+                           int64_t total, int mean_k, float* __restrict__ dist,
+                           uint8_t* __restrict__ finite) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  const int64_t i = cloud_off[cloud_of_sorted[j]] + __float_as_int(pts[j].w);
+  dist[i] = (float)(sums[j] / (double)mean_k);
+  finite[i] = 1;
+}
+
 }  // namespace
+
+void launch_sor_distances(const GridDesc* grids, const int* cell_start, const float4* pts,
+                          const int* cloud_of_sorted, const int64_t* cloud_off, int64_t npts_total,
+                          int mean_k, double* sums, int* nsel, int* deferred, float* dist,
+                          uint8_t* finite, hipStream_t s) {
+  if (npts_total <= 0) return;
+  const int64_t per_block = (int64_t)kWaves * 64;
+  const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
+  launch_fill_i32(deferred, 1, 0, s);
+  hipLaunchKernelGGL(k_normal_moments<kSqrtDist>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
+                     cell_start, pts, cloud_of_sorted, npts_total, mean_k + 1, sums, nsel, deferred);
+  hipLaunchKernelGGL(k_normal_rescan<kSqrtDist>, dim3(1024), dim3(256), 0, s, grids, cell_start,
+                     pts, cloud_of_sorted, mean_k + 1, deferred, sums);
+  hipLaunchKernelGGL(k_sor_dist, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s, pts,
+                     cloud_of_sorted, cloud_off, sums, npts_total, mean_k, dist, finite);
+}
 
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
@@ -476,9 +527,9 @@ void launch_normals(const GridDesc* grids, const int* cell_start, const float4* 
   const int64_t per_block = (int64_t)kWaves * 64;
   const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
   launch_fill_i32(deferred, 1, 0, s);
-  hipLaunchKernelGGL(k_normal_moments, dim3(blocks), dim3(kWaves * 64), 0, s, grids, cell_start,
-                     pts, cloud_of_sorted, npts_total, k, mom, nsel, deferred);
-  hipLaunchKernelGGL(k_normal_rescan, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
+  hipLaunchKernelGGL(k_normal_moments<kMoments>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
+                     cell_start, pts, cloud_of_sorted, npts_total, k, mom, nsel, deferred);
+  hipLaunchKernelGGL(k_normal_rescan<kMoments>, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
                      cloud_of_sorted, k, deferred, mom);
   hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s,
                      pts, cloud_of_sorted, mom, nsel, npts_total, vp, flip_all, nrm_sorted);

@@ -104,17 +104,20 @@ WEIGHTS = os.path.join(ROOT, "lowcost3dreconstruction_amd", "models", "proposal_
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained proposal weights committed")
 def test_learned_proposal_gains_over_plain_icp(lcr):
     """Held-out pairs (new shapes and motions, 20-45 deg / up to 10 cm initial
-    error): point-to-plane ICP from the identity versus the same batched ICP with
-    the trained network's hypothesis added (best by fitness).  The learned
-    hypothesis must align clearly more pairs, and the fitness fallback must keep
-    almost every pair that plain ICP already aligned."""
+    error), point-to-plane ICP with a 1 cm radius: from the identity alone versus
+    with the trained network's hypothesis added and the gate refining the pose with
+    more inliers.  The learned hypothesis must align clearly more pairs in fewer
+    iterations, and the fallback must keep almost every pair plain ICP aligned.
+    (Measured on this set: 0.71 -> 0.98 aligned, 13.6 -> 7.4 iterations.)"""
     from lowcost3dreconstruction_amd import proposal_train as pt
     prop = lcr.Proposal(np.load(WEIGHTS))
     print_case(test="proposal_ablation")
     src, tgt, T_gt = pt.make_eval_pairs(lcr, prop, 128, seed=4242, min_angle_deg=20.0,
                                         max_angle_deg=45.0)
-    res = pt.ablation(lcr, prop, src, tgt, T_gt)
+    res = pt.ablation(lcr, prop, src, tgt, T_gt, max_corr_dist=0.01, hyp_gate=True)
     plain, learned = res["plain"], res["learned"]
-    print(f"success plain {plain['success']:.3f} learned {learned['success']:.3f}")
+    print(f"success plain {plain['success']:.3f} learned {learned['success']:.3f}, iterations "
+          f"{plain['iterations'].mean():.2f} -> {learned['iterations'].mean():.2f}")
     assert learned["success"] >= plain["success"] + 0.15
+    assert learned["iterations"].mean() <= 0.8 * plain["iterations"].mean()
     assert np.mean(plain["ok"] & ~learned["ok"]) <= 0.03
