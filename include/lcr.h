@@ -199,6 +199,43 @@ int lcr_cycle_autotune(lcr_cycle *c, const char *cache_path, lcr_tune_result *ou
 int lcr_transform_points(const float *xyz, const float *nrm, int64_t n, int64_t stride_bytes,
                          const double T[16], float *out_xyz, float *out_nrm);
 
+/* ---- the steps before ICP (VoxelGrid, outlier removal, depth) ---- */
+
+/* pcl::VoxelGrid<PointXYZRGBNormal> with setLeafSize(leaf, leaf, leaf) and filter
+ * (pcl_tools/cloud_downsampling.cpp:73-75): one point per occupied voxel of the
+ * lattice floor(p / leaf), in increasing voxel index (x fastest): xyz mean, normal
+ * mean normalised, curvature mean, per-channel rgb mean truncated (nrm, curv, rgb
+ * and their outputs nullable; xyz/nrm/rgb packed, 3 per point).  Output buffers
+ * hold up to n points; *out_n receives the count.  A lattice of more than
+ * INT32_MAX voxels returns the input unchanged, as PCL does; one of more than
+ * 2^30 voxels fails with LCR_E_LIMIT. */
+int lcr_voxel_grid(const float *xyz, const float *nrm, const float *curv, const uint8_t *rgb,
+                   int64_t n, float leaf, float *out_xyz, float *out_nrm, float *out_curv,
+                   uint8_t *out_rgb, int64_t *out_n);
+
+/* pcl::StatisticalOutlierRemoval with setMeanK(mean_k), setStddevMulThresh(mult)
+ * and filter (pcl_tools/outlier_removal.cpp:80-84): per finite point the mean
+ * distance to its mean_k nearest neighbours; mean and sample stddev over finite
+ * points; keep[i] = 1 when the point's value is <= mean + mult * stddev
+ * (non-finite points count 0 and are kept, as PCL's loop leaves them).
+ * mean_dist (nullable): per-point values; stats (nullable): mean, stddev,
+ * threshold; kept (nullable): number kept.  setNegative(true) is keep[i] == 0. */
+int lcr_outlier_removal(const float *xyz, int64_t n, int32_t mean_k, double stddev_mult,
+                        uint8_t *keep, float *mean_dist, double *stats, int64_t *kept);
+
+/* The depth tool's back-projection (capture/depth_capture/depth_capture.cpp:230-305):
+ * a width x height 16-bit depth map (millimetres, row-major) becomes points
+ * (z (j - cx) / fx, -z (i - cy) / fy, -z) for pixels with depth_min <= z <=
+ * depth_max, z != 0 and (i, j) inside crop = {top, bottom, left, right} (NULL =
+ * whole image), in row-major order, with cross-product normals (neighbours that
+ * differ by more than 20 mm ignored) and the pixel's colour (color: width x height
+ * x 3 bytes, nullable).  intrinsics = {fx, fy, cx, cy}.  Outputs hold up to
+ * width * height points. */
+int lcr_depth_to_cloud(const uint16_t *depth, int32_t width, int32_t height, const uint8_t *color,
+                       const double intrinsics[4], int32_t depth_min, int32_t depth_max,
+                       const int32_t crop[4], float *out_xyz, float *out_nrm, uint8_t *out_rgb,
+                       int64_t *out_n);
+
 /* One-shot: create, run, results, destroy. */
 int lcr_icp_batch_align(const lcr_cycle_desc *d, lcr_cycle_result *r);
 

@@ -135,6 +135,21 @@ using Matrix4d = Matrix4<double>;
 
 // Eigen's default IOFormat: every coefficient printed at the stream's precision,
 // right-aligned to the widest one, separated by one space, one row per line.
+// The cloud summary the reference's filter tools print with std::cout << *cloud
+// (cloud_downsampling.cpp:69-78): header, point count, width, height, density,
+// sensor origin and orientation.
+template <class PointT>
+std::ostream& operator<<(std::ostream& os, const PointCloud<PointT>& c) {
+  os << "header: seq: 0 stamp: 0 frame_id: \n";
+  os << "points[]: " << c.points.size() << "\n";
+  os << "width: " << c.width << "\n";
+  os << "height: " << c.height << "\n";
+  os << "is_dense: " << (c.is_dense ? 1 : 0) << "\n";
+  os << "sensor origin (xyz): [" << c.sensor_origin[0] << ", " << c.sensor_origin[1] << ", "
+     << c.sensor_origin[2] << "] / orientation (xyzw): [0, 0, 0, 1]\n";
+  return os;
+}
+
 template <class Scalar>
 std::ostream& operator<<(std::ostream& os, const Matrix4<Scalar>& M) {
   std::streamsize width = 0;

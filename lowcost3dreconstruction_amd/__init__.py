@@ -44,7 +44,8 @@ EXPORTS = (
     "lcr_proposal_create", "lcr_proposal_destroy", "lcr_proposal_set_weights",
     "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
     "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
-    "lcr_proposal_predict", "lcr_cycle_autotune", "lcr_transform_points",
+    "lcr_proposal_predict", "lcr_cycle_autotune", "lcr_transform_points", "lcr_voxel_grid",
+    "lcr_outlier_removal", "lcr_depth_to_cloud",
 )
 
 
@@ -159,6 +160,12 @@ def _load():
     L.lcr_proposal_predict.argtypes = [vp, C.c_int32, C.c_int32, fp, fp]
     L.lcr_cycle_autotune.argtypes = [vp, C.c_char_p, C.POINTER(TuneResult)]
     L.lcr_transform_points.argtypes = [fp, fp, C.c_int64, C.c_int64, dp, fp, fp]
+    u8 = C.POINTER(C.c_uint8)
+    i64p = C.POINTER(C.c_int64)
+    L.lcr_voxel_grid.argtypes = [fp, fp, fp, u8, C.c_int64, C.c_float, fp, fp, fp, u8, i64p]
+    L.lcr_outlier_removal.argtypes = [fp, C.c_int64, C.c_int32, C.c_double, u8, fp, dp, i64p]
+    L.lcr_depth_to_cloud.argtypes = [C.POINTER(C.c_uint16), C.c_int32, C.c_int32, u8, dp,
+                                     C.c_int32, C.c_int32, C.POINTER(C.c_int32), fp, fp, u8, i64p]
     L.lcr_last_error.restype = C.c_char_p
     L.lcr_version.restype = C.c_char_p
     return L
@@ -321,6 +328,73 @@ def transform_points(xyz, T, nrm=None):
     _check(lib.lcr_transform_points(_fp(x), _fp(n) if n is not None else None, len(x), 12, _dp(M),
                                     _fp(out), _fp(out_n) if out_n is not None else None))
     return out, out_n
+
+
+def _u8p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8)) if a is not None else None
+
+
+def voxel_grid(xyz, leaf, nrm=None, curv=None, rgb=None):
+    """pcl::VoxelGrid with setLeafSize(leaf, leaf, leaf) (cloud_downsampling.cpp:73-75)
+    on the device.  Returns a dict: xyz and, when given, nrm / curv / rgb of the
+    voxel centroids in increasing voxel index."""
+    x = _f32(xyz)
+    n = len(x)
+    nm = _f32(nrm) if nrm is not None else None
+    cv = np.ascontiguousarray(curv, np.float32).reshape(-1) if curv is not None else None
+    cl = np.ascontiguousarray(rgb, np.uint8).reshape(-1, 3) if rgb is not None else None
+    ox = np.empty((max(n, 1), 3), np.float32)
+    on = np.empty_like(ox) if nm is not None else None
+    oc = np.empty(max(n, 1), np.float32) if cv is not None else None
+    orgb = np.empty((max(n, 1), 3), np.uint8) if cl is not None else None
+    m = C.c_int64()
+    _check(lib.lcr_voxel_grid(_fp(x), _fp(nm), _fp(cv), _u8p(cl), n, float(leaf), _fp(ox), _fp(on),
+                              _fp(oc), _u8p(orgb), C.byref(m)))
+    k = m.value
+    out = {"xyz": ox[:k]}
+    if on is not None:
+        out["nrm"] = on[:k]
+    if oc is not None:
+        out["curv"] = oc[:k]
+    if orgb is not None:
+        out["rgb"] = orgb[:k]
+    return out
+
+
+def outlier_removal(xyz, mean_k=50, stddev_mult=1.0):
+    """pcl::StatisticalOutlierRemoval (outlier_removal.cpp:80-84) on the device:
+    (keep (N,) bool, per-point mean neighbour distance (N,) float32,
+    (mean, stddev, threshold))."""
+    x = _f32(xyz)
+    keep = np.empty(len(x), np.uint8)
+    dist = np.empty(len(x), np.float32)
+    stats = np.empty(3)
+    kept = C.c_int64()
+    _check(lib.lcr_outlier_removal(_fp(x), len(x), int(mean_k), float(stddev_mult), _u8p(keep),
+                                   _fp(dist), _dp(stats), C.byref(kept)))
+    return keep.astype(bool), dist, tuple(stats)
+
+
+def depth_to_cloud(depth, intrinsics=(572.882768, 542.739980, 314.649173, 240.160459),
+                   depth_min=0, depth_max=65535, crop=None, color=None):
+    """The depth tool's back-projection with cross-product normals
+    (depth_capture.cpp:230-305) on the device.  depth: (H, W) uint16 millimetres;
+    the default intrinsics are the tool's.  Returns (xyz, normals, rgb or None)."""
+    d = np.ascontiguousarray(depth, dtype=np.uint16)
+    h, w = d.shape
+    col = np.ascontiguousarray(color, np.uint8).reshape(h, w, 3) if color is not None else None
+    intr = np.ascontiguousarray(intrinsics, np.float64)
+    cr = np.ascontiguousarray(crop, np.int32) if crop is not None else None
+    ox = np.empty((max(h * w, 1), 3), np.float32)
+    on = np.empty_like(ox)
+    orgb = np.empty((max(h * w, 1), 3), np.uint8) if col is not None else None
+    m = C.c_int64()
+    _check(lib.lcr_depth_to_cloud(d.ctypes.data_as(C.POINTER(C.c_uint16)), w, h, _u8p(col),
+                                  _dp(intr), int(depth_min), int(depth_max),
+                                  cr.ctypes.data_as(C.POINTER(C.c_int32)) if cr is not None else None,
+                                  _fp(ox), _fp(on), _u8p(orgb), C.byref(m)))
+    k = m.value
+    return ox[:k], on[:k], (orgb[:k] if orgb is not None else None)
 
 
 class NormalEstimation:

@@ -8,6 +8,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <functional>
 #include <memory>
@@ -652,6 +653,207 @@ int lcr_transform_points(const float* xyz, const float* nrm, int64_t n, int64_t 
     if (nrm && out_nrm)
       LCR_HIP(hipMemcpyAsync(out_nrm, dn, n * 3 * sizeof(float), hipMemcpyDeviceToHost, st.s));
     LCR_HIP(hipStreamSynchronize(st.s));
+    return LCR_OK;
+  });
+}
+
+int lcr_voxel_grid(const float* xyz, const float* nrm, const float* curv, const uint8_t* rgb,
+                   int64_t n, float leaf, float* out_xyz, float* out_nrm, float* out_curv,
+                   uint8_t* out_rgb, int64_t* out_n) {
+  return guarded([&] {
+    if (n < 0 || !out_n || (n > 0 && (!xyz || !out_xyz))) return fail(LCR_E_ARG, "invalid arguments");
+    if (!(leaf > 0.0f) || !std::isfinite(leaf)) return fail(LCR_E_ARG, "leaf must be > 0");
+    if (n > kMaxCloudPoints) return fail(LCR_E_LIMIT, "cloud larger than 2^24 points");
+    *out_n = 0;
+    if (n == 0) return LCR_OK;
+    Stream st;
+    CloudSet cs;
+    cs.upload(st.s, {xyz}, {nrm}, {n}, 3 * sizeof(float));
+    DevBuf part, box;
+    part.reserve((size_t)kBBoxChunks * 7 * sizeof(float));
+    box.reserve(7 * sizeof(float));
+    launch_bbox(cs.d_raw.as<float4>(), cs.d_cloud_off.as<int64_t>(), 1, part.as<float>(),
+                box.as<float>(), st.s);
+    float b[7];
+    LCR_HIP(hipMemcpyAsync(b, box.p, sizeof b, hipMemcpyDeviceToHost, st.s));
+    LCR_HIP(hipStreamSynchronize(st.s));
+    const int64_t nfin = (int64_t)b[6];
+    if (nfin == 0) return LCR_OK;
+    // voxel_grid.hpp: the lattice from the min/max over finite points, FP32
+    const float inv = 1.0f / leaf;
+    int64_t d[3];
+    for (int a = 0; a < 3; ++a) d[a] = (int64_t)((b[3 + a] - b[a]) * inv) + 1;
+    if (d[0] * d[1] * d[2] > (int64_t)INT32_MAX) {
+      // PCL leaves the cloud unchanged when the lattice overflows an int
+      memcpy(out_xyz, xyz, (size_t)n * 3 * sizeof(float));
+      if (out_nrm && nrm) memcpy(out_nrm, nrm, (size_t)n * 3 * sizeof(float));
+      if (out_curv && curv) memcpy(out_curv, curv, (size_t)n * sizeof(float));
+      if (out_rgb && rgb) memcpy(out_rgb, rgb, (size_t)n * 3);
+      *out_n = n;
+      return LCR_OK;
+    }
+    VoxLattice L;
+    int64_t nvox = 1;
+    L.inv = inv;
+    for (int a = 0; a < 3; ++a) {
+      L.minb[a] = (int)floorf(b[a] * inv);
+      const int maxb = (int)floorf(b[3 + a] * inv);
+      L.div[a] = (int64_t)maxb - L.minb[a] + 1;
+      nvox *= L.div[a];
+    }
+    if (nvox > ((int64_t)1 << 30)) return fail(LCR_E_LIMIT, "voxel lattice larger than 2^30");
+    DevBuf key, slot_raw, sorted_raw, counts, start, flags, outidx, scan_tmp, dcurv, drgb, ox, on,
+        oc, orgb;
+    key.reserve((size_t)n * sizeof(int64_t));
+    slot_raw.reserve((size_t)nfin * sizeof(int));
+    sorted_raw.reserve((size_t)nfin * sizeof(int));
+    for (DevBuf* v : {&counts, &start, &flags, &outidx}) v->reserve((size_t)nvox * sizeof(int));
+    scan_tmp.reserve((size_t)scan_tmp_ints(nvox) * sizeof(int));
+    if (curv) {
+      dcurv.reserve((size_t)n * sizeof(float));
+      LCR_HIP(hipMemcpyAsync(dcurv.p, curv, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    }
+    if (rgb) {
+      drgb.reserve((size_t)n * 3);
+      LCR_HIP(hipMemcpyAsync(drgb.p, rgb, (size_t)n * 3, hipMemcpyHostToDevice, st.s));
+    }
+    ox.reserve((size_t)nfin * 3 * sizeof(float));
+    if (nrm && out_nrm) on.reserve((size_t)nfin * 3 * sizeof(float));
+    if (curv && out_curv) oc.reserve((size_t)nfin * sizeof(float));
+    if (rgb && out_rgb) orgb.reserve((size_t)nfin * 3);
+    VoxScratch w{key.as<int64_t>(), slot_raw.as<int>(), sorted_raw.as<int>(), counts.as<int>(),
+                 start.as<int>(), flags.as<int>(), outidx.as<int>(), scan_tmp.as<int>()};
+    launch_voxel_grid(cs.d_raw.as<float4>(), (nrm && cs.has_nrm) ? cs.d_raw_nrm.as<float4>() : nullptr,
+                      curv ? dcurv.as<float>() : nullptr, rgb ? drgb.as<uint8_t>() : nullptr, n, L,
+                      nvox, nfin, w, ox.as<float>(), on.p ? on.as<float>() : nullptr,
+                      oc.p ? oc.as<float>() : nullptr, orgb.p ? orgb.as<uint8_t>() : nullptr, st.s);
+    int last[2];
+    LCR_HIP(hipMemcpyAsync(&last[0], outidx.as<int>() + nvox - 1, sizeof(int), hipMemcpyDeviceToHost,
+                           st.s));
+    LCR_HIP(hipMemcpyAsync(&last[1], counts.as<int>() + nvox - 1, sizeof(int), hipMemcpyDeviceToHost,
+                           st.s));
+    LCR_HIP(hipStreamSynchronize(st.s));
+    const int64_t nocc = (int64_t)last[0] + (last[1] > 0 ? 1 : 0);
+    LCR_HIP(hipMemcpyAsync(out_xyz, ox.p, (size_t)nocc * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                           st.s));
+    if (on.p)
+      LCR_HIP(hipMemcpyAsync(out_nrm, on.p, (size_t)nocc * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                             st.s));
+    if (oc.p)
+      LCR_HIP(hipMemcpyAsync(out_curv, oc.p, (size_t)nocc * sizeof(float), hipMemcpyDeviceToHost,
+                             st.s));
+    if (orgb.p)
+      LCR_HIP(hipMemcpyAsync(out_rgb, orgb.p, (size_t)nocc * 3, hipMemcpyDeviceToHost, st.s));
+    LCR_HIP(hipStreamSynchronize(st.s));
+    *out_n = nocc;
+    return LCR_OK;
+  });
+}
+
+int lcr_outlier_removal(const float* xyz, int64_t n, int32_t mean_k, double stddev_mult,
+                        uint8_t* keep, float* mean_dist, double* stats, int64_t* kept) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!xyz || !keep))) return fail(LCR_E_ARG, "invalid arguments");
+    if (mean_k < 1) return fail(LCR_E_ARG, "mean_k must be >= 1");
+    if (n > kMaxCloudPoints) return fail(LCR_E_LIMIT, "cloud larger than 2^24 points");
+    if (kept) *kept = 0;
+    if (n == 0) return LCR_OK;
+    Stream st;
+    CloudSet cs;
+    cs.upload(st.s, {xyz}, {nullptr}, {n}, 3 * sizeof(float));
+    cs.build(st.s, std::max(2.0, 0.75 * sqrt((double)mean_k + 1.0)), nullptr);
+    DevBuf dist, fin, sums, nsel, defer, part, dstats, dkeep;
+    dist.reserve((size_t)n * sizeof(float));
+    fin.reserve((size_t)n);
+    const size_t np = (size_t)std::max<int64_t>(cs.total_pts, 1);
+    sums.reserve(np * sizeof(double));
+    nsel.reserve(np * sizeof(int));
+    defer.reserve((np + 1) * sizeof(int));
+    part.reserve((size_t)sor_stat_blocks(n) * 3 * sizeof(double));
+    dstats.reserve(3 * sizeof(double));
+    dkeep.reserve((size_t)n);
+    LCR_HIP(hipMemsetAsync(dist.p, 0, (size_t)n * sizeof(float), st.s));
+    LCR_HIP(hipMemsetAsync(fin.p, 0, (size_t)n, st.s));
+    launch_sor_distances(cs.d_grids.as<GridDesc>(), cs.d_cell_start.as<int>(), cs.d_pts.as<float4>(),
+                         cs.d_cloud_of_sorted.as<int>(), cs.d_cloud_off.as<int64_t>(), cs.total_pts,
+                         mean_k, sums.as<double>(), nsel.as<int>(), defer.as<int>(),
+                         dist.as<float>(), fin.as<uint8_t>(), st.s);
+    launch_sor_stats(dist.as<float>(), fin.as<uint8_t>(), n, stddev_mult, part.as<double>(),
+                     dstats.as<double>(), dkeep.as<uint8_t>(), st.s);
+    LCR_HIP(hipMemcpyAsync(keep, dkeep.p, (size_t)n, hipMemcpyDeviceToHost, st.s));
+    if (mean_dist)
+      LCR_HIP(hipMemcpyAsync(mean_dist, dist.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                             st.s));
+    double hs[3];
+    LCR_HIP(hipMemcpyAsync(hs, dstats.p, sizeof hs, hipMemcpyDeviceToHost, st.s));
+    LCR_HIP(hipStreamSynchronize(st.s));
+    if (stats) memcpy(stats, hs, sizeof hs);
+    if (kept) {
+      int64_t k = 0;
+      for (int64_t i = 0; i < n; ++i) k += keep[i] ? 1 : 0;
+      *kept = k;
+    }
+    return LCR_OK;
+  });
+}
+
+int lcr_depth_to_cloud(const uint16_t* depth, int32_t width, int32_t height, const uint8_t* color,
+                       const double intrinsics[4], int32_t depth_min, int32_t depth_max,
+                       const int32_t crop[4], float* out_xyz, float* out_nrm, uint8_t* out_rgb,
+                       int64_t* out_n) {
+  return guarded([&] {
+    if (width < 0 || height < 0 || !intrinsics || !out_n ||
+        ((int64_t)width * height > 0 && (!depth || !out_xyz)))
+      return fail(LCR_E_ARG, "invalid arguments");
+    const int64_t np = (int64_t)width * height;
+    if (np > ((int64_t)1 << 26)) return fail(LCR_E_LIMIT, "depth map larger than 2^26 pixels");
+    *out_n = 0;
+    if (np == 0) return LCR_OK;
+    DepthParams P;
+    P.fx = intrinsics[0];
+    P.fy = intrinsics[1];
+    P.cx = intrinsics[2];
+    P.cy = intrinsics[3];
+    P.depth_min = depth_min;
+    P.depth_max = depth_max;
+    P.top = crop ? crop[0] : 0;
+    P.bottom = crop ? crop[1] : height - 1;
+    P.left = crop ? crop[2] : 0;
+    P.right = crop ? crop[3] : width - 1;
+    Stream st;
+    DevBuf dd, dc, flags, pos, tmp, ox, on, orgb;
+    dd.reserve((size_t)np * sizeof(uint16_t));
+    LCR_HIP(hipMemcpyAsync(dd.p, depth, (size_t)np * sizeof(uint16_t), hipMemcpyHostToDevice, st.s));
+    if (color) {
+      dc.reserve((size_t)np * 3);
+      LCR_HIP(hipMemcpyAsync(dc.p, color, (size_t)np * 3, hipMemcpyHostToDevice, st.s));
+    }
+    flags.reserve((size_t)np * sizeof(int));
+    pos.reserve((size_t)np * sizeof(int));
+    tmp.reserve((size_t)scan_tmp_ints(np) * sizeof(int));
+    launch_depth_flags(dd.as<uint16_t>(), width, height, P, flags.as<int>(), st.s);
+    launch_exclusive_scan(flags.as<int>(), pos.as<int>(), np, tmp.as<int>(), st.s);
+    int last[2];
+    LCR_HIP(hipMemcpyAsync(&last[0], pos.as<int>() + np - 1, sizeof(int), hipMemcpyDeviceToHost, st.s));
+    LCR_HIP(hipMemcpyAsync(&last[1], flags.as<int>() + np - 1, sizeof(int), hipMemcpyDeviceToHost,
+                           st.s));
+    LCR_HIP(hipStreamSynchronize(st.s));
+    const int64_t m = (int64_t)last[0] + last[1];
+    ox.reserve((size_t)std::max<int64_t>(m, 1) * 3 * sizeof(float));
+    if (out_nrm) on.reserve((size_t)std::max<int64_t>(m, 1) * 3 * sizeof(float));
+    if (out_rgb && color) orgb.reserve((size_t)std::max<int64_t>(m, 1) * 3);
+    launch_depth_points(dd.as<uint16_t>(), width, height, color ? dc.as<uint8_t>() : nullptr, P,
+                        pos.as<int>(), ox.as<float>(), on.p ? on.as<float>() : nullptr,
+                        orgb.p ? orgb.as<uint8_t>() : nullptr, st.s);
+    if (m > 0) {
+      LCR_HIP(hipMemcpyAsync(out_xyz, ox.p, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost, st.s));
+      if (on.p)
+        LCR_HIP(hipMemcpyAsync(out_nrm, on.p, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                               st.s));
+      if (orgb.p) LCR_HIP(hipMemcpyAsync(out_rgb, orgb.p, (size_t)m * 3, hipMemcpyDeviceToHost, st.s));
+    }
+    LCR_HIP(hipStreamSynchronize(st.s));
+    *out_n = m;
     return LCR_OK;
   });
 }

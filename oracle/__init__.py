@@ -115,6 +115,14 @@ def lib():
         L.oenc_forward.argtypes = [dp, fp, fp, C.c_int, C.c_int, dp, dp, ip]
         L.oenc_loss_grad.argtypes = [dp, fp, fp, C.c_int, C.c_int, dp, C.c_int, dp]
         L.oenc_loss_grad.restype = C.c_double
+        u8 = C.POINTER(C.c_uint8)
+        L.oracle_voxel_grid.argtypes = [fp, fp, fp, u8, C.c_int64, C.c_float, fp, fp, fp, u8]
+        L.oracle_voxel_grid.restype = C.c_int64
+        L.oracle_sor.argtypes = [fp, C.c_int64, C.c_int, C.c_double, u8, fp, dp]
+        L.oracle_sor.restype = C.c_int64
+        L.oracle_depth_to_cloud.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_int, u8] + \
+            [C.c_double] * 4 + [C.c_int] * 6 + [fp, fp, u8]
+        L.oracle_depth_to_cloud.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -339,3 +347,60 @@ def enc_loss_grad(w, src, tgt, labels, quantize=True, grad=True):
                                 _p(lab, C.c_double), int(bool(quantize)),
                                 _p(g, C.c_double) if grad else None)
     return (loss, g) if grad else loss
+
+
+def voxel_grid(xyz, leaf, nrm=None, curv=None, rgb=None):
+    """pcl::VoxelGrid restated (lcr_filters_oracle.c).  Returns dict with xyz and,
+    when given as input, nrm / curv / rgb of the voxel centroids."""
+    x = _f32(xyz).reshape(-1, 3)
+    n = len(x)
+    nm = _f32(nrm).reshape(-1, 3) if nrm is not None else None
+    cv = _f32(curv).reshape(-1) if curv is not None else None
+    cl = np.ascontiguousarray(rgb, dtype=np.uint8).reshape(-1, 3) if rgb is not None else None
+    ox = np.empty((max(n, 1), 3), np.float32)
+    on = np.empty_like(ox) if nm is not None else None
+    oc = np.empty(max(n, 1), np.float32) if cv is not None else None
+    orgb = np.empty((max(n, 1), 3), np.uint8) if cl is not None else None
+    m = lib().oracle_voxel_grid(
+        _p(x, C.c_float), _p(nm, C.c_float) if nm is not None else None,
+        _p(cv, C.c_float) if cv is not None else None,
+        _p(cl, C.c_uint8) if cl is not None else None, n, float(leaf), _p(ox, C.c_float),
+        _p(on, C.c_float) if on is not None else None, _p(oc, C.c_float) if oc is not None else None,
+        _p(orgb, C.c_uint8) if orgb is not None else None)
+    out = {"xyz": ox[:m]}
+    if on is not None:
+        out["nrm"] = on[:m]
+    if oc is not None:
+        out["curv"] = oc[:m]
+    if orgb is not None:
+        out["rgb"] = orgb[:m]
+    return out
+
+
+def sor(xyz, mean_k, mul):
+    """pcl::StatisticalOutlierRemoval restated: (keep (n,) bool, per-point mean
+    distance (n,) float32, (mean, stddev, threshold))."""
+    x = _f32(xyz).reshape(-1, 3)
+    keep = np.empty(len(x), np.uint8)
+    dist = np.empty(len(x), np.float32)
+    stats = np.empty(3)
+    lib().oracle_sor(_p(x, C.c_float), len(x), int(mean_k), float(mul), _p(keep, C.c_uint8),
+                     _p(dist, C.c_float), _p(stats, C.c_double))
+    return keep.astype(bool), dist, tuple(stats)
+
+
+def depth_to_cloud(depth, intrinsics, depth_min, depth_max, crop=None, color=None):
+    """The depth tool's back-projection restated: (xyz, normals, rgb or None)."""
+    d = np.ascontiguousarray(depth, dtype=np.uint16)
+    h, w = d.shape
+    col = np.ascontiguousarray(color, dtype=np.uint8).reshape(h, w, 3) if color is not None else None
+    top, bottom, left, right = crop if crop is not None else (0, h - 1, 0, w - 1)
+    ox = np.empty((max(h * w, 1), 3), np.float32)
+    on = np.empty_like(ox)
+    orgb = np.empty((max(h * w, 1), 3), np.uint8) if col is not None else None
+    fx, fy, cx, cy = intrinsics
+    m = lib().oracle_depth_to_cloud(
+        _p(d, C.c_uint16), w, h, _p(col, C.c_uint8) if col is not None else None, fx, fy, cx, cy,
+        int(depth_min), int(depth_max), int(top), int(bottom), int(left), int(right),
+        _p(ox, C.c_float), _p(on, C.c_float), _p(orgb, C.c_uint8) if orgb is not None else None)
+    return ox[:m], on[:m], (orgb[:m] if orgb is not None else None)

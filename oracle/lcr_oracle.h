@@ -190,6 +190,17 @@ void oenc_forward(const double *w, const float *src, const float *tgt, int n, in
 double oenc_loss_grad(const double *w, const float *src, const float *tgt, int n, int batch,
                       const double *labels, int quantize, double *grad);
 
+/* ---- filters before ICP and depth back-projection (lcr_filters_oracle.c) ---- */
+int64_t oracle_voxel_grid(const float *xyz, const float *nrm, const float *curv,
+                          const uint8_t *rgb, int64_t n, float leaf, float *out_xyz,
+                          float *out_nrm, float *out_curv, uint8_t *out_rgb);
+int64_t oracle_sor(const float *xyz, int64_t n, int mean_k, double mul, uint8_t *keep,
+                   float *mean_dist, double *stats);
+int64_t oracle_depth_to_cloud(const uint16_t *depth, int w, int h, const uint8_t *color, double fx,
+                              double fy, double cx, double cy, int depth_min, int depth_max,
+                              int top, int bottom, int left, int right, float *out_xyz,
+                              float *out_nrm, uint8_t *out_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,3 +159,28 @@ def test_align_cycle_cli_chains_poses(lcr, tmp_path):
         got, _ = read_binary_ply(out / f"{name}.ply")
         exp = oracle.transform_points(c, M)
         np.testing.assert_array_equal(got[:, :3].astype(np.float32), exp.astype(np.float32))
+
+
+def test_cloud_downsampling_and_outlier_removal_cli(tmp_path):
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(0, 0.3, size=(4000, 3)).astype(np.float32)
+    pts = np.concatenate([pts, rng.uniform(2, 3, size=(10, 3)).astype(np.float32)])
+    src = tmp_path / "in.ply"
+    write_ascii_ply(src, pts)
+    print_case(test="cli_filters")
+    out = tmp_path / "down.ply"
+    r = subprocess.run([os.path.join(BIN, "cloud_downsampling"), "-i", str(src), "-o", str(out),
+                        "-s", "0.02"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Cloud before filtering:" in r.stdout and "Cloud after filtering:" in r.stdout
+    got, _ = read_binary_ply(out)
+    np.testing.assert_array_equal(got[:, :3].astype(np.float32), oracle.voxel_grid(pts, 0.02)["xyz"])
+    out2 = tmp_path / "clean.ply"
+    r = subprocess.run([os.path.join(BIN, "outlier_removal"), "-i", str(src), "-o", str(out2),
+                        "-n", "20", "-d", "1.0", "-f"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    keep, _, _ = oracle.sor(pts, 20, 1.0)
+    kept, _ = read_binary_ply(out2)
+    removed, _ = read_binary_ply(tmp_path / "clean_outliers.ply")
+    assert len(kept) == keep.sum() and len(removed) == (~keep).sum()
+    np.testing.assert_array_equal(kept[:, :3].astype(np.float32), pts[keep])

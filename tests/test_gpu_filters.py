@@ -1,0 +1,79 @@
+"""The device filters against the oracle: VoxelGrid (bitwise: same FP32 sums in
+the same order), StatisticalOutlierRemoval (per-point values within 1e-6, the same
+keep flags away from the threshold) and depth back-projection (bitwise)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import print_case
+from lowcost3dreconstruction_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def depth_cloud():
+    rng = np.random.default_rng(5)
+    shape = synth.make_shape(rng, scale=1.5)
+    T = synth.turntable_pose(0.3, (0.0, 0.0, -0.8))
+    pts = synth.depth_cloud(shape, T, leaf=0.0, rng=rng).astype(np.float32)  # no voxelisation
+    return pts
+
+
+@pytest.mark.parametrize("leaf", [0.002, 0.0047, 0.02])
+def test_voxel_grid_matches_oracle(lcr, depth_cloud, leaf):
+    rng = np.random.default_rng(int(leaf * 1e4))
+    pts = depth_cloud.copy()
+    pts[::997] = np.nan
+    nrm = rng.normal(size=pts.shape).astype(np.float32)
+    curv = rng.uniform(0, 0.3, size=len(pts)).astype(np.float32)
+    rgb = rng.integers(0, 256, size=pts.shape).astype(np.uint8)
+    print_case(test="voxel_grid", leaf=leaf, n=len(pts))
+    got = lcr.voxel_grid(pts, leaf, nrm=nrm, curv=curv, rgb=rgb)
+    ref = oracle.voxel_grid(pts, leaf, nrm=nrm, curv=curv, rgb=rgb)
+    assert len(got["xyz"]) == len(ref["xyz"]) > 0
+    for k in ("xyz", "nrm", "curv", "rgb"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+def test_voxel_grid_edges(lcr):
+    print_case(test="voxel_grid_edges")
+    assert len(lcr.voxel_grid(np.full((4, 3), np.nan, np.float32), 0.1)["xyz"]) == 0
+    p = np.array([[0, 0, 0], [1, 1, 1], [0.5, 0.2, 0.1]], np.float32)
+    np.testing.assert_array_equal(lcr.voxel_grid(p, 1e-4)["xyz"], p)  # lattice overflow: unchanged
+    q = np.random.default_rng(0).uniform(0, 1, size=(5000, 3)).astype(np.float32)
+    np.testing.assert_array_equal(lcr.voxel_grid(q, 0.05)["xyz"], oracle.voxel_grid(q, 0.05)["xyz"])
+
+
+@pytest.mark.parametrize("k,mul", [(50, 1.0), (20, 3.0), (8, 0.5)])
+def test_outlier_removal_matches_oracle(lcr, depth_cloud, k, mul):
+    rng = np.random.default_rng(k)
+    pts = synth.voxel_downsample(depth_cloud.astype(np.float64), 0.003).astype(np.float32)
+    outl = (pts[rng.integers(0, len(pts), 40)] + rng.normal(scale=0.05, size=(40, 3))).astype(np.float32)
+    pts = np.concatenate([pts, outl])
+    pts[11] = np.nan
+    print_case(test="outlier_removal", k=k, mul=mul, n=len(pts))
+    keep, dist, stats = lcr.outlier_removal(pts, k, mul)
+    rkeep, rdist, rstats = oracle.sor(pts, k, mul)
+    np.testing.assert_allclose(dist, rdist, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(stats, rstats, rtol=1e-9)
+    near = np.abs(rdist.astype(np.float64) - rstats[2]) <= 1e-6 * rstats[2]
+    np.testing.assert_array_equal(keep[~near], rkeep[~near])
+    assert keep[11] and dist[11] == 0
+
+
+@pytest.mark.parametrize("crop", [None, (40, 400, 60, 600)])
+def test_depth_to_cloud_matches_oracle(lcr, crop):
+    rng = np.random.default_rng(3)
+    shape = synth.make_shape(rng, scale=1.5)
+    d = synth.render_depth(shape, synth.turntable_pose(0.7, (0.0, 0.0, -0.9)))
+    depth = np.round(d * 1000.0).astype(np.uint16)
+    color = rng.integers(0, 256, size=depth.shape + (3,)).astype(np.uint8)
+    intr = (572.882768, 542.739980, 314.649173, 240.160459)
+    print_case(test="depth_to_cloud", crop=crop)
+    xyz, nrm, rgb = lcr.depth_to_cloud(depth, intr, 700, 1100, crop=crop, color=color)
+    rxyz, rnrm, rrgb = oracle.depth_to_cloud(depth, intr, 700, 1100, crop=crop, color=color)
+    assert len(xyz) == len(rxyz) > 0
+    np.testing.assert_array_equal(xyz, rxyz)
+    np.testing.assert_array_equal(nrm, rnrm)
+    np.testing.assert_array_equal(rgb, rrgb)
