@@ -42,6 +42,7 @@ extern "C" {
 #define LCR_PREC_FP32 0 /* exact FP32 scan (FLANN distance form) */
 #define LCR_PREC_FP16 1 /* FP16 matrix-core distance tiles + exact FP32 rerank (same matches) */
 #define LCR_PREC_BF16 2 /* BF16 matrix-core distance tiles + exact FP32 rerank (same matches) */
+#define LCR_PREC_FP32_TILES 3 /* wavefront-staged tiles scanned with exact FP32 distances */
 
 /* ICP parameters.  Mirrors the setters called at fine_registration.cpp:112-118:
  * setMaxCorrespondenceDistance, setMaximumIterations, setTransformationEpsilon,
@@ -56,7 +57,7 @@ typedef struct {
   int32_t precision_mode; /* LCR_PREC_* */
   double cell_factor;     /* grid cell edge / estimated point spacing; <= 0 = default */
   int32_t tile;           /* source points per work tile: 64, 128 or 256; 0 = default
-                             (LCR_PREC_FP16/BF16 always use 64) */
+                             (LCR_PREC_FP16/BF16/FP32_TILES always use 64) */
 } lcr_icp_params;
 
 typedef struct lcr_icp lcr_icp;

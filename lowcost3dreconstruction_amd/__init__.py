@@ -31,6 +31,7 @@ CONV_NO_CORRESPONDENCES = 5
 PREC_FP32 = 0
 PREC_FP16 = 1  # matrix-core FP16 distance tiles + exact FP32 rerank
 PREC_BF16 = 2  # matrix-core BF16 distance tiles + exact FP32 rerank
+PREC_FP32_TILES = 3  # wavefront-staged tiles scanned with exact FP32 distances
 
 # symbols declared in include/lcr.h
 EXPORTS = (

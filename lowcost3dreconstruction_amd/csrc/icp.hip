@@ -124,7 +124,8 @@ __device__ __forceinline__ void accumulate(double* v, float qx, float qy, float 
 }
 
 // PREC 0: exact per-lane grid search; 1 / 2: FP16 / BF16 matrix-core distance
-// tiles with exact FP32 rerank (tile_search.hpp; TILE is then 64, one wavefront).
+// tiles with exact FP32 rerank; 3: staged tiles scanned with exact FP32 distances
+// (tile_search.hpp; for 1-3 TILE is 64, one wavefront).
 template <bool P2PL, int TILE, int PREC>
 __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const IcpConst c) {
   constexpr int NV = P2PL ? kP2LVals : kP2PVals;
@@ -140,9 +141,6 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
     const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
-    double v[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = 0.0;
     const bool own = local < gs.npts;
     const float r = (float)sqrt(pd.max_corr_d2);
     const bool bounded = isfinite(r);
@@ -158,9 +156,14 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     if constexpr (PREC != 0) {
       __shared__ TileLds tl;
       const bool q_ok = search && isfinite(qx) && isfinite(qy) && isfinite(qz);
-      bool cert = false;
-      if (bounded && tile_nn<PREC>(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r,
-                                   0.75f * gt.h, tl, nn.d2, nn.id, nn.j, cert)) {
+      bool cert = false, tiled;
+      if constexpr (PREC == 3)
+        tiled = bounded && tile_nn_fp32(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r,
+                                        0.75f * gt.h, tl, nn.d2, nn.id, nn.j, cert);
+      else
+        tiled = bounded && tile_nn<PREC>(gt, b.cell_start + gt.cell_off, b.pts, q_ok, qx, qy, qz, r,
+                                         0.75f * gt.h, tl, nn.d2, nn.id, nn.j, cert);
+      if (tiled) {
         if (q_ok && cert) search = false;
         if (nn.j >= 0) seed = nn.j;
       }
@@ -171,6 +174,9 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     const float rlim = r * 1.00001f;
     bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
                 nn.id, nn.j);
+    double v[NV];  // FP64 sums of this tile (zeroed after the search: fewer live registers)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = 0.0;
     if (own) {
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
@@ -469,6 +475,7 @@ void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, h
   const bool pl = c.point_to_plane != 0;
   if (c.precision == 1) return launch_iter_tiles<1>(b, c, nblocks, s);
   if (c.precision == 2) return launch_iter_tiles<2>(b, c, nblocks, s);
+  if (c.precision == 3) return launch_iter_tiles<3>(b, c, nblocks, s);
   switch (b.tile) {
     case 64:
       pl ? launch_iter_t<true, 64>(b, c, nblocks, s) : launch_iter_t<false, 64>(b, c, nblocks, s);
@@ -539,13 +546,16 @@ void launch_transform_points(const float4* raw, const float4* raw_nrm, int64_t n
 int icp_max_blocks_per_cu(int tile, bool p2pl, int precision) {
   int nb = 0;
   hipError_t e = hipSuccess;
-  if (precision == 1 || precision == 2) {
+  if (precision >= 1 && precision <= 3) {
     if (precision == 1)
       e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 1>, 64, 0)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64, 1>, 64, 0);
-    else
+    else if (precision == 2)
       e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 2>, 64, 0)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64, 2>, 64, 0);
+    else
+      e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 3>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 64, 3>, 64, 0);
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
   }

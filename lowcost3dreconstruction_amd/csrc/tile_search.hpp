@@ -38,7 +38,7 @@ namespace lcr {
 
 #pragma clang fp contract(off)
 
-constexpr int kTileCap = 1024;
+constexpr int kTileCap = 512;
 
 // Staging area of one wavefront.
 struct TileLds {
@@ -123,38 +123,39 @@ __device__ __forceinline__ bool axis_cells(float lo, float hi, float o, float in
   return c0 <= c1;
 }
 
-// Must be called by all 64 lanes.  `own` lanes hold a finite query (qx, qy, qz).
-// The staged block covers the queries' box grown by rho = min(r, margin).  On
-// success the lane's (best, bid, bj) is the FLANN-form nearest staged point, and
-// `cert` says whether that is final: the nearest point overall (its distance is
-// below the query's distance to the grown box), or, when that distance exceeds r,
-// no point outside the block can be within r.  An uncertified lane's result is a
-// valid candidate (an upper bound) for the caller's exact search.  Returns false,
-// leaving the outputs untouched, when the tile cannot be staged.
-template <int PREC>
-__device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
-                        const float4* __restrict__ P, bool own, float qx, float qy, float qz,
-                        float r, float margin, TileLds& L, float& best, int& bid, int& bj,
-                        bool& cert) {
+// The staged block of one wavefront's tile.
+struct TileBox {
+  float lox, hix, loy, hiy, loz, hiz;  // the queries' box
+  float rr;                            // growth rho = min(r, margin)
+  int x0, x1, y0, y1, z0, z1;          // staged cells
+  int M;                               // staged points
+};
+
+// Stages the cells overlapping the own queries' box grown by rho = min(r, margin).
+// Must be called by all 64 lanes.  Returns 0 when the block cannot be staged
+// (more than kTileCap points or rows), 1 when staged (B.M may be 0), 2 when no
+// grid cell lies within rho of any query.
+__device__ __forceinline__ int tile_stage(const GridDesc& g, const int* __restrict__ cs,
+                                          const float4* __restrict__ P, bool own, float qx,
+                                          float qy, float qz, float r, float margin, TileLds& L,
+                                          TileBox& B) {
   const int lane = threadIdx.x & 63;
-  cert = false;
-  const uint64_t owners = __ballot(own);
-  if (owners == 0 || g.npts == 0) return true;
-  const float lox = wave_minf(own ? qx : INFINITY), hix = wave_maxf(own ? qx : -INFINITY);
-  const float loy = wave_minf(own ? qy : INFINITY), hiy = wave_maxf(own ? qy : -INFINITY);
-  const float loz = wave_minf(own ? qz : INFINITY), hiz = wave_maxf(own ? qz : -INFINITY);
-  const float rr = fminf(r * 1.00001f, margin);
-  int x0, x1, y0, y1, z0, z1;
-  if (!axis_cells(lox - rr, hix + rr, g.ox, g.inv_h, g.nx, x0, x1) ||
-      !axis_cells(loy - rr, hiy + rr, g.oy, g.inv_h, g.ny, y0, y1) ||
-      !axis_cells(loz - rr, hiz + rr, g.oz, g.inv_h, g.nz, z0, z1)) {
-    // no grid cell within rho of any query
-    cert = rr >= r * 1.00001f;
-    return true;
-  }
-  // ---- stage the block's points
+  B.lox = wave_minf(own ? qx : INFINITY);
+  B.hix = wave_maxf(own ? qx : -INFINITY);
+  B.loy = wave_minf(own ? qy : INFINITY);
+  B.hiy = wave_maxf(own ? qy : -INFINITY);
+  B.loz = wave_minf(own ? qz : INFINITY);
+  B.hiz = wave_maxf(own ? qz : -INFINITY);
+  B.rr = fminf(r * 1.00001f, margin);
+  B.M = 0;
+  const float rr = B.rr;
+  if (!axis_cells(B.lox - rr, B.hix + rr, g.ox, g.inv_h, g.nx, B.x0, B.x1) ||
+      !axis_cells(B.loy - rr, B.hiy + rr, g.oy, g.inv_h, g.ny, B.y0, B.y1) ||
+      !axis_cells(B.loz - rr, B.hiz + rr, g.oz, g.inv_h, g.nz, B.z0, B.z1))
+    return 2;
+  const int x0 = B.x0, x1 = B.x1, y0 = B.y0, y1 = B.y1, z0 = B.z0, z1 = B.z1;
   const int nyb = y1 - y0 + 1, nrows = (z1 - z0 + 1) * nyb;
-  if (nrows > kTileCap) return false;
+  if (nrows > kTileCap) return 0;
   tile_lds_sync();
   int M = 0;
   for (int rb = 0; rb < nrows; rb += 64) {
@@ -173,7 +174,7 @@ __device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
       if (lane >= off) inc += t;
     }
     const int total = __shfl(inc, 63, 64);
-    if (M + total > kTileCap) return false;
+    if (M + total > kTileCap) return 0;
     const int excl = inc - len;
     const int nr = min(64, nrows - rb);
     for (int t0 = 0; t0 < total; t0 += 256) {
@@ -201,10 +202,85 @@ __device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
     }
     M += total;
   }
-  if (M == 0) {
-    cert = rr >= r * 1.00001f;
+  B.M = M;
+  tile_lds_sync();
+  return 1;
+}
+
+// Certification of a lane's staged answer: every point within distance D of the
+// query lies in the grown box, whose cells were all staged.
+__device__ __forceinline__ bool tile_certified(const TileBox& B, float qx, float qy, float qz,
+                                               float r, float best) {
+  const float rr = B.rr;
+  const float D = fminf(fminf(fminf(qx - (B.lox - rr), (B.hix + rr) - qx),
+                              fminf(qy - (B.loy - rr), (B.hiy + rr) - qy)),
+                        fminf(qz - (B.loz - rr), (B.hiz + rr) - qz));
+  return (D >= r * 1.00001f) || (D > 0.0f && best < D * D * 0.99998f);
+}
+
+// Exact FP32 variant: each lane scans every staged point with the FLANN-form
+// distance (LDS broadcast reads).  Same contract as tile_nn.
+__device__ __forceinline__ bool tile_nn_fp32(const GridDesc& g, const int* __restrict__ cs,
+                                             const float4* __restrict__ P, bool own, float qx,
+                                             float qy, float qz, float r, float margin, TileLds& L,
+                                             float& best, int& bid, int& bj, bool& cert) {
+  cert = false;
+  if (__ballot(own) == 0 || g.npts == 0) return true;
+  TileBox B;
+  const int st = tile_stage(g, cs, P, own, qx, qy, qz, r, margin, L, B);
+  if (st == 0) return false;
+  if (st == 2 || B.M == 0) {
+    cert = B.rr >= r * 1.00001f;
     return true;
   }
+  if (own) {
+    float bd = INFINITY;
+    int bi = 0x7fffffff, bt = -1;
+    int t = 0;
+    for (; t + 4 <= B.M; t += 4) {
+      const float4 p0 = L.p[t], p1 = L.p[t + 1], p2 = L.p[t + 2], p3 = L.p[t + 3];
+      consider(p0, t, qx, qy, qz, bd, bi, bt);
+      consider(p1, t + 1, qx, qy, qz, bd, bi, bt);
+      consider(p2, t + 2, qx, qy, qz, bd, bi, bt);
+      consider(p3, t + 3, qx, qy, qz, bd, bi, bt);
+    }
+    for (; t < B.M; ++t) consider(L.p[t], t, qx, qy, qz, bd, bi, bt);
+    if (bt >= 0 && (bd < best || (bd == best && bi < bid))) {
+      best = bd;
+      bid = bi;
+      bj = L.j[bt];
+    }
+    cert = tile_certified(B, qx, qy, qz, r, best);
+  }
+  return true;
+}
+
+// Must be called by all 64 lanes.  `own` lanes hold a finite query (qx, qy, qz).
+// The staged block covers the queries' box grown by rho = min(r, margin).  On
+// success the lane's (best, bid, bj) is the FLANN-form nearest staged point, and
+// `cert` says whether that is final: the nearest point overall (its distance is
+// below the query's distance to the grown box), or, when that distance exceeds r,
+// no point outside the block can be within r.  An uncertified lane's result is a
+// valid candidate (an upper bound) for the caller's exact search.  Returns false,
+// leaving the outputs untouched, when the tile cannot be staged.
+template <int PREC>
+__device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
+                        const float4* __restrict__ P, bool own, float qx, float qy, float qz,
+                        float r, float margin, TileLds& L, float& best, int& bid, int& bj,
+                        bool& cert) {
+  const int lane = threadIdx.x & 63;
+  cert = false;
+  if (__ballot(own) == 0 || g.npts == 0) return true;
+  TileBox B;
+  const int staged = tile_stage(g, cs, P, own, qx, qy, qz, r, margin, L, B);
+  if (staged == 0) return false;
+  if (staged == 2 || B.M == 0) {
+    cert = B.rr >= r * 1.00001f;
+    return true;
+  }
+  const int M = B.M;
+  const float lox = B.lox, hix = B.hix, loy = B.loy, hiy = B.hiy, loz = B.loz, hiz = B.hiz;
+  const int x0 = B.x0, x1 = B.x1, y0 = B.y0, y1 = B.y1, z0 = B.z0, z1 = B.z1;
   // ---- centre, rounding bounds
   const float cx = 0.5f * (lox + hix), cy = 0.5f * (loy + hiy), cz = 0.5f * (loz + hiz);
   // every staged point and query lies in the cell box [x0, x1+1] x ... (h units)
@@ -307,12 +383,7 @@ __device__ bool tile_nn(const GridDesc& g, const int* __restrict__ cs,
       bid = id;
       bj = L.rj[lane];
     }
-    // every point within distance D of the query lies in the grown box, whose
-    // cells were all staged
-    const float D = fminf(fminf(fminf(qx - (lox - rr), (hix + rr) - qx),
-                                fminf(qy - (loy - rr), (hiy + rr) - qy)),
-                          fminf(qz - (loz - rr), (hiz + rr) - qz));
-    cert = (D >= r * 1.00001f) || (D > 0.0f && best < D * D * 0.99998f);
+    cert = tile_certified(B, qx, qy, qz, r, best);
   }
   return true;
 }

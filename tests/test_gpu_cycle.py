@@ -139,8 +139,8 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
     c = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=cyc.meta["prior"],
                   point_to_plane=True, **KW)
     t = c.autotune(str(cache))
-    assert t["from_cache"] == 0 and t["candidates"] == 25
-    assert t["tile"] in (64, 128, 256) and t["precision_mode"] in (0, 1, 2)
+    assert t["from_cache"] == 0 and t["candidates"] == 30
+    assert t["tile"] in (64, 128, 256) and t["precision_mode"] in (0, 1, 2, 3)
     assert 0 < t["best_ms"] <= t["default_ms"] * 1.5
     c.run()
     r1 = c.results()

@@ -190,7 +190,7 @@ def test_tiles_and_cell_factors_agree(lcr, c1):
         assert_close_T(T, Ts[0], rot=1e-9, tr=1e-10)
 
 
-@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("prec", [1, 2, 3])
 @pytest.mark.parametrize("p2pl", [False, True])
 def test_distance_tiles_match_exact_search(lcr, c1, prec, p2pl):
     """FP16/BF16 matrix-core distance tiles with the exact FP32 rerank pick the same
@@ -209,7 +209,7 @@ def test_distance_tiles_match_exact_search(lcr, c1, prec, p2pl):
     assert out[0][1:] == out[1][1:]
 
 
-@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("prec", [1, 2, 3])
 def test_distance_tiles_fallback_and_small_radius(lcr, prec):
     """A dense cloud with a radius spanning many cells overflows the staged block
     (scalar fallback), a tiny radius stages almost nothing; both stay identical to
@@ -233,7 +233,7 @@ def test_distance_tiles_cycle_matches_exact(lcr):
                                 centre=(0.0, 0.0, -0.8))
     print_case(test="distance_tiles_cycle")
     res = []
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         c = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=cyc.meta["prior"], point_to_plane=True,
                       max_corr_dist=0.01, max_iterations=60, tile=64, precision_mode=mode)
         c.run()
