@@ -251,9 +251,9 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
     PairState* st = b.state + p;
     if (st->gated) continue;  // uniform across the block
     const PairDesc pd = b.pairs[p];
-    // the gate's score (per_instance) counts every kGateStride-th tile of the
-    // source: the same source points for every hypothesis of a pair
-    const int stride = per_instance ? kGateStride : 1;
+    // the gate's score (per_instance) counts a fixed subsample of the source's
+    // tiles: the same source points for every hypothesis of a pair
+    const int stride = per_instance ? max(1, min(kGateStride, pd.ntiles / kGateMinTiles)) : 1;
     if ((tile - pd.tile_begin) % stride != 0) continue;  // uniform across the block
     const int nparts = (pd.ntiles + stride - 1) / stride;  // tiles taking part
     const GridDesc gs = b.grids[pd.src];

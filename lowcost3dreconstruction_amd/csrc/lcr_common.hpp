@@ -80,8 +80,11 @@ struct IcpConst {
   int interleave;                // tile order: 0 XCD-contiguous, 1 grid stride
 };
 
-// The hypothesis gate scores each pose on every kGateStride-th work tile.
-constexpr int kGateStride = 8;
+// The hypothesis gate scores each pose on every s-th work tile of the source,
+// s = clamp(ntiles / kGateMinTiles, 1, kGateStride): at most 1/32 of the points,
+// but at least kGateMinTiles tiles where the source has them.
+constexpr int kGateStride = 32;
+constexpr int kGateMinTiles = 64;
 
 // Flat FP32 weight layout of the pose-proposal network, row-major [in][out]:
 // W1 3x64, b1, W2 64x128, b2, W3 128x256, b3, Hw1 512x256, hb1, Hw2 256x6, hb2.

@@ -10,17 +10,23 @@
 // One wavefront per query, 64 consecutive cell-sorted queries per wavefront:
 //   1. stage every point of the 3x3x3 cells around the query's cell in LDS (one
 //      lane per row of cells finds the runs, then a flat coalesced copy); the
-//      next query in the same cell reuses the staged block;
-//   2. find the exact K-th smallest FLANN-form d2 by bisection over the float bit
-//      pattern (count via wave ballots), resolving equal distances by original
-//      index;
+//      next query in the same cell reuses the staged block.  A block larger than
+//      LDS is staged as an evenly strided subsample, whose K-th smallest d2 bounds
+//      the block's from above, and then restaged keeping only the points under
+//      that bound;
+//   2. find the exact K-th smallest FLANN-form d2: the K-th smallest of the 64
+//      per-lane minima bounds it; the points under that bound are compacted to
+//      one per lane and a bisection over the float bit pattern (counts by wave
+//      ballot) finds it there; equal distances are resolved by original index;
 //   3. certify: the K-th distance must be below the distance to the staged
-//      block's boundary, else stage the next larger block and repeat;
+//      block's boundary, else stage the next larger block, keeping only the points
+//      within the K-th distance found so far, and repeat;
 //   4. accumulate FP64 moments of the selected points relative to the query.
 // A second pass solves one FP64 3x3 eigenproblem per lane from the stored moment
-// sums (keeping the solver's registers out of the search).  Queries whose candidate
-// block overflows LDS are listed and finished by a rescan pass that reads the block
-// from memory for every count.
+// sums (keeping the solver's registers out of the search).  Queries the staged
+// search gives up on (very many equal distances, K above half the LDS capacity)
+// are listed and finished by a rescan pass that reads the block from memory for
+// every count.
 #include <hip/hip_runtime.h>
 
 #include "lcr_common.hpp"
@@ -35,8 +41,8 @@ namespace lcr {
 namespace {
 
 constexpr int kWaves = 4;
-constexpr int kCap = 512;
-constexpr int kSlots = kCap / 64;
+constexpr int kCap = 496;  // staged points per wave: 4 x (496 x 16 + 64 x 4) B = 32 KiB per block
+constexpr int kSlots = (kCap + 63) / 64;
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -191,40 +197,70 @@ __device__ void select_rescan(const GridDesc& g, const int* cs, const float4* pt
   });
 }
 
-// The wave's staged candidate block: every point of the (2K+1)^3 cells around
-// `cell` (a global cell index), copied to LDS.  Consecutive cell-sorted queries
-// mostly share a cell, so most queries reuse the block without touching memory.
+// The wave's staged candidate block in LDS.  When the (2K+1)^3 cells around the
+// query's cell hold at most kCap points it holds all of them, and the following
+// cell-sorted queries in the same cell reuse it.  A larger block is held filtered
+// -- the points whose FLANN d2 to the query is <= filt (the first kCap of them
+// when more qualify) -- or, without a bound to filter by, as an evenly strided
+// subsample; both serve that query only.
 struct Staged {
-  int64_t cell;
-  int K, cnt;
-  bool ovf;
+  int64_t cell;   // cell whose complete unfiltered block is staged, else -1
+  int K;          // block radius in cells
+  int cnt;        // points in LDS
+  int total;      // points that qualified (more than kCap: LDS holds a subset)
+  unsigned filt;  // d2 filter as a bit pattern; 0xffffffff: none
 };
 
-// Stages the points of block K around cp into sp.  Each lane looks up one
-// x-contiguous row of cells (all cell_start loads in flight at once), a wave
-// prefix sum places the rows, then the points are copied four per lane at a time.
-__device__ __forceinline__ void stage_block(const GridDesc& g, const int* cs, const float4* __restrict__ pts,
-                            const CellPos& cp, int K, float4* sp, Staged& st) {
+constexpr unsigned kNoFilter = 0xffffffffu;
+constexpr unsigned kInfBits = 0x7f800000u;
+
+// Sorted-index start and length of row r (z-major, then y) of block K.
+__device__ __forceinline__ void block_row(const GridDesc& g, const int* cs, int x0, int x1, int y0,
+                                          int z0, int nyb, int r, int& s, int& len) {
+  const int z = z0 + r / nyb, y = y0 + r % nyb;
+  const int row = (z * g.ny + y) * g.nx;
+  s = cs[row + x0];
+  len = cs[row + x1 + 1] - s;
+}
+
+// Number of set bits of b in the lanes below this one.
+__device__ __forceinline__ int lane_prefix(uint64_t b) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+}
+
+// Stages block K around cp into sp (see Staged): the whole block when it fits
+// LDS, else the points with d2 <= U when a bound U is given, else an evenly
+// strided subsample.  Each lane looks up one x-contiguous row of cells, a wave
+// prefix sum places the rows, then the points are loaded four per lane at a time;
+// the filtered form keeps the qualifying ones in block order through ballots.
+__device__ __forceinline__ void stage_block(const GridDesc& g, const int* cs,
+                                            const float4* __restrict__ pts, const CellPos& cp, int K,
+                                            const float4& q, unsigned U, float4* sp, Staged& st) {
   const int lane = threadIdx.x & 63;
   wave_lds_sync();  // the previous block's reads come before these writes
   st.K = K;
   st.cnt = 0;
-  st.ovf = false;
+  st.total = 0;
+  st.filt = kNoFilter;
   const int z0 = max(cp.cz - K, 0), z1 = min(cp.cz + K, g.nz - 1);
   const int y0 = max(cp.cy - K, 0), y1 = min(cp.cy + K, g.ny - 1);
   const int x0 = max(cp.cx - K, 0), x1 = min(cp.cx + K, g.nx - 1);
   if (z0 > z1 || y0 > y1 || x0 > x1) return;
   const int nyb = y1 - y0 + 1, nrows = (z1 - z0 + 1) * nyb;
-  int cnt = 0;
+  int tot = 0;
   for (int rb = 0; rb < nrows; rb += 64) {
-    const int r = rb + lane;
     int s = 0, len = 0;
-    if (r < nrows) {
-      const int z = z0 + r / nyb, y = y0 + r % nyb;
-      const int row = (z * g.ny + y) * g.nx;
-      s = cs[row + x0];
-      len = cs[row + x1 + 1] - s;
-    }
+    if (rb + lane < nrows) block_row(g, cs, x0, x1, y0, z0, nyb, rb + lane, s, len);
+    tot += wave_sum_i(len);
+  }
+  const bool filter = tot > kCap && U != kNoFilter;
+  // otherwise block positions i * stride are kept
+  const int stride = tot > kCap ? (tot + kCap - 1) / kCap : 1;
+  int pos = 0;  // filter: points kept so far; else block position of the group's first point
+  for (int rb = 0; rb < nrows; rb += 64) {
+    int s = 0, len = 0;
+    if (rb + lane < nrows) block_row(g, cs, x0, x1, y0, z0, nyb, rb + lane, s, len);
     int inc = len;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -232,17 +268,20 @@ __device__ __forceinline__ void stage_block(const GridDesc& g, const int* cs, co
       if (lane >= off) inc += t;
     }
     const int total = __shfl(inc, 63, 64);
-    if (cnt + total > kCap) {
-      st.ovf = true;
-      return;
-    }
     const int excl = inc - len;
     const int nr = min(64, nrows - rb);
-    for (int t0 = 0; t0 < total; t0 += 256) {
+    // filter: group positions; else kept indices i with block position i * stride
+    // inside this group
+    const int i0 = filter ? 0 : (pos + stride - 1) / stride;
+    const int i1 = filter ? total : (pos + total + stride - 1) / stride;
+    for (int ib = i0; ib < i1; ib += 256) {
       // positions past the end are clamped to the last one (loaded, not stored)
       int tt[4], jb[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int u = 0; u < 4; ++u) tt[u] = min(t0 + lane + 64 * u, total - 1);
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(ib + lane + 64 * u, i1 - 1);
+        tt[u] = filter ? i : i * stride - pos;
+      }
       // row of position t: the last row whose first position is <= t
       for (int rr = 0; rr < nr; ++rr) {
         const int ex = __builtin_amdgcn_readlane(excl, rr);
@@ -255,86 +294,137 @@ __device__ __forceinline__ void stage_block(const GridDesc& g, const int* cs, co
 #pragma unroll
       for (int u = 0; u < 4; ++u) v[u] = pts[jb[u] + tt[u]];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (t0 + lane + 64 * u < total) sp[cnt + t0 + lane + 64 * u] = v[u];
+      for (int u = 0; u < 4; ++u) {
+        const int i = ib + lane + 64 * u;
+        if (filter) {
+          const bool keep = i < i1 && __float_as_uint(flann_d2(q.x, q.y, q.z, v[u])) <= U;
+          const uint64_t b = __ballot(keep);
+          const int at = pos + lane_prefix(b);
+          if (keep && at < kCap) sp[at] = v[u];
+          pos += __popcll(b);
+        } else if (i < i1) {
+          sp[i] = v[u];
+        }
+      }
     }
-    cnt += total;
+    if (!filter) pos += total;
   }
-  st.cnt = cnt;
+  if (filter) {
+    st.filt = U;
+    st.total = pos;
+    st.cnt = min(pos, kCap);
+  } else {
+    st.total = tot;
+    st.cnt = (tot + stride - 1) / stride;
+  }
   wave_lds_sync();
 }
 
-// Returns false (acc untouched) when the candidates overflow LDS; the query is
-// then left to the rescan pass.
-template <int MODE>
-__device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, const float4* pts,
-                                              const float4& q, int kk, float4* sp, Staged& st,
-                                              Moments& acc) {
+// A complete unfiltered block can serve the following queries of its cell.
+__device__ __forceinline__ bool reusable(const Staged& st) {
+  return st.filt == kNoFilter && st.total <= kCap;
+}
+
+// FLANN d2 bit patterns of the staged points, point lane + 64 m in slot m
+// (0xffffffff past the end).
+__device__ __forceinline__ void fill_slots(const float4* sp, int cnt, const float4& q,
+                                           unsigned (&slot)[kSlots]) {
   const int lane = threadIdx.x & 63;
-  const CellPos cp = cell_pos(g, q.x, q.y, q.z);
-  const int64_t cell = (int64_t)g.cell_off + ((int64_t)cp.cz * g.ny + cp.cy) * g.nx + cp.cx;
-  if (st.cell != cell) {
-    st.cell = cell;
-    stage_block(g, cs, pts, cp, 1, sp, st);
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    const int i = lane + 64 * m;
+    slot[m] = i < cnt ? __float_as_uint(flann_d2(q.x, q.y, q.z, sp[i])) : 0xffffffffu;
   }
-  int K = st.K;
-  unsigned slot[kSlots];
-  unsigned T = 0;
-  for (;;) {
-    if (st.ovf) return false;
-    const float Lc = block_gap(g, cp, K);
-    const bool covers = (Lc == INFINITY);
-    if (st.cnt >= kk) {
+}
+
+// Smallest t <= hi0 with at least kk values <= t: the kk-th smallest when at least
+// kk of the values are <= hi0.  Bisection over the bit pattern, counts by ballot.
+template <int NS>
+__device__ __forceinline__ unsigned kth_slots(const unsigned (&d)[NS], int ns, int kk, unsigned hi0) {
+  unsigned lo = 0u, hi = hi0;
+  while (lo < hi) {
+    const unsigned mid = lo + ((hi - lo) >> 1);
+    int c = 0;
 #pragma unroll
-      for (int m = 0; m < kSlots; ++m) {
-        const int i = lane + 64 * m;
-        slot[m] = i < st.cnt ? __float_as_uint(flann_d2(q.x, q.y, q.z, sp[i])) : 0xffffffffu;
-      }
-      const int nslots = (st.cnt + 63) >> 6;
-      unsigned lo = 0u, hi = 0x7f800000u;
-      while (lo < hi) {
-        const unsigned mid = lo + ((hi - lo) >> 1);
-        int c = 0;
+    for (int m = 0; m < NS; ++m)
+      if (m < ns) c += popc_ballot(d[m] <= mid);
+    if (c >= kk)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// The kk-th smallest d2 of the cnt staged points (cnt >= kk).  With more than 64
+// points and kk <= 64, the kk-th smallest of the 64 per-lane minima (distinct
+// staged points) bounds it from above; when at most 64 points lie under that
+// bound they are compacted to one per lane (cd, ci: d2 bits and staged index,
+// through cb) and the bisection runs over that single slot.  comp tells which.
+__device__ __forceinline__ unsigned kth_staged(const unsigned (&slot)[kSlots], int cnt, int kk,
+                                               const float4* sp, const float4& q, int* cb,
+                                               unsigned& cd, int& ci, bool& comp) {
+  const int lane = threadIdx.x & 63;
+  const int nslots = (cnt + 63) >> 6;
+  comp = false;
+  if (nslots > 1 && kk <= 64) {
+    unsigned mn = slot[0];
 #pragma unroll
-        for (int m = 0; m < kSlots; ++m)
-          if (m < nslots) c += popc_ballot(slot[m] <= mid);
-        if (c >= kk)
-          hi = mid;
-        else
-          lo = mid + 1;
-      }
-      T = lo;
-      const float L = (Lc - cp.slack) * g.h;
-      if (covers || (L > 0.0f && __uint_as_float(T) < L * L * 0.99998f)) break;
+    for (int m = 1; m < kSlots; ++m) mn = min(mn, slot[m]);
+    const unsigned mins[1] = {mn};
+    const unsigned U = kth_slots<1>(mins, 1, kk, kInfBits);
+    int c = 0;
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m)
+      if (m < nslots) c += popc_ballot(slot[m] <= U);
+    if (c <= 64) {
+      wave_lds_sync();
+      int base = 0;
+#pragma unroll
+      for (int m = 0; m < kSlots; ++m)
+        if (m < nslots) {
+          const bool keep = slot[m] <= U;
+          const uint64_t b = __ballot(keep);
+          if (keep) cb[base + lane_prefix(b)] = lane + 64 * m;
+          base += __popcll(b);
+        }
+      wave_lds_sync();
+      ci = lane < c ? cb[lane] : 0;
+      cd = lane < c ? __float_as_uint(flann_d2(q.x, q.y, q.z, sp[ci])) : 0xffffffffu;
+      comp = true;
+      const unsigned cds[1] = {cd};
+      return kth_slots<1>(cds, 1, kk, U);
     }
-    ++K;
-    stage_block(g, cs, pts, cp, K, sp, st);
   }
-  const int nslots = (st.cnt + 63) >> 6;
+  return kth_slots<kSlots>(slot, nslots, kk, kInfBits);
+}
+
+// Accumulates the kk points with the smallest (d2, original index) among
+// candidates (d bits, staged index ix) that hold every point with d2 <= T, T the
+// kk-th smallest d2: all points below T and the smallest indices at T.
+template <int MODE, int NS>
+__device__ __forceinline__ void take_kth(const unsigned (&d)[NS], const int (&ix)[NS], int ns, int kk,
+                                         unsigned T, const float4* sp, const float4& q, Moments& acc) {
   int clt = 0, cle = 0;
 #pragma unroll
-  for (int m = 0; m < kSlots; ++m)
-    if (m < nslots) {
-      clt += popc_ballot(slot[m] < T);
-      cle += popc_ballot(slot[m] <= T);
+  for (int m = 0; m < NS; ++m)
+    if (m < ns) {
+      clt += popc_ballot(d[m] < T);
+      cle += popc_ballot(d[m] <= T);
     }
   int idcut = 0x7fffffff;
   if (cle > kk) {
-    // equal distances at the threshold: keep the smallest original indices
     const int need = kk - clt;
-    int ids[kSlots];
+    int ids[NS];
 #pragma unroll
-    for (int m = 0; m < kSlots; ++m) {
-      const int i = lane + 64 * m;
-      ids[m] = (i < st.cnt && slot[m] == T) ? __float_as_int(sp[i].w) : 0x7fffffff;
-    }
+    for (int m = 0; m < NS; ++m) ids[m] = d[m] == T ? __float_as_int(sp[ix[m]].w) : 0x7fffffff;
     unsigned lo = 0u, hi = 0x7fffffffu;
     while (lo < hi) {
       const unsigned mid = lo + ((hi - lo) >> 1);
       int c = 0;
 #pragma unroll
-      for (int m = 0; m < kSlots; ++m)
-        if (m < nslots) c += popc_ballot(slot[m] == T && (unsigned)ids[m] <= mid);
+      for (int m = 0; m < NS; ++m)
+        if (m < ns) c += popc_ballot(d[m] == T && (unsigned)ids[m] <= mid);
       if (c >= need)
         hi = mid;
       else
@@ -343,12 +433,83 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
     idcut = (int)lo;
   }
 #pragma unroll
-  for (int m = 0; m < kSlots; ++m) {
-    const int i = lane + 64 * m;
-    if (i < st.cnt && slot[m] <= T) {
-      const float4 p = sp[i];
-      if (slot[m] < T || __float_as_int(p.w) <= idcut) take<MODE>(acc, q, p, __uint_as_float(slot[m]));
+  for (int m = 0; m < NS; ++m)
+    if (d[m] <= T) {
+      const float4 p = sp[ix[m]];
+      if (d[m] < T || __float_as_int(p.w) <= idcut) take<MODE>(acc, q, p, __uint_as_float(d[m]));
     }
+}
+
+// Tightens an overflowing block's bound at most this many times before leaving
+// the query to the rescan pass.
+constexpr int kMaxTighten = 8;
+
+// Exact K-nearest selection for query q from LDS-staged blocks.  Block K grows
+// until the kk-th smallest d2 is certified below the distance to the block's
+// boundary.  A block larger than LDS is staged filtered by a proven upper bound
+// of its kk-th smallest d2 (the previous block's kk-th smallest, or that of a
+// staged subset of the block), so it fits LDS unless very many points tie.
+// Returns false (acc untouched) when it gives up; the query is then left to the
+// rescan pass.
+template <int MODE>
+__device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, const float4* pts,
+                                              const float4& q, int kk, float4* sp, int* cb, Staged& st,
+                                              Moments& acc) {
+  const int lane = threadIdx.x & 63;
+  const CellPos cp = cell_pos(g, q.x, q.y, q.z);
+  const int64_t cell = (int64_t)g.cell_off + ((int64_t)cp.cz * g.ny + cp.cy) * g.nx + cp.cx;
+  if (st.cell != cell) {
+    stage_block(g, cs, pts, cp, 1, q, kNoFilter, sp, st);
+    st.cell = reusable(st) ? cell : -1;
+  }
+  int K = st.K;
+  unsigned slot[kSlots];
+  unsigned T = 0, cd = 0xffffffffu;
+  int ci = 0;
+  bool comp = false;
+  // Every pass grows K (until the block covers the grid) or tightens a bound, at
+  // most kMaxTighten times: the loop ends.
+  for (int tighten = 0;;) {
+    if (st.total > kCap) {
+      // LDS holds a subset of the qualifying points of block K: the kk-th smallest
+      // d2 among them bounds the block's kk-th smallest from above
+      if (st.cnt < kk || ++tighten > kMaxTighten) return false;
+      fill_slots(sp, st.cnt, q, slot);
+      const unsigned U = kth_slots<kSlots>(slot, (st.cnt + 63) >> 6, kk, kInfBits);
+      if (U >= st.filt) return false;  // no progress: very many equal distances
+      stage_block(g, cs, pts, cp, K, q, U, sp, st);
+      st.cell = -1;
+      continue;
+    }
+    const float Lc = block_gap(g, cp, K);
+    const bool covers = (Lc == INFINITY);
+    if (st.cnt >= kk) {
+      fill_slots(sp, st.cnt, q, slot);
+      T = kth_staged(slot, st.cnt, kk, sp, q, cb, cd, ci, comp);
+      const float L = (Lc - cp.slack) * g.h;
+      if (covers || (L > 0.0f && __uint_as_float(T) < L * L * 0.99998f)) break;
+      // the next block's kk-th smallest d2 is at most T
+      ++K;
+      stage_block(g, cs, pts, cp, K, q, T, sp, st);
+      st.cell = reusable(st) ? cell : -1;
+      continue;
+    }
+    // fewer than kk points: only an unfiltered block that does not cover the grid;
+    // an isolated point's block grows by half its radius at a time
+    if (st.filt != kNoFilter || covers) return false;
+    K += max(1, K >> 1);
+    stage_block(g, cs, pts, cp, K, q, kNoFilter, sp, st);
+    st.cell = reusable(st) ? cell : -1;
+  }
+  if (comp) {
+    const unsigned d1[1] = {cd};
+    const int i1[1] = {ci};
+    take_kth<MODE, 1>(d1, i1, 1, kk, T, sp, q, acc);
+  } else {
+    int ix[kSlots];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) ix[m] = lane + 64 * m;
+    take_kth<MODE, kSlots>(slot, ix, (st.cnt + 63) >> 6, kk, T, sp, q, acc);
   }
   return true;
 }
@@ -364,6 +525,7 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
                  int* __restrict__ deferred) {
   constexpr int NV = MODE == kMoments ? 9 : 1;
   __shared__ float4 s_p[kWaves][kCap];
+  __shared__ int s_cb[kWaves][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 64;
   if (base >= total) return;
@@ -376,7 +538,8 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
   st.cell = -1;
   st.K = 0;
   st.cnt = 0;
-  st.ovf = false;
+  st.total = 0;
+  st.filt = kNoFilter;
   int mine_n = 0;
   for (int qi = 0; qi < nq; ++qi) {
     float4 q;
@@ -390,7 +553,7 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
     if (kk >= (MODE == kMoments ? 3 : 1) &&
-        !select_staged<MODE>(g, cell_start + g.cell_off, pts, q, kk, s_p[w], st, acc)) {
+        !select_staged<MODE>(g, cell_start + g.cell_off, pts, q, kk, s_p[w], s_cb[w], st, acc)) {
       // deferred[0] counts the list deferred[1..]
       if (lane == 0) deferred[1 + atomicAdd(deferred, 1)] = (int)(base + qi);
       if (lane == qi) mine_n = kk;

@@ -99,3 +99,25 @@ def test_normals_dense_cells_take_rescan_path(lcr):
     rn, rc = oracle.normals(pts, 40)
     dots = np.abs(np.sum(n.astype(np.float64) * rn, axis=1))
     assert np.all(dots > 1 - DOT_TOL), np.sort(dots)[:5]
+
+
+def test_normals_dense_blob_in_sparse_cloud(lcr):
+    """A blob far denser than the rest of the cloud: the blocks around its cells
+    hold more points than LDS, so they are staged as a strided subsample, bounded
+    and restaged filtered; the plane's points next to the blob restage larger
+    blocks filtered by the smaller block's k-th distance."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(8)
+    plane = np.c_[rng.uniform(-0.5, 0.5, size=(20000, 2)), rng.normal(scale=2e-3, size=20000)]
+    blob = rng.normal(scale=4e-3, size=(4000, 3)) + [0.1, 0.1, 0.0]
+    pts = np.concatenate([plane, blob]).astype(np.float32)
+    k = 30
+    print_case(test="normals_blob", n=len(pts), k=k)
+    n, c = lcr.normals(pts, k)
+    rn, rc = oracle.normals(pts, k)
+    d, _ = cKDTree(pts.astype(np.float64)).query(pts.astype(np.float64), k + 1)
+    ok = (d[:, k] ** 2 - d[:, k - 1] ** 2) > 1e-10
+    assert ok.mean() > 0.99
+    dots = np.sum(n[ok].astype(np.float64) * rn[ok], axis=1)
+    assert np.all(dots > 1 - DOT_TOL), np.sort(dots)[:5]
+    np.testing.assert_allclose(c[ok], rc[ok], rtol=2e-6, atol=1e-9)
