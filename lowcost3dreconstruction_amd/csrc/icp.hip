@@ -1,11 +1,12 @@
 // icp.hip -- K2..K6 and K8: one ICP iteration for every active instance of a
-// batch in a single launch, the fitness pass, and the output transform.
+// batch in two launches (search, then fold and solve), the fitness pass, and the
+// output transform.
 //
 // Replaces IterativeClosestPoint::computeTransformation's loop body
 // (pcl/registration/impl/icp.hpp; driven by icp.align at fine_registration.cpp:121):
 //   determineCorrespondences   -> exact grid 1-NN per source point (lcr_search.hpp)
-//   estimateRigidTransformation-> FP64 moments per tile, folded by the instance's
-//                                 last block, Horn (point-to-point) or Cholesky
+//   estimateRigidTransformation-> FP64 moments per tile, folded per instance in a
+//                                 fixed order, Horn (point-to-point) or Cholesky
 //                                 (point-to-plane) in FP64 (lcr_math.hpp)
 //   final = T * final          -> FP64 composition on the device
 //   convergence criteria       -> DefaultConvergenceCriteria on the device
@@ -16,11 +17,11 @@
 // of blocks (a multiple of 8) loops over tiles.  Hardware places block b on XCD
 // b % 8, so each XCD is given one contiguous eighth of the tile range: the tiles
 // of an instance, and the target grid they search, then stay in that XCD's L2.
-// Each block adds its tile's FP64 sums into a per-tile partial, then takes a
-// ticket on the instance; the block that takes the last ticket folds the
-// partials in tile order (deterministic, no floating-point atomics) and updates
-// the instance.  Converged instances are skipped, so the iteration can be
-// replayed a fixed number of times inside a hipGraph.
+// Each block writes its tile's FP64 sums into a per-tile partial; a second
+// launch (one block per instance) folds the partials in a fixed order
+// (deterministic, no floating-point atomics), solves and updates the instance.
+// Converged instances are skipped, so the iteration can be replayed a fixed
+// number of times inside a hipGraph.
 #include <hip/hip_runtime.h>
 #include <float.h>
 
@@ -86,41 +87,65 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles, int interleave) {
   return TileRange{t0 + slot, t1, per};
 }
 
-template <bool P2PL>
-__device__ __forceinline__ void accumulate(double* v, float qx, float qy, float qz,
-                                           const float4& t, const float4& n, float d2) {
-  v[0] = 1.0;
+// The tile's FP64 sums, written to out[0..NV): the correspondence count, the
+// moments of the estimator and the sum of d2.  Point-to-point: q, t and q t^T.
+// Point-to-plane (transformation_estimation_point_to_plane_lls.hpp): the upper
+// triangle of A^T A and A^T b for rows [s x n ; n], d = n . (dst - src), in
+// double; a correspondence whose normal is not finite adds only to the count and
+// the d2 sum.  Each value is formed and summed over the wavefront in turn (fixed
+// order), so only the per-lane inputs stay live; the waves' sums are added in
+// wave order.
+template <bool P2PL, int TILE>
+__device__ __forceinline__ void tile_moments(bool has, float qx, float qy, float qz, const float4& t,
+                                             const float4& n, float d2, double* lds, double* out) {
+  constexpr int NV = P2PL ? kP2LVals : kP2PVals;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int k = 0;
+  auto emit = [&](double x) {
+    const double sum = wave_sum_dpp(x);
+    if (lane == 0) lds[wid * NV + k] = sum;
+    ++k;
+  };
+  emit(has ? 1.0 : 0.0);
   if constexpr (!P2PL) {
-    v[1] = qx;
-    v[2] = qy;
-    v[3] = qz;
-    v[4] = t.x;
-    v[5] = t.y;
-    v[6] = t.z;
-    const double q[3] = {qx, qy, qz}, tt[3] = {t.x, t.y, t.z};
+    const double q[3] = {has ? (double)qx : 0.0, has ? (double)qy : 0.0, has ? (double)qz : 0.0};
+    const double tt[3] = {has ? (double)t.x : 0.0, has ? (double)t.y : 0.0, has ? (double)t.z : 0.0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) emit(q[a]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) emit(tt[a]);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) v[7 + 3 * a + b] = q[a] * tt[b];
-    v[16] = d2;
+      for (int c = 0; c < 3; ++c) emit(q[a] * tt[c]);
   } else {
-    v[28] = d2;
-    if (isfinite(n.x) && isfinite(n.y) && isfinite(n.z)) {
-      // transformation_estimation_point_to_plane_lls.hpp: row [s x n ; n],
-      // d = n . (dst - src), accumulated in double
+    double row[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, d = 0.0;
+    if (has && isfinite(n.x) && isfinite(n.y) && isfinite(n.z)) {
       const double sx = qx, sy = qy, sz = qz, nx = n.x, ny = n.y, nz = n.z;
-      const double row[6] = {nz * sy - ny * sz, nx * sz - nz * sx, ny * sx - nx * sy, nx, ny, nz};
-      const double d = nx * (double)t.x + ny * (double)t.y + nz * (double)t.z - nx * sx -
-                       ny * sy - nz * sz;
-      int k = 1;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {
-#pragma unroll
-        for (int j = i; j < 6; ++j) v[k++] = row[i] * row[j];
-        v[22 + i] = row[i] * d;
-      }
+      row[0] = nz * sy - ny * sz;
+      row[1] = nx * sz - nz * sx;
+      row[2] = ny * sx - nx * sy;
+      row[3] = nx;
+      row[4] = ny;
+      row[5] = nz;
+      d = nx * (double)t.x + ny * (double)t.y + nz * (double)t.z - nx * sx - ny * sy - nz * sz;
     }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = i; j < 6; ++j) emit(row[i] * row[j]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) emit(row[i] * d);
   }
+  emit(has ? (double)d2 : 0.0);
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double sum = lds[threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < TILE / 64; ++w) sum += lds[w * NV + threadIdx.x];
+    out[threadIdx.x] = sum;
+  }
+  __syncthreads();  // lds is reused by the next tile
 }
 
 // PREC 0: exact per-lane grid search; 1 / 2: FP16 / BF16 matrix-core distance
@@ -130,12 +155,12 @@ template <bool P2PL, int TILE, int PREC>
 __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const IcpConst c) {
   constexpr int NV = P2PL ? kP2LVals : kP2PVals;
   static_assert(PREC == 0 || TILE == 64, "distance tiles need one wavefront per tile");
+  static_assert(TILE >= NV, "one thread per value writes the tile's sums");
   __shared__ double lds[NV * (TILE / 64)];
-  __shared__ int s_last;
   const TileRange tr = xcd_tiles(b.ntiles, c.interleave);
   for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     const int p = b.tile_pair[tile];
-    PairState* st = b.state + p;
+    const PairState* st = b.state + p;
     if (st->active == 0) continue;  // uniform across the block
     const PairDesc pd = b.pairs[p];
     const GridDesc gs = b.grids[pd.src];
@@ -174,67 +199,69 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
     const float rlim = r * 1.00001f;
     bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
                 nn.id, nn.j);
-    double v[NV];  // FP64 sums of this tile (zeroed after the search: fewer live registers)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = 0.0;
+    bool has = false;
+    float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
     if (own) {
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
-        const float4 t = b.pts[nn.j];
-        const float4 n = P2PL ? b.nrm[nn.j] : make_float4(0.f, 0.f, 0.f, 0.f);
-        accumulate<P2PL>(v, qx, qy, qz, t, n, nn.d2);
+        has = true;
+        t = b.pts[nn.j];
+        if (P2PL) n = b.nrm[nn.j];
       }
     }
-    block_sum<NV>(v, lds);
-    if (threadIdx.x == 0) {
-      double* out = b.partials + (int64_t)tile * kMaxVals;
+    tile_moments<P2PL, TILE>(has, qx, qy, qz, t, n, nn.d2, lds, b.partials + (int64_t)tile * kMaxVals);
+  }
+}
+
+// Second half of an iteration, one block per instance: folds the instance's tile
+// partials in a fixed order (thread t takes tiles t, t + kSolveThreads, ..., then
+// the block sum), then the FP64 solve, the composition final = Tinc * final and the
+// convergence step.  Kept out of k_icp_iter so that the search kernel carries none
+// of the solver's registers.
+constexpr int kSolveThreads = 256;
+
+template <bool P2PL>
+__global__ void __launch_bounds__(kSolveThreads) k_icp_solve(const IcpBuffers b, const IcpConst c) {
+  constexpr int NV = P2PL ? kP2LVals : kP2PVals;
+  __shared__ double lds[NV * (kSolveThreads / 64)];
+  PairState* st = b.state + blockIdx.x;
+  if (st->active == 0) return;  // uniform across the block
+  const PairDesc pd = b.pairs[blockIdx.x];
+  double v[NV];
 #pragma unroll
-      for (int i = 0; i < NV; ++i) out[i] = v[i];
-      __threadfence();
-      const unsigned t = atomicAdd(&st->ticket, 1u);
-      s_last = (t == (unsigned)(pd.ntiles - 1));
-    }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
+  for (int i = 0; i < NV; ++i) v[i] = 0.0;
+  for (int t = threadIdx.x; t < pd.ntiles; t += kSolveThreads) {
+    const double* in = b.partials + (int64_t)(pd.tile_begin + t) * kMaxVals;
 #pragma unroll
-      for (int i = 0; i < NV; ++i) v[i] = 0.0;
-      for (int t = threadIdx.x; t < pd.ntiles; t += TILE) {
-        const double* in = b.partials + (int64_t)(pd.tile_begin + t) * kMaxVals;
+    for (int i = 0; i < NV; ++i) v[i] += in[i];
+  }
+  block_sum<NV>(v, lds);
+  if (threadIdx.x != 0) return;
+  const double ncorr = v[0];
+  st->ncorr = (int)ncorr;
+  if (ncorr < 3.0) {
+    // icp.hpp: fewer than min_number_correspondences_ (3) -> not converged
+    st->state = kNoCorrespondences;
+    st->converged = 0;
+    st->active = 0;
+    return;
+  }
+  double Tinc[16];
+  if constexpr (P2PL) {
+    st->solve_failed = p2pl_increment(v, Tinc) ? 0 : 1;
+  } else {
+    p2p_increment(v, Tinc);
+  }
+  double Tn[16];
+  mat4_mul(Tinc, st->T, Tn);
 #pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] += in[i];
-      }
-      block_sum<NV>(v, lds);
-      if (threadIdx.x == 0) {
-        const double ncorr = v[0];
-        st->ncorr = (int)ncorr;
-        if (ncorr < 3.0) {
-          // icp.hpp: fewer than min_number_correspondences_ (3) -> not converged
-          st->state = kNoCorrespondences;
-          st->converged = 0;
-          st->active = 0;
-        } else {
-          double Tinc[16];
-          if constexpr (P2PL) {
-            st->solve_failed = p2pl_increment(v, Tinc) ? 0 : 1;
-          } else {
-            p2p_increment(v, Tinc);
-          }
-          double Tn[16];
-          mat4_mul(Tinc, st->T, Tn);
-#pragma unroll
-          for (int i = 0; i < 16; ++i) st->T[i] = Tn[i];
-          st->iterations += 1;
-          const double mse = v[NV - 1] / ncorr;
-          st->last_mse = mse;
-          if (conv_step(st, c, Tinc, mse)) {
-            st->converged = 1;
-            st->active = 0;
-          }
-        }
-        st->ticket = 0u;
-      }
-    }
+  for (int i = 0; i < 16; ++i) st->T[i] = Tn[i];
+  st->iterations += 1;
+  const double mse = v[NV - 1] / ncorr;
+  st->last_mse = mse;
+  if (conv_step(st, c, Tinc, mse)) {
+    st->converged = 1;
+    st->active = 0;
   }
 }
 
@@ -469,13 +496,8 @@ void launch_iter_tiles(const IcpBuffers& b, const IcpConst& c, int nblocks, hipS
     hipLaunchKernelGGL((k_icp_iter<false, 64, PREC>), dim3(nblocks), dim3(64), 0, s, b, c);
 }
 
-}  // namespace
-
-void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+void launch_iter_scalar(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
   const bool pl = c.point_to_plane != 0;
-  if (c.precision == 1) return launch_iter_tiles<1>(b, c, nblocks, s);
-  if (c.precision == 2) return launch_iter_tiles<2>(b, c, nblocks, s);
-  if (c.precision == 3) return launch_iter_tiles<3>(b, c, nblocks, s);
   switch (b.tile) {
     case 64:
       pl ? launch_iter_t<true, 64>(b, c, nblocks, s) : launch_iter_t<false, 64>(b, c, nblocks, s);
@@ -487,6 +509,25 @@ void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, h
       pl ? launch_iter_t<true, 256>(b, c, nblocks, s) : launch_iter_t<false, 256>(b, c, nblocks, s);
       break;
   }
+}
+
+}  // namespace
+
+void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+  const bool pl = c.point_to_plane != 0;
+  if (c.precision == 1)
+    launch_iter_tiles<1>(b, c, nblocks, s);
+  else if (c.precision == 2)
+    launch_iter_tiles<2>(b, c, nblocks, s);
+  else if (c.precision == 3)
+    launch_iter_tiles<3>(b, c, nblocks, s);
+  else
+    launch_iter_scalar(b, c, nblocks, s);
+  if (b.npairs <= 0) return;
+  if (pl)
+    hipLaunchKernelGGL(k_icp_solve<true>, dim3(b.npairs), dim3(kSolveThreads), 0, s, b, c);
+  else
+    hipLaunchKernelGGL(k_icp_solve<false>, dim3(b.npairs), dim3(kSolveThreads), 0, s, b, c);
 }
 
 void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,

@@ -365,6 +365,30 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// v + (v moved by the DPP pattern `ctrl` inside each row of 16 lanes), in FP64.
+template <int ctrl>
+__device__ __forceinline__ double dpp_add(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, true);
+  return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes, returned in every lane: inside each row of 16 lanes by
+// DPP (quad swaps, then rotations by 4 and 8), then the four row totals through
+// readlane.  Same value in every lane; the order of the additions is fixed.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x124>(v);  // row_ror:4
+  v = dpp_add<0x128>(v);  // row_ror:8
+  double r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    r[k] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * k),
+                            __builtin_amdgcn_readlane(__double2loint(v), 16 * k));
+  return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
 // Block-wide sum of N doubles per thread; the totals are valid in thread 0.
 // `lds` must hold N * (blockDim.x / 64) doubles.
 template <int N>

@@ -127,8 +127,10 @@ def test_cycle_point_to_point_and_empty_cloud(lcr):
 
 def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
     """The tuner only changes the launch shape: transforms agree with the default
-    setting to rounding (the reduction order follows the tile size), and a second
-    tuner on the same workload reads its choice back from the cache."""
+    setting to rounding (the FP64 reduction order follows the tile size and the
+    source's cell order; the rotation angle, an arccos of the trace, resolves only
+    about 1e-8 rad near zero), and a second tuner on the same workload reads its
+    choice back from the cache."""
     cyc, normals = small_cycle
     print_case(test="autotune")
     base = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=cyc.meta["prior"],
@@ -146,7 +148,7 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
     r1 = c.results()
     for a, b in zip(r0["T"], r1["T"]):
         er, et = synth.transform_error(a, b)
-        assert er < 1e-9 and et < 1e-10
+        assert er < 1e-7 and et < 1e-9
     np.testing.assert_array_equal(r0["state"], r1["state"])
     c2 = lcr.Cycle(cyc.clouds, cyc.pairs, normals=normals, guesses=cyc.meta["prior"],
                    point_to_plane=True, **KW)
@@ -159,9 +161,10 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
 def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
     """With hyp_gate, each pair's hypotheses are scored by their source points
     within r at the initial pose (on a fixed subsample of at least 64 tiles of 64
-    source points) and only the best is refined: bitwise the single-pair ICP from that pose.  Against the
-    oracle's full inlier counts: a clear winner (1.5x the runner-up) must be the
-    one kept, otherwise the kept pose has at least 2/3 of the best count."""
+    source points) and only the best is refined: bitwise the single-pair ICP from
+    that pose.  Against the oracle's full inlier counts: a clear winner (1.5x the
+    runner-up) must be the one kept, otherwise the kept pose has at least 2/3 of
+    the best count."""
     from lowcost3dreconstruction_amd.synth import perturb, rigid
     cyc, normals = small_cycle
     H = 4
