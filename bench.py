@@ -220,8 +220,10 @@ def main():
     _, nrm_ms = lcr.prof_read("normals")
     _, prop_ms = lcr.prof_read("proposal")
     _, gate_ms = lcr.prof_read("gate")
+    _, solve_ms = lcr.prof_read("icp_solve")
     log(f"batched step {step_s * 1e3:.3f} ms (profiled replay {prof_step_s * 1e3:.3f} ms); "
-        f"per step: icp {iter_ms / args.steps:.3f} ms ({n_iter_launches // args.steps} launches), "
+        f"per step: icp search {iter_ms / args.steps:.3f} ms ({n_iter_launches // args.steps} "
+        f"launches) + solve {solve_ms / args.steps:.3f} ms, "
         f"fitness {fit_ms / args.steps:.3f} ms, grids {grid_ms / args.steps:.3f} ms, "
         f"normals {nrm_ms / args.steps:.3f} ms, proposal {prop_ms / args.steps:.3f} ms")
 
@@ -355,7 +357,8 @@ def main():
                      "kernel": "icp_iteration", "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "precision_modes": modes,
-        "kernel_ms_per_step": {"icp_iteration": iter_ms / args.steps, "fitness": fit_ms / args.steps,
+        "kernel_ms_per_step": {"icp_iteration": iter_ms / args.steps,
+                               "icp_solve": solve_ms / args.steps, "fitness": fit_ms / args.steps,
                                "grid_build": grid_ms / args.steps, "normals": nrm_ms / args.steps,
                                "proposal": prop_ms / args.steps, "gate": gate_ms / args.steps,
                                "profiled_step": prof_step_s * 1e3},

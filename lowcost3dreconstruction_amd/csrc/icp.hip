@@ -513,8 +513,7 @@ void launch_iter_scalar(const IcpBuffers& b, const IcpConst& c, int nblocks, hip
 
 }  // namespace
 
-void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
-  const bool pl = c.point_to_plane != 0;
+void launch_icp_search(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
   if (c.precision == 1)
     launch_iter_tiles<1>(b, c, nblocks, s);
   else if (c.precision == 2)
@@ -523,8 +522,11 @@ void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, h
     launch_iter_tiles<3>(b, c, nblocks, s);
   else
     launch_iter_scalar(b, c, nblocks, s);
+}
+
+void launch_icp_solve(const IcpBuffers& b, const IcpConst& c, hipStream_t s) {
   if (b.npairs <= 0) return;
-  if (pl)
+  if (c.point_to_plane)
     hipLaunchKernelGGL(k_icp_solve<true>, dim3(b.npairs), dim3(kSolveThreads), 0, s, b, c);
   else
     hipLaunchKernelGGL(k_icp_solve<false>, dim3(b.npairs), dim3(kSolveThreads), 0, s, b, c);

@@ -56,8 +56,10 @@ struct IcpBuffers {
   int tile;               // source points per tile (= threads per block)
 };
 
-// One ICP iteration for every active instance (one launch).
-void launch_icp_iteration(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s);
+// One ICP iteration for every active instance: the correspondence search with
+// per-tile moment partials, then the per-instance fold, solve and update.
+void launch_icp_search(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s);
+void launch_icp_solve(const IcpBuffers& b, const IcpConst& c, hipStream_t s);
 // getFitnessScore(max_range) for every instance with a nonzero iteration count or
 // all instances when `all` is set.
 // getFitnessScore(max_range) of every instance that is not gated; with

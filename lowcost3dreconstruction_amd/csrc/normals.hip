@@ -185,6 +185,7 @@ struct Staged {
   int cnt;        // points in LDS
   int total;      // points that qualified (more than kCap: LDS holds a subset)
   unsigned filt;  // d2 filter as a bit pattern; 0xffffffff: none
+  unsigned lastT; // the previous query's k-th smallest d2 (bit pattern), 0: none
 };
 
 constexpr unsigned kNoFilter = 0xffffffffu;
@@ -333,26 +334,49 @@ __device__ __forceinline__ unsigned kth_slots(const unsigned (&d)[NS], int ns, i
 }
 
 // The kk-th smallest d2 of the cnt staged points (cnt >= kk).  With more than 64
-// points and kk <= 64, the kk-th smallest of the 64 per-lane minima (distinct
-// staged points) bounds it from above; when at most 64 points lie under that
-// bound they are compacted to one per lane (cd, ci: d2 bits and staged index,
-// through cb) and the bisection runs over that single slot.  comp tells which.
+// points and kk <= 64 the points are filtered by a bound U with at least kk and
+// at most 64 staged points under it -- then the kk-th smallest lies under U --
+// and compacted to one per lane (cd, ci: d2 bits and staged index, through cb);
+// the bisection runs over that single slot.  U is the guess (a nearby query's
+// k-th d2, grown by 1.44) when its count qualifies, possibly after one doubling
+// or halving; else the kk-th smallest of the 64 per-lane minima, which are kk
+// distinct staged points.  comp tells whether the compacted slot was used.
 __device__ __forceinline__ unsigned kth_staged(const unsigned (&slot)[kSlots], int cnt, int kk,
-                                               const float4* sp, const float4& q, int* cb,
-                                               unsigned& cd, int& ci, bool& comp) {
+                                               unsigned guess, const float4* sp, const float4& q,
+                                               int* cb, unsigned& cd, int& ci, bool& comp) {
   const int lane = threadIdx.x & 63;
   const int nslots = (cnt + 63) >> 6;
   comp = false;
   if (nslots > 1 && kk <= 64) {
-    unsigned mn = slot[0];
+    unsigned U = kInfBits;
+    int c = 65;  // > 64: no qualifying bound yet
+    if (guess != 0u) {
+      float gf = __uint_as_float(guess) * 1.44f;
+      for (int t = 0; t < 2; ++t) {
+        const unsigned gb = __float_as_uint(gf);
+        int n = 0;
 #pragma unroll
-    for (int m = 1; m < kSlots; ++m) mn = min(mn, slot[m]);
-    const unsigned mins[1] = {mn};
-    const unsigned U = kth_slots<1>(mins, 1, kk, kInfBits);
-    int c = 0;
+        for (int m = 0; m < kSlots; ++m)
+          if (m < nslots) n += popc_ballot(slot[m] <= gb);
+        if (n >= kk && n <= 64) {
+          U = gb;
+          c = n;
+          break;
+        }
+        gf = n < kk ? gf * 2.0f : gf * 0.5f;
+      }
+    }
+    if (c > 64) {
+      unsigned mn = slot[0];
 #pragma unroll
-    for (int m = 0; m < kSlots; ++m)
-      if (m < nslots) c += popc_ballot(slot[m] <= U);
+      for (int m = 1; m < kSlots; ++m) mn = min(mn, slot[m]);
+      const unsigned mins[1] = {mn};
+      U = kth_slots<1>(mins, 1, kk, kInfBits);
+      c = 0;
+#pragma unroll
+      for (int m = 0; m < kSlots; ++m)
+        if (m < nslots) c += popc_ballot(slot[m] <= U);
+    }
     if (c <= 64) {
       wave_lds_sync();
       int base = 0;
@@ -461,7 +485,7 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
     const bool covers = (Lc == INFINITY);
     if (st.cnt >= kk) {
       fill_slots(sp, st.cnt, q, slot);
-      T = kth_staged(slot, st.cnt, kk, sp, q, cb, cd, ci, comp);
+      T = kth_staged(slot, st.cnt, kk, st.lastT, sp, q, cb, cd, ci, comp);
       const float L = (Lc - cp.slack) * g.h;
       if (covers || (L > 0.0f && __uint_as_float(T) < L * L * 0.99998f)) break;
       // the next block's kk-th smallest d2 is at most T
@@ -477,6 +501,7 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
     stage_block(g, cs, pts, cp, K, q, kNoFilter, sp, st);
     st.cell = reusable(st) ? cell : -1;
   }
+  st.lastT = T;
   if (comp) {
     const unsigned d1[1] = {cd};
     const int i1[1] = {ci};
@@ -502,13 +527,12 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
   constexpr int NV = MODE == kMoments ? 9 : 1;
   __shared__ float4 s_p[kWaves][kCap];
   __shared__ int s_cb[kWaves][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 64;
   if (base >= total) return;
   const int64_t jq = base + lane;
   const bool own = jq < total;
-  const float4 qmine = own ? pts[jq] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const int cmine = own ? cloud_of_sorted[jq] : -1;
   const int nq = (int)min<int64_t>(64, total - base);
   Staged st;
   st.cell = -1;
@@ -516,14 +540,14 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
   st.cnt = 0;
   st.total = 0;
   st.filt = kNoFilter;
+  st.lastT = 0u;
   int mine_n = 0;
   for (int qi = 0; qi < nq; ++qi) {
-    float4 q;
-    q.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.x), qi));
-    q.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.y), qi));
-    q.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.z), qi));
-    q.w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qmine.w), qi));
-    const GridDesc g = grids[__builtin_amdgcn_readlane(cmine, qi)];
+    // wave-uniform addresses: scalar loads, so no vector load issued before the
+    // loop stays pending into it (a pending one makes every iteration wait for
+    // the previous iteration's stores)
+    const float4 q = pts[base + qi];
+    const GridDesc g = grids[cloud_of_sorted[base + qi]];
     const int kk = min(k, g.npts);
     Moments acc;
 #pragma unroll

@@ -496,8 +496,11 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   auto chunk = [&](std::vector<Profiler::Span>* sp) {
     for (int it = 0; it < kChunk; ++it) {
       const int k = prof.begin("icp_iteration", s, sp);
-      launch_icp_iteration(b, c, nblocks, s);
+      launch_icp_search(b, c, nblocks, s);
       prof.end(k, s, sp);
+      const int k2 = prof.begin("icp_solve", s, sp);
+      launch_icp_solve(b, c, s);
+      prof.end(k2, s, sp);
     }
     hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, d_state.as<PairState>(), npairs,
                        d_active.as<int>());
