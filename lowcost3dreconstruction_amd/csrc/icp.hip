@@ -194,10 +194,10 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
       }
     }
     if (search)
-      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded, seed,
+      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, bounded, seed,
                     2, true);
     const float rlim = r * 1.00001f;
-    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
+    bvh_nn_wave(gt, b.tree(), nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
                 nn.id, nn.j);
     bool has = false;
     float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
@@ -299,10 +299,10 @@ __global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max
       const bool empty = bounded && b.near && gt.near_r >= rq && !near_cell(gt, b.near, qx, qy, qz);
       const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
       if (!empty)
-        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, bounded, seed,
+        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, bounded, seed,
                       2, true);
     }
-    bvh_nn_wave(gt, b.nodes, b.pts, nn.tree, qx, qy, qz, bounded ? r * r : INFINITY, nn.d2, nn.id,
+    bvh_nn_wave(gt, b.tree(), nn.tree, qx, qy, qz, bounded ? r * r : INFINITY, nn.d2, nn.id,
                 nn.j);
     if (own && nn.j >= 0 && (double)nn.d2 <= max_range) {
       v[0] = nn.d2;
@@ -406,7 +406,7 @@ __global__ void k_correspondences(const IcpBuffers b, int pair, int32_t* idx, fl
     return;
   }
   const NNResult nn =
-      nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.nodes, qx, qy, qz, r, isfinite(r));
+      nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, isfinite(r));
   const bool keep = nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2;
   idx[o] = keep ? nn.id : -1;
   d2[o] = nn.d2;

@@ -39,8 +39,20 @@ struct GridDesc {
   float near_r;       // radius of the cell mask "some point within near_r" (0 = no mask)
 };
 
-// Points per leaf of the bounding-volume tree over a cloud's cell-sorted points.
+// Points per leaf of the bounding-volume tree over a cloud's points.
 constexpr int kLeaf = 16;
+
+// The cloud's bounding-volume tree: an implicit complete binary tree (node i has
+// children 2i and 2i+1; empty padding leaves have lo > hi) over runs of kLeaf
+// points in tree order -- the cloud's points sorted by the Morton code of their
+// position (tree_key), so that every node bounds a compact region.  P holds the
+// points in that order (pt_off-based like the cell-sorted array) and pos[i] the
+// cell-sorted position of P[i]; nodes == nullptr: no tree.
+struct Tree {
+  const float4* nodes;
+  const float4* P;
+  const int* pos;
+};
 
 // One ICP instance (a pair, or one pose hypothesis of a pair).
 struct PairDesc {

@@ -34,6 +34,15 @@ void launch_unsort_normals(const float4* pts, const float4* nrm, const int* clou
                            const int64_t* cloud_off, int64_t n_sorted, float4* raw_nrm,
                            int64_t n_raw, hipStream_t s);
 // bounding-volume trees (GridDesc::node_off / depth set by the host)
+// Tree order (Tree): Morton keys of the cell-sorted points, a radix sort of
+// (cloud, key) carrying each point's cell-sorted position into tpos, then the
+// points gathered into tpts.  keys, keys_sorted, vals: npts each; tmp: at least
+// tree_order_tmp_bytes(npts, nclouds).
+size_t tree_order_tmp_bytes(int64_t npts, int nclouds);
+hipError_t launch_tree_order(const GridDesc* grids, int nclouds, const float4* pts,
+                             const int* cloud_of_sorted, int64_t npts, uint64_t* keys,
+                             uint64_t* keys_sorted, int* vals, int* tpos, void* tmp, size_t tmp_bytes,
+                             float4* tpts, hipStream_t s);
 void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const float4* pts,
                       float4* nodes, hipStream_t s);
 
@@ -44,6 +53,8 @@ struct IcpBuffers {
   const float4* pts;      // cell-sorted points of all clouds
   const float4* nrm;      // cell-sorted normals (nullable for point-to-point)
   const float4* nodes;    // bounding-volume trees (2 float4 per node)
+  const float4* tpts;     // points in tree order (Tree::P)
+  const int* tpos;        // cell-sorted position of each tree-order point
   const PairDesc* pairs;
   const int* tile_pair;   // tile -> instance
   PairState* state;
@@ -54,6 +65,7 @@ struct IcpBuffers {
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)
+  __host__ __device__ Tree tree() const { return Tree{nodes, tpts, tpos}; }
 };
 
 // One ICP iteration for every active instance: the correspondence search with

@@ -69,6 +69,9 @@ LCR_SHD float block_gap(const GridDesc& g, const CellPos& c, int k) {
   return L;
 }
 
+// Radius (in cells) up to which a bounded query stays in the ring search.
+constexpr float kRingOnlyCells = 3.0f;
+
 // First ring that intersects the grid.
 LCR_SHD int first_ring(const GridDesc& g, const CellPos& c) {
   const int kx = c.cx < 0 ? -c.cx : (c.cx >= g.nx ? c.cx - g.nx + 1 : 0);
@@ -155,17 +158,37 @@ LCR_SHD float box_d2(const float4& lo, const float4& hi, float qx, float qy, flo
   return (dx * dx + dy * dy) + dz * dz;
 }
 
-// Exact 1-NN by traversal of the cloud's bounding-volume tree (an implicit
-// complete binary tree over runs of kLeaf cell-sorted points; node i has children
-// 2i and 2i+1, empty padding leaves have lo > hi).  Depth-first, nearer child
+// 10 bits spread to every third bit.
+LCR_SHD uint32_t spread3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// 30-bit Morton code of a point on a 1024^3 lattice over the grid's box (the
+// same scale on every axis).  Any order is exact for the search; this one makes
+// the tree's boxes compact.
+LCR_SHD uint32_t tree_key(const GridDesc& g, float x, float y, float z) {
+  const int n = max(g.nx, max(g.ny, g.nz));
+  const float s = 1024.0f / (g.h * (float)n);
+  const int ix = min(max((int)((x - g.ox) * s), 0), 1023);
+  const int iy = min(max((int)((y - g.oy) * s), 0), 1023);
+  const int iz = min(max((int)((z - g.oz) * s), 0), 1023);
+  return spread3((uint32_t)ix) | (spread3((uint32_t)iy) << 1) | (spread3((uint32_t)iz) << 2);
+}
+
+// Exact 1-NN by traversal of the cloud's tree (see Tree).  Depth-first, nearer child
 // first, no stack: `order` keeps one bit per depth saying whether the right child
 // was entered first.  A subtree is skipped only when its box is farther than the
 // bound by a relative margin, so equal distances are still visited.  `limit2`
 // (may be +inf) caps the bound for radius-limited searches.
-LCR_SHD void bvh_nn(const GridDesc& g, const float4* __restrict__ nodes,
-                    const float4* __restrict__ P, float qx, float qy, float qz, float limit2,
+LCR_SHD void bvh_nn(const GridDesc& g, const Tree& tr, float qx, float qy, float qz, float limit2,
                     float& best, int& bid, int& bj) {
-  const float4* N = nodes + 2 * g.node_off;
+  const float4* N = tr.nodes + 2 * g.node_off;
+  int tj = -1;  // tree-order index of the best point once a leaf improves it
   const unsigned first_leaf = 1u << g.depth;
   const int end = (int)g.pt_off + g.npts;
   unsigned i = 1, order = 0;
@@ -175,7 +198,7 @@ LCR_SHD void bvh_nn(const GridDesc& g, const float4* __restrict__ nodes,
     bool descend = box_d2(N[2 * i], N[2 * i + 1], qx, qy, qz) <= bound;
     if (descend && i >= first_leaf) {
       const int s = (int)g.pt_off + (int)(i - first_leaf) * kLeaf;
-      scan_points(P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, bj);
+      scan_points(tr.P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, tj);
       descend = false;
     }
     if (descend) {
@@ -190,7 +213,10 @@ LCR_SHD void bvh_nn(const GridDesc& g, const float4* __restrict__ nodes,
     }
     // next subtree: the sibling if i was entered first, else climb
     for (;;) {
-      if (depth == 0) return;
+      if (depth == 0) {
+        if (tj >= 0) bj = tr.pos[tj];
+        return;
+      }
       const unsigned right_first = (order >> depth) & 1u;
       if ((i & 1u) == right_first) {
         i ^= 1u;
@@ -216,11 +242,14 @@ LCR_SHD void bvh_nn(const GridDesc& g, const float4* __restrict__ nodes,
 //
 // With `defer_tree`, the tree part is left to the caller (NNResult::tree), which
 // runs it for a whole wavefront at once (bvh_nn_wave); a query that found nothing
-// in its first ring is deferred straight away.
+// in its first ring is deferred straight away.  A bounded query whose radius is at
+// most kRingOnlyCells cells never uses the tree: rows and cells farther than r
+// are pruned, and the rings pass r within a few steps.
 LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
-                          const float4* __restrict__ P, const float4* __restrict__ nodes,
-                          float qx, float qy, float qz, float r, bool bounded, int seed_j = -1,
-                          int ring_limit = 2, bool defer_tree = false) {
+                          const float4* __restrict__ P, const Tree& tr, float qx, float qy,
+                          float qz, float r, bool bounded, int seed_j = -1, int ring_limit = 2,
+                          bool defer_tree = false) {
+  const float4* nodes = tr.nodes;
   NNResult res{INFINITY, 0x7fffffff, -1, false};
   if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
   const CellPos c = cell_pos(g, qx, qy, qz);
@@ -235,11 +264,15 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   int bid = 0x7fffffff, bj = -1;
   if (seed_j >= 0) scan_points(P, seed_j, seed_j + 1, qx, qy, qz, best, bid, bj);
   const float h2 = g.h * g.h;
-  if (nodes && k0 > ring_limit) {  // far outside the grid: straight to the tree
+  // a radius of at most kRingOnlyCells cells: rows and cells beyond it are pruned
+  // and the rings reach it within a few steps, so the tree is never needed
+  const bool ring_only = bounded && rlim * g.inv_h <= kRingOnlyCells;
+  const float lim2 = bounded ? rlim * rlim : INFINITY;
+  if (nodes && !ring_only && k0 > ring_limit) {  // far outside the grid: straight to the tree
     if (defer_tree)
       res.tree = true;
     else
-      bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
+      bvh_nn(g, tr, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
     res.d2 = best;
     res.id = bid;
     res.j = bj;
@@ -256,7 +289,7 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
         for (int y = y0; y <= y1; ++y) {
           const float gy = axis_gap(c.fy, y, c.cy, c.slack);
           // bound on d2 for any point of this row (cell units^2 -> m^2)
-          const float thr = best * 1.00002f;
+          const float thr = fminf(best, lim2) * 1.00002f;
           const float lyz = (gz * gz + gy * gy) * h2;
           if (lyz > thr) continue;
           int xa, xb;
@@ -294,11 +327,11 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
     const float L = (Lc - c.slack) * g.h;
     if (L > 0.0f && best < L * L * 0.99998f) break;
     if (bounded && L > rlim) break;
-    if (nodes && (k - k0 >= ring_limit || (defer_tree && best == INFINITY))) {
+    if (nodes && !ring_only && (k - k0 >= ring_limit || (defer_tree && best == INFINITY))) {
       if (defer_tree)
         res.tree = true;
       else
-        bvh_nn(g, nodes, P, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
+        bvh_nn(g, tr, qx, qy, qz, bounded ? rlim * rlim : INFINITY, best, bid, bj);
       break;
     }
   }
@@ -314,14 +347,14 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
 // majority of the active lanes.  The node index is wave-uniform, so node boxes
 // and leaf points are read once per wavefront.  Queries of one wavefront are
 // neighbouring cell-sorted source points, so their paths mostly coincide.
-__device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const float4* __restrict__ nodes,
-                                            const float4* __restrict__ P, bool active, float qx,
-                                            float qy, float qz, float limit2, float& best,
-                                            int& bid, int& bj) {
+__device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const Tree& tr, bool active,
+                                            float qx, float qy, float qz, float limit2,
+                                            float& best, int& bid, int& bj) {
   const uint64_t live = __ballot(active);
   if (live == 0) return;
   const int nlive = __popcll(live);
-  const float4* N = nodes + 2 * g.node_off;
+  const float4* N = tr.nodes + 2 * g.node_off;
+  int tj = -1;  // tree-order index of the best point once a leaf improves it
   const unsigned first_leaf = 1u << g.depth;
   const int end = (int)g.pt_off + g.npts;
   unsigned i = 1, order = 0;
@@ -331,7 +364,7 @@ __device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const float4* __r
     bool descend = __ballot(active && box_d2(N[2 * i], N[2 * i + 1], qx, qy, qz) <= bound) != 0;
     if (descend && i >= first_leaf) {
       const int s = (int)g.pt_off + (int)(i - first_leaf) * kLeaf;
-      if (active) scan_points(P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, bj);
+      if (active) scan_points(tr.P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, tj);
       descend = false;
     }
     if (descend) {
@@ -345,7 +378,10 @@ __device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const float4* __r
       continue;
     }
     for (;;) {
-      if (depth == 0) return;
+      if (depth == 0) {
+        if (tj >= 0) bj = tr.pos[tj];
+        return;
+      }
       const unsigned right_first = (order >> depth) & 1u;
       if ((i & 1u) == right_first) {
         i ^= 1u;

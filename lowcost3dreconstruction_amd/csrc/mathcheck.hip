@@ -123,7 +123,17 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
     const float* p = tgt + 3 * i;
     P[fill[cid[i]]++] = make_float4(p[0], p[1], p[2], __builtin_bit_cast(float, (int)i));
   }
-  // bounding-volume tree in the device layout
+  // tree order (Morton keys, stable) and the bounding-volume tree over it, in the
+  // device layout
+  std::vector<int> tpos((size_t)(npts > 0 ? npts : 1));
+  std::vector<uint32_t> tkey((size_t)(npts > 0 ? npts : 1));
+  for (int j = 0; j < npts; ++j) {
+    tpos[j] = j;
+    tkey[j] = tree_key(g, P[j].x, P[j].y, P[j].z);
+  }
+  std::stable_sort(tpos.begin(), tpos.begin() + npts, [&](int a, int b) { return tkey[a] < tkey[b]; });
+  std::vector<float4> TP((size_t)(npts > 0 ? npts : 1));
+  for (int i = 0; i < npts; ++i) TP[i] = P[tpos[i]];
   const int64_t leaves = (npts + kLeaf - 1) / kLeaf;
   int depth = 0;
   while (((int64_t)1 << depth) < leaves) ++depth;
@@ -135,8 +145,8 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
   for (int64_t i = 0; i < nslots; ++i) {
     float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
     for (int64_t j = i * kLeaf; j < std::min<int64_t>(i * kLeaf + kLeaf, npts); ++j) {
-      lo.x = fminf(lo.x, P[j].x); lo.y = fminf(lo.y, P[j].y); lo.z = fminf(lo.z, P[j].z);
-      hi.x = fmaxf(hi.x, P[j].x); hi.y = fmaxf(hi.y, P[j].y); hi.z = fmaxf(hi.z, P[j].z);
+      lo.x = fminf(lo.x, TP[j].x); lo.y = fminf(lo.y, TP[j].y); lo.z = fminf(lo.z, TP[j].z);
+      hi.x = fmaxf(hi.x, TP[j].x); hi.y = fmaxf(hi.y, TP[j].y); hi.z = fmaxf(hi.z, TP[j].z);
     }
     nodes[2 * (nslots + i)] = lo;
     nodes[2 * (nslots + i) + 1] = hi;
@@ -150,9 +160,9 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
   for (int j = 0; j < npts; ++j) pos[__builtin_bit_cast(int, P[j].w)] = j;
   for (int64_t i = 0; i < nq; ++i) {
     const int seed = (seed_ids && seed_ids[i] >= 0) ? pos[seed_ids[i]] : -1;
-    const NNResult res = nn_query(g, cs.data(), P.data(), ring_limit >= 0 ? nodes.data() : nullptr,
-                                  q[3 * i], q[3 * i + 1], q[3 * i + 2], r, bounded != 0, seed,
-                                  ring_limit >= 0 ? ring_limit : 0);
+    const Tree tr{ring_limit >= 0 ? nodes.data() : nullptr, TP.data(), tpos.data()};
+    const NNResult res = nn_query(g, cs.data(), P.data(), tr, q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
+                                  bounded != 0, seed, ring_limit >= 0 ? ring_limit : 0);
     const bool keep = res.j >= 0 && (!bounded || (double)res.d2 <= (double)r * (double)r);
     idx[i] = keep ? res.id : -1;
     d2[i] = res.d2;

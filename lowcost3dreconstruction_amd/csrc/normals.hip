@@ -333,6 +333,38 @@ __device__ __forceinline__ unsigned kth_slots(const unsigned (&d)[NS], int ns, i
   return lo;
 }
 
+// Maximum over the 64 lanes (every lane active): DPP inside each row of 16 lanes,
+// then the four row maxima.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, true));  // row_ror:4
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, true));  // row_ror:8
+  return max(max((unsigned)__builtin_amdgcn_readlane((int)v, 0), (unsigned)__builtin_amdgcn_readlane((int)v, 16)),
+             max((unsigned)__builtin_amdgcn_readlane((int)v, 32), (unsigned)__builtin_amdgcn_readlane((int)v, 48)));
+}
+
+// The kk-th smallest of one value per lane (d: a bit pattern <= 0x7f800000, or
+// 0xffffffff for none), given c = count(d <= hi) >= kk.  Bisection over the bit
+// pattern keeping count(d <= lo - 1) < kk <= count(d <= hi); it stops as soon as
+// exactly kk values lie at or under hi -- the kk-th smallest is then the largest
+// of them -- which takes about log2 of the values' spread instead of all 31 bits.
+__device__ __forceinline__ unsigned kth_one(unsigned d, int kk, unsigned hi, int c) {
+  unsigned lo = 0u;
+  while (c > kk && lo < hi) {
+    const unsigned mid = lo + ((hi - lo) >> 1);
+    const int m = popc_ballot(d <= mid);
+    if (m >= kk) {
+      hi = mid;
+      c = m;
+    } else {
+      lo = mid + 1;
+    }
+  }
+  if (c == kk) return wave_max_u32(d <= hi ? d : 0u);
+  return lo;  // lo == hi: equal values at the kk-th place
+}
+
 // The kk-th smallest d2 of the cnt staged points (cnt >= kk).  With more than 64
 // points and kk <= 64 the points are filtered by a bound U with at least kk and
 // at most 64 staged points under it -- then the kk-th smallest lies under U --
@@ -370,8 +402,7 @@ __device__ __forceinline__ unsigned kth_staged(const unsigned (&slot)[kSlots], i
       unsigned mn = slot[0];
 #pragma unroll
       for (int m = 1; m < kSlots; ++m) mn = min(mn, slot[m]);
-      const unsigned mins[1] = {mn};
-      U = kth_slots<1>(mins, 1, kk, kInfBits);
+      U = kth_one(mn, kk, kInfBits, 64);  // more than 64 staged points: every lane has one
       c = 0;
 #pragma unroll
       for (int m = 0; m < kSlots; ++m)
@@ -392,8 +423,7 @@ __device__ __forceinline__ unsigned kth_staged(const unsigned (&slot)[kSlots], i
       ci = lane < c ? cb[lane] : 0;
       cd = lane < c ? __float_as_uint(flann_d2(q.x, q.y, q.z, sp[ci])) : 0xffffffffu;
       comp = true;
-      const unsigned cds[1] = {cd};
-      return kth_slots<1>(cds, 1, kk, U);
+      return kth_one(cd, kk, U, c);
     }
   }
   return kth_slots<kSlots>(slot, nslots, kk, kInfBits);

@@ -310,14 +310,26 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
                       d_counts.as<int>(), d_slot_raw.as<int>(), d_slot_cell.as<int>(),
                       d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), d_pts.as<float4>(),
                       d_nrm.as<float4>(), s);
-  d_nodes.reserve((size_t)std::max<int64_t>(total_nodes, 1) * 2 * sizeof(float4));
-  launch_bvh_build(d_grids.as<GridDesc>(), n, max_depth, d_pts.as<float4>(), d_nodes.as<float4>(), s);
   // sorted point -> cloud
   for (int c = 0; c < n; ++c) hpoff[c] = grids[c].pt_off;
   hpoff[n] = total_pts;
   d_poff.reserve((size_t)(n + 1) * sizeof(int64_t));
   LCR_HIP(hipMemcpyAsync(d_poff.p, hpoff, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   launch_segments(d_poff.as<int64_t>(), n, total_pts, d_cloud_of_sorted.as<int>(), s);
+  // tree order, then the bounding-volume tree over it
+  d_tpts.reserve(npts * sizeof(float4));
+  d_tpos.reserve(npts * sizeof(int));
+  d_tkeys.reserve(npts * sizeof(uint64_t));
+  d_tkeys_sorted.reserve(npts * sizeof(uint64_t));
+  d_tvals.reserve(npts * sizeof(int));
+  const size_t tmp_bytes = tree_order_tmp_bytes(total_pts, n);
+  d_ttmp.reserve(std::max<size_t>(tmp_bytes, 16));
+  LCR_HIP(launch_tree_order(d_grids.as<GridDesc>(), n, d_pts.as<float4>(), d_cloud_of_sorted.as<int>(),
+                            total_pts, d_tkeys.as<uint64_t>(), d_tkeys_sorted.as<uint64_t>(),
+                            d_tvals.as<int>(), d_tpos.as<int>(), d_ttmp.p, tmp_bytes,
+                            d_tpts.as<float4>(), s));
+  d_nodes.reserve((size_t)std::max<int64_t>(total_nodes, 1) * 2 * sizeof(float4));
+  launch_bvh_build(d_grids.as<GridDesc>(), n, max_depth, d_tpts.as<float4>(), d_nodes.as<float4>(), s);
   prof.end(span, s, spans);
   LCR_HIP(hipStreamSynchronize(s));  // the pinned regions are reused by the next build
 }
@@ -431,6 +443,8 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.pts = cs.d_pts.as<float4>();
   b.nrm = cs.has_nrm ? cs.d_nrm.as<float4>() : nullptr;
   b.nodes = cs.d_nodes.as<float4>();
+  b.tpts = cs.d_tpts.as<float4>();
+  b.tpos = cs.d_tpos.as<int>();
   b.pairs = d_pairs.as<PairDesc>();
   b.tile_pair = d_tile_pair.as<int>();
   b.state = d_state.as<PairState>();
