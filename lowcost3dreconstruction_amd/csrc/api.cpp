@@ -449,7 +449,7 @@ int lcr_normals(const float* xyz, int64_t n, int64_t stride_bytes, int32_t k, co
     cs.upload(st.s, {xyz}, {nullptr}, {n}, stride_bytes);
     // cells sized for a k-neighbourhood: ~sqrt(k) point spacings
     const double factor = std::max(2.5, 0.75 * sqrt((double)k));
-    cs.build(st.s, factor, nullptr);
+    cs.build(st.s, factor, nullptr, false);
     const float v[3] = {vp ? vp[0] : 0.f, vp ? vp[1] : 0.f, vp ? vp[2] : 0.f};
     cs.compute_normals(st.s, k, v, flip_all, nullptr);
     cs.normals_to_raw(st.s);  // input order; NaN for non-finite points
@@ -548,7 +548,7 @@ int lcr_cycle_run(lcr_cycle* c) {
       // normals on a grid with cells sized for a k-neighbourhood (~sqrt(k) point
       // spacings), then the ICP grid with cells sized for 1-NN
       c->cs.has_nrm = false;
-      c->cs.build(c->stream.s, std::max(2.5, 0.75 * sqrt((double)k)), &spans);
+      c->cs.build(c->stream.s, std::max(2.5, 0.75 * sqrt((double)k)), &spans, false);
       c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
       c->cs.normals_to_raw(c->stream.s);
     }
@@ -761,7 +761,7 @@ int lcr_outlier_removal(const float* xyz, int64_t n, int32_t mean_k, double stdd
     Stream st;
     CloudSet cs;
     cs.upload(st.s, {xyz}, {nullptr}, {n}, 3 * sizeof(float));
-    cs.build(st.s, std::max(2.5, 0.75 * sqrt((double)mean_k + 1.0)), nullptr);
+    cs.build(st.s, std::max(2.5, 0.75 * sqrt((double)mean_k + 1.0)), nullptr, false);
     DevBuf dist, fin, sums, nsel, defer, part, dstats, dkeep;
     dist.reserve((size_t)n * sizeof(float));
     fin.reserve((size_t)n);

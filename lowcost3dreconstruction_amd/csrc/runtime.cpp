@@ -177,7 +177,8 @@ struct BuildRegions {
 };
 }  // namespace
 
-void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans) {
+void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans,
+                     bool with_tree) {
   Profiler& prof = Profiler::get();
   if (factor <= 0.0) factor = 2.5;
   const BuildRegions R(n);
@@ -317,6 +318,11 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   LCR_HIP(hipMemcpyAsync(d_poff.p, hpoff, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   launch_segments(d_poff.as<int64_t>(), n, total_pts, d_cloud_of_sorted.as<int>(), s);
   // tree order, then the bounding-volume tree over it
+  if (!with_tree) {
+    prof.end(span, s, spans);
+    LCR_HIP(hipStreamSynchronize(s));  // the pinned regions are reused by the next build
+    return;
+  }
   d_tpts.reserve(npts * sizeof(float4));
   d_tpos.reserve(npts * sizeof(int));
   d_tkeys.reserve(npts * sizeof(uint64_t));

@@ -132,7 +132,9 @@ struct CloudSet {
               const std::vector<const float*>& nrm, const std::vector<int64_t>& counts,
               int64_t stride_bytes);
   // Builds every cloud's grid; cell edge = factor * estimated point spacing.
-  void build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans);
+  // with_tree: also the tree order and the bounding-volume tree (the 1-NN searches
+  // need them; the k-NN selections of normals and outlier removal do not)
+  void build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans, bool with_tree = true);
   // K7 normals for every cloud into d_nrm (cell-sorted order).
   void compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
                        std::vector<Profiler::Span>* spans);
