@@ -305,31 +305,36 @@ __device__ __forceinline__ int grid_of_cell(const GridDesc* __restrict__ g, int 
 
 // One axis of the dilation: out[c] = OR of in over the cells within kd along
 // `axis` (pass 0 reads occupancy from cell_start instead of `in`).
-__global__ void k_near_pass(const GridDesc* __restrict__ grids, int ngrids,
-                            const int* __restrict__ cs, const uint8_t* __restrict__ in,
-                            uint8_t* __restrict__ out, int64_t total, int axis, float r) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= total) return;
-  const GridDesc g = grids[grid_of_cell(grids, ngrids, c)];
-  const int64_t local = c - g.cell_off;
-  const int64_t ncells = (int64_t)g.nx * g.ny * g.nz;
-  if (g.npts == 0 || local >= ncells) {
-    out[c] = 0;
-    return;
+// One pass of the near mask's separable dilation along `axis`, one row of
+// blocks per grid (blockIdx.y): a cell is marked when a cell within kd along the
+// axis is (pass 0 reads occupancy from the cell-start table).  32-bit local
+// indices; the trailing sentinel cell of each grid is cleared.
+__global__ void k_near_pass(const GridDesc* __restrict__ grids, const int* __restrict__ cs,
+                            const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int axis,
+                            float r) {
+  const GridDesc& gd = grids[blockIdx.y];
+  const int nx = gd.nx, ny = gd.ny, nz = gd.nz, npts = gd.npts;
+  const int64_t off = gd.cell_off;
+  const int ncells = nx * ny * nz;
+  const int kd = (int)ceilf(r * 1.0001f * gd.inv_h) + 1;
+  const int dim = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  const int stride = axis == 0 ? 1 : (axis == 1 ? nx : nx * ny);
+  for (int local = blockIdx.x * blockDim.x + threadIdx.x; local <= ncells;
+       local += gridDim.x * blockDim.x) {
+    const int64_t c = off + local;
+    if (npts == 0 || local == ncells) {
+      out[c] = 0;
+      continue;
+    }
+    const int pos = axis == 0 ? local % nx : (axis == 1 ? (local / nx) % ny : local / (nx * ny));
+    const int lo = max(pos - kd, 0), hi = min(pos + kd, dim - 1);
+    uint8_t v = 0;
+    for (int p = lo; p <= hi && !v; ++p) {
+      const int64_t idx = c + (int64_t)(p - pos) * stride;
+      v = in ? in[idx] : (uint8_t)(cs[idx + 1] > cs[idx]);
+    }
+    out[c] = v;
   }
-  const int x = (int)(local % g.nx), y = (int)((local / g.nx) % g.ny),
-            z = (int)(local / ((int64_t)g.nx * g.ny));
-  const int kd = (int)ceilf(r * 1.0001f * g.inv_h) + 1;
-  const int pos = axis == 0 ? x : (axis == 1 ? y : z);
-  const int dim = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
-  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)g.nx : (int64_t)g.nx * g.ny);
-  const int lo = max(pos - kd, 0), hi = min(pos + kd, dim - 1);
-  uint8_t v = 0;
-  for (int p = lo; p <= hi && !v; ++p) {
-    const int64_t idx = c + (int64_t)(p - pos) * stride;
-    v = in ? in[idx] : (uint8_t)(cs[idx + 1] > cs[idx]);
-  }
-  out[c] = v;
 }
 
 __global__ void k_set_near_r(GridDesc* grids, int n, float r) {
@@ -398,13 +403,15 @@ void launch_unsort_normals(const float4* pts, const float4* nrm, const int* clou
 void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_cells,
                       float r, uint8_t* out, uint8_t* tmp, hipStream_t s) {
   if (total_cells <= 0 || ngrids <= 0) return;
-  const unsigned b = blocks_for(total_cells, 256);
-  hipLaunchKernelGGL(k_near_pass, dim3(b), dim3(256), 0, s, grids, ngrids, cell_start,
-                     (const uint8_t*)nullptr, out, total_cells, 0, r);
-  hipLaunchKernelGGL(k_near_pass, dim3(b), dim3(256), 0, s, grids, ngrids, cell_start,
-                     (const uint8_t*)out, tmp, total_cells, 1, r);
-  hipLaunchKernelGGL(k_near_pass, dim3(b), dim3(256), 0, s, grids, ngrids, cell_start,
-                     (const uint8_t*)tmp, out, total_cells, 2, r);
+  // per grid a row of blocks, grid-stride over its cells
+  const unsigned bx = (unsigned)std::min<int64_t>(blocks_for(total_cells / ngrids + 1, 256), 512);
+  const dim3 grid(bx, (unsigned)ngrids);
+  hipLaunchKernelGGL(k_near_pass, grid, dim3(256), 0, s, grids, cell_start, (const uint8_t*)nullptr,
+                     out, 0, r);
+  hipLaunchKernelGGL(k_near_pass, grid, dim3(256), 0, s, grids, cell_start, (const uint8_t*)out, tmp,
+                     1, r);
+  hipLaunchKernelGGL(k_near_pass, grid, dim3(256), 0, s, grids, cell_start, (const uint8_t*)tmp, out,
+                     2, r);
 }
 
 void launch_set_near_r(GridDesc* grids, int ngrids, float r, hipStream_t s) {
