@@ -193,8 +193,11 @@ def main():
         return lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses, n_hyp=n_hyp, point_to_plane=True,
                          normals_k=30, precision_mode=mode, proposal=p, hyp_gate=gate, **icp_kw())
 
-    # ---------------- batched cycle (the measured path)
+    # ---------------- batched cycle (the measured path); creating it copies the
+    # clouds host -> device once (the PCIe-inclusive rate below adds that time)
+    t_up = time.perf_counter()
     cycle = make_cycle(PRECISION[args.precision])
+    upload_s = time.perf_counter() - t_up
     tune = None
     if args.autotune:
         t_t = time.time()
@@ -309,6 +312,10 @@ def main():
         errs = np.zeros((len(cyc.pairs), 2))
     n_aligned = int(ok.sum())
     value = n_aligned / step_s
+    # aligned pairs by the hypothesis the step kept (0: the turntable prior)
+    kept_prior = np.asarray(res["best_hyp"]) == 0
+    by_kept = {"prior": [int(ok[kept_prior].sum()), int(kept_prior.sum())],
+               "other": [int(ok[~kept_prior].sum()), int((~kept_prior).sum())]}
     plain_value = float(ok_plain.mean()) / plain_s_per_pair if n_plain else None
     if no_prop is not None:
         no_prop["value"] = no_prop.get("aligned_pairs", len(cyc.pairs)) / (no_prop["ms_per_step"] / 1e3)
@@ -341,6 +348,9 @@ def main():
             "parallelism": "single GPU",
         },
         "aligned_pairs": n_aligned,
+        "aligned_by_kept_hypothesis": by_kept,
+        "upload_ms": upload_s * 1e3,
+        "pcie_inclusive_pairs_per_s": n_aligned / (step_s + upload_s),
         "max_rot_err_deg": float(np.rad2deg(errs[:, 0].max())),
         "max_trans_err_m": float(errs[:, 1].max()),
         "iterations_mean": float(res["iterations"].mean()),

@@ -102,8 +102,8 @@ int icp_max_blocks_per_cu(int tile, bool p2pl, int precision);
 // ---- K7 normals (normals.hip)
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
-                    int flip_all, double* mom, int* nsel, int* deferred, float4* nrm_sorted,
-                    hipStream_t s);
+                    int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
+                    int* deferred, float4* nrm_sorted, hipStream_t s);
 
 // ---- K11-K13 filters before ICP (filters.hip)
 // VoxelGrid lattice: voxel of p = floor(p * inv) - minb (FP32, per axis); index

@@ -21,9 +21,10 @@
 //   3. certify: the K-th distance must be below the distance to the staged
 //      block's boundary, else stage the next larger block, keeping only the points
 //      within the K-th distance found so far, and repeat;
-//   4. accumulate FP64 moments of the selected points relative to the query.
-// A second pass solves one FP64 3x3 eigenproblem per lane from the stored moment
-// sums (keeping the solver's registers out of the search).  Queries the staged
+//   4. list the selected points' original indices (ballot positions, fixed order).
+// A second pass, one point per lane, sums the FP64 moments of its listed
+// neighbours relative to it and solves the FP64 3x3 eigenproblem (keeping the
+// nine cross-lane FP64 sums and the solver's registers out of the search).  Queries the staged
 // search gives up on (very many equal distances, K above half the LDS capacity)
 // are listed and finished by a rescan pass that reads the block from memory for
 // every count.
@@ -58,6 +59,12 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 
 __device__ __forceinline__ int popc_ballot(bool p) { return __popcll(__ballot(p)); }
 
+// Number of set bits of b in the lanes below this one.
+__device__ __forceinline__ int lane_prefix(uint64_t b) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+}
+
 struct Moments {
   double v[9];  // sum dx, dy, dz, dx2, dxdy, dxdz, dy2, dydz, dz2 (d = p - q)
 };
@@ -91,6 +98,23 @@ __device__ __forceinline__ void take(Moments& m, const float4& q, const float4& 
     m.v[0] += sqrt((double)d2);
 }
 
+// One selected point per lane where sel: kMoments writes its original index to
+// nbr[base + rank among the wave's selected lanes] (the moments are summed from
+// this list by the solve pass); kSqrtDist adds sqrt(d2).  Every lane of the
+// wavefront calls it; returns how many lanes selected.
+template <int MODE>
+__device__ __forceinline__ int select_out(bool sel, const float4& p, unsigned d, const float4& q,
+                                          Moments& acc, int* nbr, int base) {
+  if constexpr (MODE == kMoments) {
+    const uint64_t b = __ballot(sel);
+    if (sel) nbr[base + lane_prefix(b)] = __float_as_int(p.w);
+    return __popcll(b);
+  } else {
+    if (sel) take<MODE>(acc, q, p, __uint_as_float(d));
+    return 0;
+  }
+}
+
 // Calls f(j) for this lane's share of the points of rings 0..K.
 template <class F>
 __device__ __forceinline__ void for_block_points(const GridDesc& g, const int* cs, const CellPos& cp,
@@ -114,7 +138,7 @@ __device__ __forceinline__ int block_count(const GridDesc& g, const int* cs, con
 // Exact path without an LDS buffer: every count rescans the block from memory.
 template <int MODE>
 __device__ void select_rescan(const GridDesc& g, const int* cs, const float4* pts, const float4& q,
-                              const CellPos& cp, int K, int kk, Moments& acc) {
+                              const CellPos& cp, int K, int kk, Moments& acc, int* nbr) {
   unsigned T = 0;
   for (;; ++K) {
     const float Lc = block_gap(g, cp, K);
@@ -166,11 +190,26 @@ __device__ void select_rescan(const GridDesc& g, const int* cs, const float4* pt
     }
     idcut = (int)lo;
   }
-  for_block_points(g, cs, cp, K, [&](int j) {
-    const float4 p = pts[j];
-    const unsigned d = __float_as_uint(flann_d2(q.x, q.y, q.z, p));
-    if (d < T || (d == T && __float_as_int(p.w) <= idcut)) take<MODE>(acc, q, p, __uint_as_float(d));
-  });
+  // the selected points, in block order (wave-uniform loop: ballot positions)
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  for (int ring = 0; ring <= K; ++ring) {
+    for_ring_runs(g, cp, ring, [&](int c0, int c1) {
+      const int s = cs[c0], e = cs[c1 + 1];
+      for (int jb = s; jb < e; jb += 64) {
+        const int j = jb + lane;
+        bool sel = false;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        unsigned d = 0u;
+        if (j < e) {
+          p = pts[j];
+          d = __float_as_uint(flann_d2(q.x, q.y, q.z, p));
+          sel = d < T || (d == T && __float_as_int(p.w) <= idcut);
+        }
+        base += select_out<MODE>(sel, p, d, q, acc, nbr, base);
+      }
+    });
+  }
 }
 
 // The wave's staged candidate block in LDS.  When the (2K+1)^3 cells around the
@@ -198,12 +237,6 @@ __device__ __forceinline__ void block_row(const GridDesc& g, const int* cs, int 
   const int row = (z * g.ny + y) * g.nx;
   s = cs[row + x0];
   len = cs[row + x1 + 1] - s;
-}
-
-// Number of set bits of b in the lanes below this one.
-__device__ __forceinline__ int lane_prefix(uint64_t b) {
-  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
 }
 
 // Stages block K around cp into sp (see Staged): the whole block when it fits
@@ -434,7 +467,8 @@ __device__ __forceinline__ unsigned kth_staged(const unsigned (&slot)[kSlots], i
 // kk-th smallest d2: all points below T and the smallest indices at T.
 template <int MODE, int NS>
 __device__ __forceinline__ void take_kth(const unsigned (&d)[NS], const int (&ix)[NS], int ns, int kk,
-                                         unsigned T, const float4* sp, const float4& q, Moments& acc) {
+                                         unsigned T, const float4* sp, const float4& q, Moments& acc,
+                                         int* nbr) {
   int clt = 0, cle = 0;
 #pragma unroll
   for (int m = 0; m < NS; ++m)
@@ -462,11 +496,17 @@ __device__ __forceinline__ void take_kth(const unsigned (&d)[NS], const int (&ix
     }
     idcut = (int)lo;
   }
+  int base = 0;
 #pragma unroll
   for (int m = 0; m < NS; ++m)
-    if (d[m] <= T) {
-      const float4 p = sp[ix[m]];
-      if (d[m] < T || __float_as_int(p.w) <= idcut) take<MODE>(acc, q, p, __uint_as_float(d[m]));
+    if (m < ns) {
+      bool sel = false;
+      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (d[m] <= T) {
+        p = sp[ix[m]];
+        sel = d[m] < T || __float_as_int(p.w) <= idcut;
+      }
+      base += select_out<MODE>(sel, p, d[m], q, acc, nbr, base);
     }
 }
 
@@ -484,7 +524,7 @@ constexpr int kMaxTighten = 8;
 template <int MODE>
 __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, const float4* pts,
                                               const float4& q, int kk, float4* sp, int* cb, Staged& st,
-                                              Moments& acc) {
+                                              Moments& acc, int* nbr) {
   const int lane = threadIdx.x & 63;
   const CellPos cp = cell_pos(g, q.x, q.y, q.z);
   const int64_t cell = (int64_t)g.cell_off + ((int64_t)cp.cz * g.ny + cp.cy) * g.nx + cp.cx;
@@ -535,26 +575,26 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
   if (comp) {
     const unsigned d1[1] = {cd};
     const int i1[1] = {ci};
-    take_kth<MODE, 1>(d1, i1, 1, kk, T, sp, q, acc);
+    take_kth<MODE, 1>(d1, i1, 1, kk, T, sp, q, acc, nbr);
   } else {
     int ix[kSlots];
 #pragma unroll
     for (int m = 0; m < kSlots; ++m) ix[m] = lane + 64 * m;
-    take_kth<MODE, kSlots>(slot, ix, (st.cnt + 63) >> 6, kk, T, sp, q, acc);
+    take_kth<MODE, kSlots>(slot, ix, (st.cnt + 63) >> 6, kk, T, sp, q, acc, nbr);
   }
   return true;
 }
 
-// Search pass: per point of its K-neighbourhood either the 9 FP64 moment sums
-// (kMoments, mom holds 9 doubles per point) or the sum of distances (kSqrtDist,
-// mom holds 1), and the neighbour count, in cell-sorted order.
+// Search pass, per point in cell-sorted order: the neighbour count and either
+// the original indices of its K nearest neighbours (kMoments: nbr holds k per
+// point, the moments are summed by the solve pass) or the FP64 sum of their
+// distances (kSqrtDist: mom holds 1 per point).
 template <int MODE>
 __global__ void __launch_bounds__(kWaves * 64)
 k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
                  const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
-                 int64_t total, int k, double* __restrict__ mom, int* __restrict__ nsel,
-                 int* __restrict__ deferred) {
-  constexpr int NV = MODE == kMoments ? 9 : 1;
+                 int64_t total, int k, double* __restrict__ mom, int* __restrict__ nbr,
+                 int* __restrict__ nsel, int* __restrict__ deferred) {
   __shared__ float4 s_p[kWaves][kCap];
   __shared__ int s_cb[kWaves][64];
   const int lane = threadIdx.x & 63;
@@ -582,20 +622,18 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
     Moments acc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
+    int* out = MODE == kMoments ? nbr + (base + qi) * k : nullptr;
     if (kk >= (MODE == kMoments ? 3 : 1) &&
-        !select_staged<MODE>(g, cell_start + g.cell_off, pts, q, kk, s_p[w], s_cb[w], st, acc)) {
+        !select_staged<MODE>(g, cell_start + g.cell_off, pts, q, kk, s_p[w], s_cb[w], st, acc, out)) {
       // deferred[0] counts the list deferred[1..]
       if (lane == 0) deferred[1 + atomicAdd(deferred, 1)] = (int)(base + qi);
       if (lane == qi) mine_n = kk;
       continue;
     }
-    double mv = 0.0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const double s = wave_sum_dpp(acc.v[i]);
-      if (lane == i) mv = s;
+    if constexpr (MODE == kSqrtDist) {
+      const double s = wave_sum_dpp(acc.v[0]);
+      if (lane == 0) mom[base + qi] = s;
     }
-    if (lane < NV) mom[(base + qi) * NV + lane] = mv;
     if (lane == qi) mine_n = kk;
   }
   if (own) nsel[jq] = mine_n;
@@ -607,8 +645,7 @@ template <int MODE>
 __global__ void __launch_bounds__(256)
 k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
                 const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted, int k,
-                const int* __restrict__ deferred, double* __restrict__ mom) {
-  constexpr int NV = MODE == kMoments ? 9 : 1;
+                const int* __restrict__ deferred, double* __restrict__ mom, int* __restrict__ nbr) {
   const int lane = threadIdx.x & 63;
   const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
   const int count = deferred[0];
@@ -621,22 +658,22 @@ k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell
     Moments acc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
-    select_rescan<MODE>(g, cell_start + g.cell_off, pts, q, cell_pos(g, q.x, q.y, q.z), 1, kk, acc);
-    double mv = 0.0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const double s = wave_sum_dpp(acc.v[i]);
-      if (lane == i) mv = s;
+    select_rescan<MODE>(g, cell_start + g.cell_off, pts, q, cell_pos(g, q.x, q.y, q.z), 1, kk, acc,
+                        MODE == kMoments ? nbr + j * k : nullptr);
+    if constexpr (MODE == kSqrtDist) {
+      const double s = wave_sum_dpp(acc.v[0]);
+      if (lane == 0) mom[j] = s;
     }
-    if (lane < NV) mom[j * NV + lane] = mv;
   }
 }
 
-// Solve pass, one point per lane: covariance, FP64 eigen-decomposition, normal
-// (eigenvector of the smallest eigenvalue) oriented towards the viewpoint.
+// Solve pass, one point per lane: FP64 moments of its neighbours relative to it
+// (in the order the search listed them), covariance, FP64 eigen-decomposition,
+// normal (eigenvector of the smallest eigenvalue) oriented towards the viewpoint.
 __global__ void __launch_bounds__(256)
 k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
-               const double* __restrict__ mom, const int* __restrict__ nsel, int64_t total,
+               const float4* __restrict__ raw, const int64_t* __restrict__ cloud_off,
+               const int* __restrict__ nbr, int k, const int* __restrict__ nsel, int64_t total,
                const float* __restrict__ vp, int flip_all, float4* __restrict__ out) {
   const int64_t jq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jq >= total) return;
@@ -646,9 +683,13 @@ k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_
     return;
   }
   const float4 q = pts[jq];
-  double m[9];
+  const float4* R = raw + cloud_off[cloud_of_sorted[jq]];
+  const int* list = nbr + jq * k;
+  Moments acc;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) m[i] = mom[jq * 9 + i];
+  for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
+  for (int i = 0; i < cnt; ++i) add_moment(acc, q, R[list[i]]);
+  const double* m = acc.v;
   const double n = (double)cnt;
   const double mx = m[0] / n, my = m[1] / n, mz = m[2] / n;
   double C[3][3];
@@ -705,27 +746,30 @@ void launch_sor_distances(const GridDesc* grids, const int* cell_start, const fl
   const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
   launch_fill_i32(deferred, 1, 0, s);
   hipLaunchKernelGGL(k_normal_moments<kSqrtDist>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
-                     cell_start, pts, cloud_of_sorted, npts_total, mean_k + 1, sums, nsel, deferred);
+                     cell_start, pts, cloud_of_sorted, npts_total, mean_k + 1, sums, (int*)nullptr, nsel,
+                     deferred);
   hipLaunchKernelGGL(k_normal_rescan<kSqrtDist>, dim3(1024), dim3(256), 0, s, grids, cell_start,
-                     pts, cloud_of_sorted, mean_k + 1, deferred, sums);
+                     pts, cloud_of_sorted, mean_k + 1, deferred, sums, (int*)nullptr);
   hipLaunchKernelGGL(k_sor_dist, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s, pts,
                      cloud_of_sorted, cloud_off, sums, npts_total, mean_k, dist, finite);
 }
 
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
-                    int flip_all, double* mom, int* nsel, int* deferred, float4* nrm_sorted,
-                    hipStream_t s) {
+                    int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
+                    int* deferred, float4* nrm_sorted, hipStream_t s) {
   if (npts_total <= 0) return;
   const int64_t per_block = (int64_t)kWaves * 64;
   const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
   launch_fill_i32(deferred, 1, 0, s);
   hipLaunchKernelGGL(k_normal_moments<kMoments>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
-                     cell_start, pts, cloud_of_sorted, npts_total, k, mom, nsel, deferred);
+                     cell_start, pts, cloud_of_sorted, npts_total, k, (double*)nullptr, nbr, nsel,
+                     deferred);
   hipLaunchKernelGGL(k_normal_rescan<kMoments>, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
-                     cloud_of_sorted, k, deferred, mom);
+                     cloud_of_sorted, k, deferred, (double*)nullptr, nbr);
   hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s,
-                     pts, cloud_of_sorted, mom, nsel, npts_total, vp, flip_all, nrm_sorted);
+                     pts, cloud_of_sorted, raw, cloud_off, nbr, k, nsel, npts_total, vp, flip_all,
+                     nrm_sorted);
 }
 
 }  // namespace lcr

@@ -351,12 +351,13 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
   LCR_HIP(hipMemcpyAsync(d_vp.p, v, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
   const int span = prof.begin("normals", s, spans);
   const size_t np = (size_t)std::max<int64_t>(total_pts, 1);
-  d_mom.reserve(np * 9 * sizeof(double));
+  d_nbr.reserve(np * (size_t)k * sizeof(int));
   d_nsel.reserve(np * sizeof(int));
   d_defer.reserve((np + 1) * sizeof(int));
   launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
                  d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
-                 d_mom.as<double>(), d_nsel.as<int>(), d_defer.as<int>(), d_nrm.as<float4>(), s);
+                 d_raw.as<float4>(), d_cloud_off.as<int64_t>(), d_nbr.as<int>(), d_nsel.as<int>(),
+                 d_defer.as<int>(), d_nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
   LCR_HIP(hipStreamSynchronize(s));

@@ -121,7 +121,7 @@ struct CloudSet {
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
   DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
-  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_mom, d_nsel, d_defer;
+  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel, d_defer;
   DevBuf d_tpts, d_tpos, d_tkeys, d_tkeys_sorted, d_tvals, d_ttmp;  // tree order
   DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
