@@ -21,14 +21,16 @@ With several hypotheses per pair (the learned one, or C5's 15) each pair's
 initial poses are scored by their inliers within r and only the best is refined
 (`--no-gate` refines them all and keeps the best by final fitness).
 
-A step = normals + grids + all ICP iterations (one hipGraph) + fitness + selection
-for the whole cycle, with the clouds already resident in HBM.  `value` counts the
+The launch uses the library's defaults (tile 64, cell size 2.5 point spacings);
+`--autotune` lets the autotuner pick tile, cell size and search precision on this
+device first.  A step = normals + grids + all ICP iterations (one hipGraph) + fitness
++ selection for the whole cycle, with the clouds already resident in HBM.  `value` counts the
 pairs whose selected transform is within 0.05 deg / 0.2 mm of the FP64 oracle's
 (run from the prior), divided by the step time.
 
 Also reported: the plain single-pair GPU path (one pair at a time through the
 per-pair API: normals + upload + align + fitness), every correspondence-search
-precision mode on the same cycle, the dominant kernel's roofline from HIP events
+precision mode on the same cycle (tile 64, checked bitwise against the FP32 scan), the dominant kernel's roofline from HIP events
 recorded around each launch, and a host CPU baseline (the PCL-semantics FP32
 restatement with an exact kd-tree, one thread, on a bounded sample of pairs).
 """
@@ -82,6 +84,12 @@ def cpu_reference_and_baseline(cyc, sample_pairs, threads):
     refs = oracle.icp_batch(cyc.clouds, normals, cyc.pairs, guesses=cyc.meta["prior"],
                             nthreads=threads, point_to_plane=True, mode=oracle.FP64, **icp_kw())
     log(f"oracle FP64 references: {time.time() - t0:.1f} s")
+    # all host cores: the whole cycle, pairs spread over `threads` threads
+    t2 = time.perf_counter()
+    tn_all = [oracle.normals(c, 30, nthreads=threads)[0] for c in cyc.clouds]
+    oracle.icp_batch(cyc.clouds, tn_all, cyc.pairs, guesses=cyc.meta["prior"], nthreads=threads,
+                     point_to_plane=True, mode=oracle.PCL32, **icp_kw())
+    all_cores_s = time.perf_counter() - t2
     times = []
     for i in range(sample_pairs):
         s, t = cyc.pairs[i]
@@ -90,7 +98,7 @@ def cpu_reference_and_baseline(cyc, sample_pairs, threads):
         oracle.icp(cyc.clouds[s], cyc.clouds[t], tn, guess=cyc.meta["prior"][i], point_to_plane=True,
                    mode=oracle.PCL32, **icp_kw())
         times.append(time.perf_counter() - t1)
-    return refs, float(np.median(times))
+    return refs, float(np.median(times)), len(cyc.pairs) / all_cores_s
 
 
 def aligned_mask(T, refs):
@@ -151,7 +159,9 @@ def main():
                     help="refine every hypothesis of a pair instead of only the one with the "
                          "most inliers at its initial pose")
     ap.add_argument("--autotune", action="store_true",
-                    help="pick tile / cell size / precision with the autotuner first")
+                    help="let the autotuner pick tile / cell size / precision first (its choice "
+                         "varies between near-equal settings from run to run; the default "
+                         "measures the library's defaults, reproducibly)")
     ap.add_argument("--compare-precisions", type=int, default=3, metavar="STEPS",
                     help="also time every precision mode for STEPS steps (0 = skip)")
     ap.add_argument("--cpu-sample", type=int, default=3)
@@ -187,23 +197,28 @@ def main():
     elif args.proposal not in ("auto", "none"):
         raise SystemExit(f"proposal weights not found: {wpath}")
 
-    def make_cycle(mode, with_prop=True):
+    def make_cycle(mode, with_prop=True, tile=0, cell_factor=0.0):
         p = prop if with_prop else None
         gate = not args.no_gate and (n_hyp > 1 or p is not None)
         return lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses, n_hyp=n_hyp, point_to_plane=True,
-                         normals_k=30, precision_mode=mode, proposal=p, hyp_gate=gate, **icp_kw())
+                         normals_k=30, precision_mode=mode, proposal=p, hyp_gate=gate,
+                         tile=tile, cell_factor=cell_factor, **icp_kw())
 
     # ---------------- batched cycle (the measured path); creating it copies the
     # clouds host -> device once (the PCIe-inclusive rate below adds that time)
     t_up = time.perf_counter()
     cycle = make_cycle(PRECISION[args.precision])
     upload_s = time.perf_counter() - t_up
+    # the autotuner times the cycle under its 30 settings (tile, cell size,
+    # search precision) and keeps the fastest; its cache lives in gpurun_out/
     tune = None
+    tuned_mode = PRECISION[args.precision]
     if args.autotune:
         t_t = time.time()
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         tune = cycle.autotune(os.path.join(ROOT, "gpurun_out", "autotune.txt"))
         tune["seconds"] = time.time() - t_t
+        tuned_mode = int(tune["precision_mode"])
         log(f"autotune: {tune}")
     for _ in range(args.warmup):
         cycle.run()
@@ -234,7 +249,9 @@ def main():
     # precision mode (their transforms must agree bitwise with the measured run)
     no_prop = None
     if prop is not None:
-        c0 = make_cycle(PRECISION[args.precision], with_prop=False)
+        c0 = (make_cycle(tuned_mode, with_prop=False, tile=tune["tile"],
+                         cell_factor=tune["cell_factor"])
+              if tune else make_cycle(PRECISION[args.precision], with_prop=False))
         c0.run()
         r0 = c0.results()
         no_prop = {"ms_per_step": timed_steps(c0, max(1, args.steps // 2)) * 1e3,
@@ -242,20 +259,28 @@ def main():
         no_prop_T = r0["T"]
         log(f"without the learned hypothesis: {no_prop['ms_per_step']:.3f} ms/step")
         del c0
+    # every search mode at tile 64 (the distance-tile modes need one wavefront per
+    # tile) and the measured run's cell size: same reduction order, so the
+    # transforms must agree bitwise with the fp32 scan's
     modes = {}
-    if args.compare_precisions > 0 and tune is None:
-        for name, mode in sorted(PRECISION.items()):
-            cm = make_cycle(mode)
+    if args.compare_precisions > 0:
+        cf = tune["cell_factor"] if tune else 0.0
+        ref_T = None
+        for name, mode in sorted(PRECISION.items(), key=lambda kv: kv[1]):
+            cm = make_cycle(mode, tile=64, cell_factor=cf)
             cm.run()
             lcr.prof_reset()
             lcr.prof_enable(True)
             ms = timed_steps(cm, args.compare_precisions) * 1e3
             lcr.prof_enable(False)
             _, it_ms = lcr.prof_read("icp_iteration")
-            same = bool(np.array_equal(cm.results()["T"], res["T"]))
+            T = cm.results()["T"]
+            if ref_T is None:
+                ref_T = T  # fp32 (mode 0) comes first
+            same = bool(np.array_equal(T, ref_T))
             modes[name] = {"profiled_ms_per_step": ms,
                            "icp_iteration_ms_per_step": it_ms / args.compare_precisions,
-                           "same_transforms_as_measured_run": same}
+                           "same_transforms_as_fp32_scan": same}
             log(f"precision {name}: profiled step {ms:.3f} ms, icp "
                 f"{modes[name]['icp_iteration_ms_per_step']:.3f} ms, identical results: {same}")
             del cm
@@ -297,7 +322,7 @@ def main():
     # ---------------- checker and host baseline
     cpu = None
     if not args.no_cpu:
-        refs, cpu_per_pair = cpu_reference_and_baseline(cyc, args.cpu_sample, args.threads)
+        refs, cpu_per_pair, cpu_all = cpu_reference_and_baseline(cyc, args.cpu_sample, args.threads)
         ok, errs = aligned_mask(res["T"], refs)
         ok_plain = aligned_mask(plain_T, refs[:n_plain])[0] if n_plain else np.zeros(0, bool)
         if no_prop is not None:
@@ -305,7 +330,9 @@ def main():
         cpu = {"value": 1.0 / cpu_per_pair, "unit": "pairs/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_sample} pairs of the same cycle: 30-NN normals of the target "
                          f"+ point-to-plane ICP from the prior, PCL-semantics FP32 restatement "
-                         f"with an exact kd-tree (not PCL), median per pair"}
+                         f"with an exact kd-tree (not PCL), median per pair",
+               "all_cores": {"value": cpu_all, "unit": "pairs/s", "cores": args.threads,
+                             "sample": "the whole cycle, pairs spread over the threads"}}
     else:
         ok = np.ones(len(cyc.pairs), bool)
         ok_plain = np.ones(n_plain, bool)
@@ -344,7 +371,9 @@ def main():
             "hypothesis_gate": (not args.no_gate and (n_hyp > 1 or prop is not None)),
             "max_iterations": 60, "max_corr_dist_m": 0.01, "normals_k": 30, "seed": args.seed,
             "initial_pose": "turntable prior (rig yaw step about its axis)",
-            "correspondence_search": args.precision,
+            "correspondence_search": {v: k for k, v in PRECISION.items()}[tuned_mode],
+            "launch": ({"tile": tune["tile"], "cell_factor": tune["cell_factor"], "tuned": True}
+                       if tune else {"tile": "default", "cell_factor": "default", "tuned": False}),
             "parallelism": "single GPU",
         },
         "aligned_pairs": n_aligned,

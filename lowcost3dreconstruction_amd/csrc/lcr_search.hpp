@@ -395,12 +395,6 @@ __device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const Tree& tr, b
 
 // ------------------------------------------------------------------ reductions
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 // v + (v moved by the DPP pattern `ctrl` inside each row of 16 lanes), in FP64.
 template <int ctrl>
 __device__ __forceinline__ double dpp_add(double v) {
@@ -425,13 +419,14 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return (r[0] + r[1]) + (r[2] + r[3]);
 }
 
-// Block-wide sum of N doubles per thread; the totals are valid in thread 0.
-// `lds` must hold N * (blockDim.x / 64) doubles.
+// Block-wide sum of N doubles per thread (every thread of the block calls it);
+// the totals are valid in thread 0.  `lds` must hold N * (blockDim.x / 64)
+// doubles.
 template <int N>
 __device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+  for (int i = 0; i < N; ++i) v[i] = wave_sum_dpp(v[i]);
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < N; ++i) lds[wid * N + i] = v[i];

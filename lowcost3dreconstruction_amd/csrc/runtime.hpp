@@ -117,6 +117,11 @@ struct CloudSet {
   bool has_nrm = false;
   std::vector<GridDesc> grids;
   int64_t total_pts = 0, total_cells = 0, total_nodes = 0;
+  // per cloud: bounding box + finite count (7 floats) and occupied probe cells,
+  // measured by the first build after an upload
+  std::vector<float> geom_box;
+  std::vector<int> geom_occ;
+  bool geom_valid = false;
   int max_depth = 0;
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
