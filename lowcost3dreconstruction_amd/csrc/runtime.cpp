@@ -1,6 +1,9 @@
 // runtime.cpp -- host runtime of liblcr (see runtime.hpp).
 #include "runtime.hpp"
 
+#include <atomic>
+#include <thread>
+
 #include <float.h>
 #include <math.h>
 #include <string.h>
@@ -114,6 +117,10 @@ constexpr int64_t kMaxDim = 4096;
 
 }  // namespace
 
+// Host threads staging an upload, and the points below which one thread does it.
+constexpr int kStageThreads = 8;
+constexpr int64_t kStageMinPoints = 1 << 18;
+
 void CloudSet::upload(hipStream_t s, const std::vector<const float*>& xyz,
                       const std::vector<const float*>& nrm, const std::vector<int64_t>& counts,
                       int64_t stride_bytes) {
@@ -133,7 +140,8 @@ void CloudSet::upload(hipStream_t s, const std::vector<const float*>& xyz,
   h_stage_nrm.reserve(np * (sizeof(float4) + sizeof(int)));
   float4* sn = h_stage_nrm.as<float4>();
   pc = reinterpret_cast<int*>(sn + np);
-  for (int c = 0; c < n; ++c) {
+  // host staging into pinned memory, clouds spread over a few threads
+  auto stage = [&](int c) {
     const char* base = reinterpret_cast<const char*>(xyz[c]);
     const char* nb = reinterpret_cast<const char*>(nrm[c]);
     for (int64_t i = 0; i < counts[c]; ++i) {
@@ -145,6 +153,19 @@ void CloudSet::upload(hipStream_t s, const std::vector<const float*>& xyz,
         sn[raw_off[c] + i] = make_float4(m[0], m[1], m[2], 0.0f);
       }
     }
+  };
+  const int nthreads = (int)std::min<int64_t>(
+      {(int64_t)kStageThreads, (int64_t)n, std::max<int64_t>(total_raw / kStageMinPoints, 1)});
+  if (nthreads <= 1) {
+    for (int c = 0; c < n; ++c) stage(c);
+  } else {
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&] {
+        for (int c = next++; c < n; c = next++) stage(c);
+      });
+    for (std::thread& th : pool) th.join();
   }
   d_raw.reserve(np * sizeof(float4));
   d_point_cloud.reserve(np * sizeof(int));
