@@ -199,8 +199,18 @@ struct BuildRegions {
 };
 }  // namespace
 
+CloudSet::~CloudSet() {
+  if (ev_build) (void)hipEventDestroy(ev_build);
+  if (ev_vp) (void)hipEventDestroy(ev_vp);
+}
+
 void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans,
                      bool with_tree) {
+  // the previous build's copies out of the pinned region must be done
+  if (ev_build)
+    LCR_HIP(hipEventSynchronize(ev_build));
+  else
+    LCR_HIP(hipEventCreateWithFlags(&ev_build, hipEventDisableTiming));
   Profiler& prof = Profiler::get();
   if (factor <= 0.0) factor = 2.5;
   const BuildRegions R(n);
@@ -352,7 +362,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   // tree order, then the bounding-volume tree over it
   if (!with_tree) {
     prof.end(span, s, spans);
-    LCR_HIP(hipStreamSynchronize(s));  // the pinned regions are reused by the next build
+    LCR_HIP(hipEventRecord(ev_build, s));  // the pinned regions are reused by the next build
     return;
   }
   d_tpts.reserve(npts * sizeof(float4));
@@ -360,7 +370,12 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   d_tkeys.reserve(npts * sizeof(uint64_t));
   d_tkeys_sorted.reserve(npts * sizeof(uint64_t));
   d_tvals.reserve(npts * sizeof(int));
-  const size_t tmp_bytes = tree_order_tmp_bytes(total_pts, n);
+  if (ttmp_pts != total_pts || ttmp_n != n) {  // the size query is slow: once per shape
+    ttmp_bytes = tree_order_tmp_bytes(total_pts, n);
+    ttmp_pts = total_pts;
+    ttmp_n = n;
+  }
+  const size_t tmp_bytes = ttmp_bytes;
   d_ttmp.reserve(std::max<size_t>(tmp_bytes, 16));
   LCR_HIP(launch_tree_order(d_grids.as<GridDesc>(), n, d_pts.as<float4>(), d_cloud_of_sorted.as<int>(),
                             total_pts, d_tkeys.as<uint64_t>(), d_tkeys_sorted.as<uint64_t>(),
@@ -369,18 +384,24 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   d_nodes.reserve((size_t)std::max<int64_t>(total_nodes, 1) * 2 * sizeof(float4));
   launch_bvh_build(d_grids.as<GridDesc>(), n, max_depth, d_tpts.as<float4>(), d_nodes.as<float4>(), s);
   prof.end(span, s, spans);
-  LCR_HIP(hipStreamSynchronize(s));  // the pinned regions are reused by the next build
+  LCR_HIP(hipEventRecord(ev_build, s));  // the pinned regions are reused by the next build
 }
 
 void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
                                std::vector<Profiler::Span>* spans) {
   Profiler& prof = Profiler::get();
+  // the previous pass's copy out of h_vp must be done before it is rewritten
+  if (ev_vp)
+    LCR_HIP(hipEventSynchronize(ev_vp));
+  else
+    LCR_HIP(hipEventCreateWithFlags(&ev_vp, hipEventDisableTiming));
   d_vp.reserve((size_t)n * 3 * sizeof(float));
   h_vp.reserve((size_t)n * 3 * sizeof(float));
   float* v = h_vp.as<float>();
   for (int c = 0; c < n; ++c)
     for (int a = 0; a < 3; ++a) v[3 * c + a] = vp_host ? vp_host[3 * c + a] : 0.0f;
   LCR_HIP(hipMemcpyAsync(d_vp.p, v, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  LCR_HIP(hipEventRecord(ev_vp, s));
   const int span = prof.begin("normals", s, spans);
   const size_t np = (size_t)std::max<int64_t>(total_pts, 1);
   d_nbr.reserve(np * (size_t)k * sizeof(int));
@@ -392,7 +413,6 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
                  d_defer.as<int>(), d_nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
-  LCR_HIP(hipStreamSynchronize(s));
 }
 
 void CloudSet::build_near_mask(hipStream_t s, double r) {

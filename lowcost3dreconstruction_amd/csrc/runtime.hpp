@@ -122,6 +122,18 @@ struct CloudSet {
   std::vector<float> geom_box;
   std::vector<int> geom_occ;
   bool geom_valid = false;
+  // recorded after the last copy out of the pinned build region / viewpoint
+  // buffer: the next build or normals pass waits on it before rewriting them
+  // (instead of synchronising the stream at the end of each)
+  hipEvent_t ev_build = nullptr, ev_vp = nullptr;
+  // radix-sort scratch size for (tree order points, clouds)
+  int64_t ttmp_pts = -1;
+  int ttmp_n = -1;
+  size_t ttmp_bytes = 0;
+  CloudSet() = default;
+  CloudSet(const CloudSet&) = delete;
+  CloudSet& operator=(const CloudSet&) = delete;
+  ~CloudSet();
   int max_depth = 0;
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
