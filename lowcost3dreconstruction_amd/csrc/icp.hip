@@ -197,8 +197,11 @@ __global__ void __launch_bounds__(TILE) k_icp_iter(const IcpBuffers b, const Icp
       nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, bounded, seed,
                     2, true);
     const float rlim = r * 1.00001f;
-    bvh_nn_wave(gt, b.tree(), nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
-                nn.id, nn.j);
+    // tree walks happen only for radii of more than kRingOnlyCells cells: kept out
+    // of the hot path's code layout
+    if (__builtin_expect(__ballot(nn.tree) != 0, 0))
+      bvh_nn_wave(gt, b.tree(), nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
+                  nn.id, nn.j);
     bool has = false;
     float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
     if (own) {

@@ -341,56 +341,163 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   return res;
 }
 
+// A read-only array seen through the constant address space: a load from it
+// whose address is the same in every lane of the wavefront is served by the
+// scalar cache, one request per wavefront instead of one per lane.
+// (The host pass only parses the device functions that use it.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const float4 __attribute__((address_space(4)))* UniformF4;
+#else
+typedef const float4* UniformF4;
+#endif
+
+// Tree levels one step of the wavefront walk descends: the 2^e descendants of a
+// node e levels down are contiguous in the implicit tree, so one load brings all
+// their boxes (4 boxes, 128 bytes).  Three levels (8 boxes) measured slower.
+constexpr int kWalkLevels = 2;
+constexpr int kWalkFan = 1 << kWalkLevels;
+static_assert((kWalkFan - 1) * (27 / kWalkLevels) + kWalkFan <= 64, "walk stack fits a register");
+
+// The set bit of `mask` whose key is smallest (nearest) or largest, lowest bit on
+// ties.  Keys and mask are wave-uniform.
+__device__ __forceinline__ int walk_pick(uint32_t mask, const float (&key)[kWalkFan], bool nearest) {
+  int sel = -1;
+  float kb = 0.0f;
+#pragma unroll
+  for (int c = 0; c < kWalkFan; ++c) {
+    const bool better = nearest ? key[c] < kb : key[c] > kb;
+    if (((mask >> c) & 1u) && (sel < 0 || better)) {
+      sel = c;
+      kb = key[c];
+    }
+  }
+  return sel;
+}
+
+// Points of leaf node `leaf` against every active lane's query.  A full leaf has
+// all of its loads in flight at once, one latency per leaf; selects rather than a
+// branch on `active` keep the compiler from sinking the loads into it.
+__device__ __forceinline__ void walk_leaf(const GridDesc& g, const Tree& tr, UniformF4 TP,
+                                          unsigned leaf, bool active, float qx, float qy,
+                                          float qz, float& best, int& bid, int& tj) {
+  const int end = (int)g.pt_off + g.npts;
+  const int s = (int)g.pt_off + (int)(leaf - (1u << g.depth)) * kLeaf;
+  if (s + kLeaf <= end) {
+    float4 p[kLeaf];
+#pragma unroll
+    for (int j = 0; j < kLeaf; ++j) p[j] = TP[s + j];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < kLeaf; ++j) {
+      const float d = flann_d2(qx, qy, qz, p[j]);
+      const int id = f2i(p[j].w);
+      const bool take = active && (d < best || (d == best && id < bid));
+      best = take ? d : best;
+      bid = take ? id : bid;
+      tj = take ? s + j : tj;
+    }
+  } else if (active && s < end) {
+    scan_points(tr.P, s, end, qx, qy, qz, best, bid, tj);
+  }
+}
+
 // bvh_nn for the lanes of a wavefront at once (`active` lanes; every lane of the
-// wavefront must call it).  The traversal is shared: a subtree is entered when any
-// active lane may hold a closer point in it, and near-first order follows the
-// majority of the active lanes.  The node index is wave-uniform, so node boxes
-// and leaf points are read once per wavefront.  Queries of one wavefront are
-// neighbouring cell-sorted source points, so their paths mostly coincide.
+// wavefront must call it).  The traversal is shared and the node index is
+// wave-uniform, so node boxes and leaf points are read once per wavefront through
+// the scalar cache.  Queries of one wavefront are neighbouring cell-sorted source
+// points, so their paths mostly coincide.
+//
+// Each step expands a node kWalkLevels levels at once (the root by depth mod
+// kWalkLevels, so that every later step ends exactly on the leaves) and keeps the
+// descendants whose box some active lane may find a closer point in (bound with
+// the same relative margin as bvh_nn).  Leaves are then scanned nearest first,
+// each tested again against the tightened bounds from registers; inner nodes are
+// pushed farthest first on a stack that holds one entry per lane of a register,
+// the nearest is entered at once, and a popped node is tested again before it is
+// expanded.  "Nearest" is by the first active lane's box distance.  A step costs
+// one load latency where a binary walk spends one per level and another per
+// sibling.  The stack never holds more than kWalkFan - 1 entries per inner step
+// along one path: at most 3 * 13 + 1 < 64 for the 27 levels of a tree over 2^31
+// points.
 __device__ __forceinline__ void bvh_nn_wave(const GridDesc& g, const Tree& tr, bool active,
                                             float qx, float qy, float qz, float limit2,
                                             float& best, int& bid, int& bj) {
   const uint64_t live = __ballot(active);
   if (live == 0) return;
-  const int nlive = __popcll(live);
-  const float4* N = tr.nodes + 2 * g.node_off;
+  const int leader = __ffsll((unsigned long long)live) - 1;
+  const UniformF4 N = (UniformF4)(tr.nodes + 2 * g.node_off);
+  const UniformF4 TP = (UniformF4)tr.P;
+  const int D = g.depth;
   int tj = -1;  // tree-order index of the best point once a leaf improves it
-  const unsigned first_leaf = 1u << g.depth;
-  const int end = (int)g.pt_off + g.npts;
-  unsigned i = 1, order = 0;
-  int depth = 0;
+  const int lane = (int)__lane_id();
+  int stack = 0, sp = 0;
+  unsigned i = 1;
+  bool fresh = false;  // i passed its test under the current bounds
   for (;;) {
-    const float bound = fminf(best, limit2) * 1.00002f;
-    bool descend = __ballot(active && box_d2(N[2 * i], N[2 * i + 1], qx, qy, qz) <= bound) != 0;
-    if (descend && i >= first_leaf) {
-      const int s = (int)g.pt_off + (int)(i - first_leaf) * kLeaf;
-      if (active) scan_points(tr.P, s, min(s + kLeaf, end), qx, qy, qz, best, bid, tj);
-      descend = false;
+    if (!fresh) {
+      const float bound = fminf(best, limit2) * 1.00002f;
+      if (__ballot(active && box_d2(N[2 * i], N[2 * i + 1], qx, qy, qz) <= bound) == 0) {
+        if (sp == 0) break;
+        i = (unsigned)__builtin_amdgcn_readlane(stack, --sp);
+        continue;
+      }
     }
-    if (descend) {
-      const unsigned l = 2 * i;
-      const float dl = box_d2(N[2 * l], N[2 * l + 1], qx, qy, qz);
-      const float dr = box_d2(N[2 * l + 2], N[2 * l + 3], qx, qy, qz);
-      const bool right_first = 2 * __popcll(__ballot(active && dr < dl)) > nlive;
-      ++depth;
-      order = right_first ? (order | (1u << depth)) : (order & ~(1u << depth));
-      i = right_first ? l + 1 : l;
+    const int di = 31 - __clz((int)i);
+    if (di == D) {  // the root of a one-leaf tree
+      walk_leaf(g, tr, TP, i, active, qx, qy, qz, best, bid, tj);
+      break;
+    }
+    const int e = (di == 0 && D % kWalkLevels) ? D % kWalkLevels : kWalkLevels;
+    const unsigned c0 = i << e;
+    const int nc = 1 << e;
+    float dc[kWalkFan], key[kWalkFan];
+    if (e == kWalkLevels) {
+      float4 bx[2 * kWalkFan];
+#pragma unroll
+      for (int c = 0; c < 2 * kWalkFan; ++c) bx[c] = N[2 * c0 + c];
+#pragma unroll
+      for (int c = 0; c < kWalkFan; ++c) dc[c] = box_d2(bx[2 * c], bx[2 * c + 1], qx, qy, qz);
+    } else {
+#pragma unroll
+      for (int c = 0; c < kWalkFan; ++c)
+        dc[c] = c < nc ? box_d2(N[2 * (c0 + c)], N[2 * (c0 + c) + 1], qx, qy, qz) : INFINITY;
+    }
+    const float bound = fminf(best, limit2) * 1.00002f;
+    uint32_t pass = 0;
+#pragma unroll
+    for (int c = 0; c < kWalkFan; ++c) {
+      key[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dc[c]), leader));
+      if (c < nc && __ballot(active && dc[c] <= bound) != 0) pass |= 1u << c;
+    }
+    if (di + e == D) {
+      while (pass) {
+        const int c = walk_pick(pass, key, true);
+        pass &= ~(1u << c);
+        float d = dc[0];
+#pragma unroll
+        for (int k = 1; k < kWalkFan; ++k) d = (c == k) ? dc[k] : d;
+        const float bnd = fminf(best, limit2) * 1.00002f;
+        if (__ballot(active && d <= bnd) != 0)
+          walk_leaf(g, tr, TP, c0 + (unsigned)c, active, qx, qy, qz, best, bid, tj);
+      }
+    } else if (pass) {
+      const int cn = walk_pick(pass, key, true);
+      pass &= ~(1u << cn);
+      while (pass) {
+        const int c = walk_pick(pass, key, false);
+        pass &= ~(1u << c);
+        stack = (lane == sp) ? (int)(c0 + (unsigned)c) : stack;
+        ++sp;
+      }
+      i = c0 + (unsigned)cn;
+      fresh = true;
       continue;
     }
-    for (;;) {
-      if (depth == 0) {
-        if (tj >= 0) bj = tr.pos[tj];
-        return;
-      }
-      const unsigned right_first = (order >> depth) & 1u;
-      if ((i & 1u) == right_first) {
-        i ^= 1u;
-        break;
-      }
-      i >>= 1;
-      --depth;
-    }
+    if (sp == 0) break;
+    i = (unsigned)__builtin_amdgcn_readlane(stack, --sp);
+    fresh = false;
   }
+  if (tj >= 0) bj = tr.pos[tj];
 }
 
 // ------------------------------------------------------------------ reductions

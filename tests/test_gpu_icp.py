@@ -177,6 +177,29 @@ def test_fitness_max_range(lcr, c1):
     assert icp.getFitnessScore(-1.0) == np.finfo(np.float64).max  # no d2 <= -1: DBL_MAX
 
 
+@pytest.mark.parametrize("n_tgt", [5, 20, 40, 100, 20000])
+def test_fitness_far_queries_match_oracle(lcr, n_tgt):
+    """Source points far outside the target (half of a cloud as the target, the
+    source moved up to 25 object sizes away) leave the ring search for the shared
+    wavefront tree walk of the fitness pass.  Trees of one to several thousand
+    leaves (1-leaf and partial-root-step cases included), every tile size and two
+    cell sizes must give the oracle's exact nearest-neighbour fitness.  A radius of
+    1e-12 finds no correspondence, so the fitness is taken at the guess itself."""
+    base = synth.pair_ellipsoids(5, n_src=6000, n_tgt=max(n_tgt, 2) * 2)
+    tgt = base.tgt[np.argsort(base.tgt[:, 0], kind="stable")[:n_tgt]]
+    for shift in (0.0, 0.05, 0.5, 5.0):
+        guess = synth.rigid(synth.rot_axis_angle([0, 0, 1], 0.3), [shift, 0.3 * shift, -shift])
+        ref = oracle.fitness(base.src, tgt, guess)
+        for tile in (64, 128, 256):
+            for cf in (1.5, 4.0):
+                print_case(test="fitness_far", n_tgt=n_tgt, shift=shift, tile=tile, cf=cf)
+                icp = make_icp(lcr, base.src, tgt, r=1e-12, iters=5, tile=tile, cell_factor=cf)
+                icp.align(guess)
+                assert icp.nr_iterations() == 0
+                f = icp.getFitnessScore()
+                assert abs(f - ref) <= 1e-12 * ref, (shift, tile, cf, f, ref)
+
+
 def test_tiles_and_cell_factors_agree(lcr, c1):
     """Tile size and cell size change the launch shape, never the answer."""
     print_case(test="shapes")
