@@ -270,8 +270,10 @@ __global__ void __launch_bounds__(kSolveThreads) k_icp_solve(const IcpBuffers b,
 
 // per_instance: each instance's own max_corr_d2 is the range.  A finite range
 // bounds the search at sqrt(range): only points within it are counted.
+// Six waves per SIMD (80 VGPRs, a few bytes spilled) measured 3 % faster than
+// the unconstrained five; eight (64 VGPRs) spilled in the walk and was slower.
 template <int TILE>
-__global__ void __launch_bounds__(TILE) k_fitness(const IcpBuffers b, double max_range_arg,
+__global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double max_range_arg,
                                                   int per_instance, int interleave) {
   __shared__ double lds[2 * (TILE / 64)];
   __shared__ int s_last;
@@ -617,6 +619,24 @@ int icp_max_blocks_per_cu(int tile, bool p2pl, int precision) {
     default:
       e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 256, 0>, 256, 0)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<false, 256, 0>, 256, 0);
+      break;
+  }
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+int fitness_max_blocks_per_cu(int tile) {
+  int nb = 0;
+  hipError_t e;
+  switch (tile) {
+    case 64:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fitness<64>, 64, 0);
+      break;
+    case 128:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fitness<128>, 128, 0);
+      break;
+    default:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fitness<256>, 256, 0);
       break;
   }
   if (e != hipSuccess || nb < 1) nb = 1;

@@ -98,6 +98,8 @@ void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n
 void launch_transform_points(const float4* raw, const float4* raw_nrm, int64_t n, const double* T,
                              float* out_xyz, float* out_nrm, hipStream_t s);
 int icp_max_blocks_per_cu(int tile, bool p2pl, int precision);
+// Resident blocks per CU of the fitness pass at `tile` points per block.
+int fitness_max_blocks_per_cu(int tile);
 
 // ---- K7 normals (normals.hip)
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,

@@ -492,6 +492,8 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   int nb = std::min(std::max(ntiles, 1), cus * per_cu);
   nb = std::max(8, (nb + 7) / 8 * 8);
   nblocks = nb;
+  int nf = std::min(std::max(ntiles, 1), cus * fitness_max_blocks_per_cu(tile));
+  nblocks_fit = std::max(8, (nf + 7) / 8 * 8);
 }
 
 IcpBuffers Engine::buffers(const CloudSet& cs) const {
@@ -562,7 +564,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   std::vector<Profiler::Span> spans;
   if (gate_h > 1 && ntiles > 0) {
     const int k = prof.begin("gate", s, &spans);
-    launch_fitness(b, 0.0, nblocks, s, true, c.interleave);
+    launch_fitness(b, 0.0, nblocks_fit, s, true, c.interleave);
     launch_gate(d_state.as<PairState>(), npairs / gate_h, gate_h, s);
     prof.end(k, s, &spans);
   }
@@ -583,7 +585,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
     const int k = prof.begin("fitness", s, sp);
     // one launch over every instance: XCD-contiguous tiles keep each target's
     // tree in one L2 (measured faster here; the iterations keep their order)
-    launch_fitness(b, DBL_MAX, nblocks, s, false, 0);
+    launch_fitness(b, DBL_MAX, nblocks_fit, s, false, 0);
     prof.end(k, s, sp);
     LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),
                            hipMemcpyDeviceToHost, s));
@@ -641,6 +643,7 @@ Engine::GraphKey Engine::graph_key(const IcpBuffers& b) const {
   k.b = b;
   k.c = c;
   k.nblocks = nblocks;
+  k.nblocks_fit = nblocks_fit;
   k.guess = d_guess.p;
   k.hstate = h_state.p;
   k.active = d_active.p;
@@ -655,7 +658,7 @@ void Engine::run_fitness(hipStream_t s, const CloudSet& cs, double max_range) {
   const IcpBuffers b = buffers(cs);
   if (ntiles > 0) {
     const int sp = prof.begin("fitness", s, &spans);
-    launch_fitness(b, max_range, nblocks, s, false, c.interleave);
+    launch_fitness(b, max_range, nblocks_fit, s, false, c.interleave);
     prof.end(sp, s, &spans);
   }
   LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),

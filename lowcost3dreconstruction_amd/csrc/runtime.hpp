@@ -168,6 +168,7 @@ struct CloudSet {
 // A batch of ICP instances over a CloudSet.
 struct Engine {
   int npairs = 0, ntiles = 0, tile = 256, nblocks = 8;
+  int nblocks_fit = 8;  // persistent blocks of the fitness passes (their own occupancy)
   IcpConst c{};
   std::vector<PairDesc> hp;
   DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess, d_prev;
@@ -180,7 +181,7 @@ struct Engine {
   struct GraphKey {
     IcpBuffers b{};
     IcpConst c{};
-    int nblocks = 0;
+    int nblocks = 0, nblocks_fit = 0;
     const void* guess = nullptr;
     const void* hstate = nullptr;
     const void* active = nullptr;
