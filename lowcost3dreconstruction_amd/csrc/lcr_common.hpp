@@ -97,6 +97,10 @@ struct IcpConst {
 // but at least kGateMinTiles tiles where the source has them.
 constexpr int kGateStride = 32;
 constexpr int kGateMinTiles = 64;
+__host__ __device__ inline int gate_stride(int ntiles) {
+  const int s = ntiles / kGateMinTiles;
+  return s < 1 ? 1 : (s > kGateStride ? kGateStride : s);
+}
 
 // Flat FP32 weight layout of the pose-proposal network, row-major [in][out]:
 // W1 3x64, b1, W2 64x128, b2, W3 128x256, b3, Hw1 512x256, hb1, Hw2 256x6, hb2.

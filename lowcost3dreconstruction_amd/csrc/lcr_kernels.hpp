@@ -62,6 +62,9 @@ struct IcpBuffers {
   int* prev;              // ntiles * tile: last match (sorted target position) per source
                           // point, -1 = none; seeds the next search (nullable)
   const uint8_t* near;    // per target cell: some point within GridDesc::near_r (nullable)
+  const int* gate_tiles;  // the tiles the gate's pass scores (every gate_stride-th of an
+                          // instance), in tile order (nullable: all, skipping the others)
+  int ngate_tiles;
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)

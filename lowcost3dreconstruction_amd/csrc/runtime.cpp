@@ -462,7 +462,7 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   tile = (tile_pts == 64 || tile_pts == 128 || tile_pts == 256) ? tile_pts : 256;
   npairs = (int)src.size();
   hp.assign(npairs, PairDesc{});
-  std::vector<int> tp;
+  std::vector<int> tp, gt;
   ntiles = 0;
   for (int p = 0; p < npairs; ++p) {
     PairDesc& d = hp[p];
@@ -472,6 +472,7 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
     d.ntiles = (cs.grids[d.src].npts + tile - 1) / tile;
     d.max_corr_d2 = r2[p];
     for (int t = 0; t < d.ntiles; ++t) tp.push_back(p);
+    for (int t = 0; t < d.ntiles; t += gate_stride(d.ntiles)) gt.push_back(ntiles + t);
     ntiles += d.ntiles;
   }
   d_pairs.reserve(std::max(npairs, 1) * sizeof(PairDesc));
@@ -486,6 +487,10 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
     LCR_HIP(hipMemcpy(d_pairs.p, hp.data(), npairs * sizeof(PairDesc), hipMemcpyHostToDevice));
   if (ntiles > 0)
     LCR_HIP(hipMemcpy(d_tile_pair.p, tp.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+  ngate_tiles = (int)gt.size();
+  d_gate_tiles.reserve(std::max<size_t>(gt.size(), 1) * sizeof(int));
+  if (ngate_tiles > 0)
+    LCR_HIP(hipMemcpy(d_gate_tiles.p, gt.data(), gt.size() * sizeof(int), hipMemcpyHostToDevice));
   const int cus = device_cu_count();
   const int per_cu = icp_max_blocks_per_cu(tile, c.point_to_plane != 0, c.precision);
   setup_precision = c.precision;
@@ -512,6 +517,8 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.partials = d_partials.as<double>();
   b.prev = d_prev.as<int>();
   b.near = cs.d_near.p ? cs.d_near.as<uint8_t>() : nullptr;
+  b.gate_tiles = d_gate_tiles.as<int>();
+  b.ngate_tiles = ngate_tiles;
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;

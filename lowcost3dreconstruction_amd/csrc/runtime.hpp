@@ -171,7 +171,8 @@ struct Engine {
   int nblocks_fit = 8;  // persistent blocks of the fitness passes (their own occupancy)
   IcpConst c{};
   std::vector<PairDesc> hp;
-  DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess, d_prev;
+  DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess, d_prev, d_gate_tiles;
+  int ngate_tiles = 0;
   HostBuf h_state, h_guess;
   static constexpr int kChunk = 4;  // iterations per captured chunk
   DevBuf d_active;
