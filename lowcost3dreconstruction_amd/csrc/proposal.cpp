@@ -48,22 +48,49 @@ __global__ void k_sum_loss(const float* __restrict__ part, int n, float* __restr
 
 // Inputs of the network for every pair of a cycle: n_out points subsampled evenly
 // from each raw cloud, the source moved by the pair's seed T0, both normalised by
-// the target sample's centroid and RMS radius.  One block per pair.
-__global__ void __launch_bounds__(256)
+// the target sample's centroid and RMS radius.  One block per pair; the sums are
+// folded by a fixed LDS tree, so the frame does not depend on timing.
+constexpr int kPrepThreads = 1024;
+
+// Sample i of n_out taken evenly from a cloud of n points: floor(i * n / n_out),
+// in 32-bit arithmetic when the product fits.
+__device__ __forceinline__ int64_t sample_at(int i, int64_t n, int n_out) {
+  if (n < ((int64_t)1 << 31) / n_out) return (int64_t)((unsigned)(i * (int)n) / (unsigned)n_out);
+  return (int64_t)i * n / n_out;
+}
+
+template <int N>
+__device__ __forceinline__ void prep_sum(double (&v)[N], double (*red)[N]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < N; ++k) red[t][k] = v[k];
+  __syncthreads();
+  for (int o = kPrepThreads / 2; o > 0; o >>= 1) {
+    if (t < o) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) red[t][k] += red[t + o][k];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = red[0][k];
+  __syncthreads();  // red is reused
+}
+
+__global__ void __launch_bounds__(kPrepThreads)
 k_prop_prepare(const float4* __restrict__ raw, const int64_t* __restrict__ cloud_off,
                const int* __restrict__ src, const int* __restrict__ tgt,
                const double* __restrict__ guesses, int guess_stride, int n_out,
                float4* __restrict__ X, float* __restrict__ frame) {
-  __shared__ double red[256][4];
-  __shared__ double cs[4];
+  __shared__ double red[kPrepThreads][4];
   const int pr = blockIdx.x, t = threadIdx.x;
   const int64_t s0 = cloud_off[src[pr]], ns = cloud_off[src[pr] + 1] - s0;
   const int64_t t0 = cloud_off[tgt[pr]], nt = cloud_off[tgt[pr] + 1] - t0;
   float4* Xs = X + (int64_t)pr * 2 * n_out;
   float4* Xt = Xs + n_out;
   double a[4] = {0, 0, 0, 0};
-  for (int i = t; i < n_out; i += 256) {
-    const float4 q = raw[t0 + (int64_t)i * nt / n_out];
+  for (int i = t; i < n_out; i += kPrepThreads) {
+    const float4 q = raw[t0 + sample_at(i, nt, n_out)];
     if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) {
       a[0] += q.x;
       a[1] += q.y;
@@ -71,48 +98,36 @@ k_prop_prepare(const float4* __restrict__ raw, const int64_t* __restrict__ cloud
       a[3] += 1.0;
     }
   }
-  for (int k = 0; k < 4; ++k) red[t][k] = a[k];
-  __syncthreads();
-  if (t == 0) {
-    double s[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 256; ++k)
-      for (int m = 0; m < 4; ++m) s[m] += red[k][m];
-    const double n = s[3] > 0 ? s[3] : 1.0;
-    for (int m = 0; m < 3; ++m) cs[m] = s[m] / n;
-  }
-  __syncthreads();
-  double ss = 0.0, cnt = 0.0;
-  for (int i = t; i < n_out; i += 256) {
-    const float4 q = raw[t0 + (int64_t)i * nt / n_out];
+  prep_sum<4>(a, red);
+  const double n = a[3] > 0 ? a[3] : 1.0;
+  const double cx = a[0] / n, cy = a[1] / n, cz = a[2] / n;
+  double b[4] = {0, 0, 0, 0};
+  for (int i = t; i < n_out; i += kPrepThreads) {
+    const float4 q = raw[t0 + sample_at(i, nt, n_out)];
     if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) {
-      const double dx = q.x - cs[0], dy = q.y - cs[1], dz = q.z - cs[2];
-      ss += dx * dx + dy * dy + dz * dz;
-      cnt += 1.0;
+      const double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+      b[0] += dx * dx + dy * dy + dz * dz;
+      b[1] += 1.0;
     }
   }
-  red[t][0] = ss;
-  red[t][1] = cnt;
-  __syncthreads();
+  prep_sum<4>(b, red);
+  const double rr = b[1] > 0 ? sqrt(b[0] / b[1]) : 1.0;
+  const double rad = rr > 1e-12 ? rr : 1.0;
   if (t == 0) {
-    double s = 0, c = 0;
-    for (int k = 0; k < 256; ++k) {
-      s += red[k][0];
-      c += red[k][1];
-    }
-    const double r = c > 0 ? sqrt(s / c) : 1.0;
-    cs[3] = r > 1e-12 ? r : 1.0;
-    for (int m = 0; m < 4; ++m) frame[pr * 4 + m] = (float)cs[m];
+    frame[pr * 4 + 0] = (float)cx;
+    frame[pr * 4 + 1] = (float)cy;
+    frame[pr * 4 + 2] = (float)cz;
+    frame[pr * 4 + 3] = (float)rad;
   }
-  __syncthreads();
   const double* T = guesses + (int64_t)pr * guess_stride * 16;
-  const float inv = (float)(1.0 / cs[3]);
-  const float c0 = (float)cs[0], c1 = (float)cs[1], c2 = (float)cs[2];
-  for (int i = t; i < n_out; i += 256) {
-    const float4 q = raw[t0 + (int64_t)i * nt / n_out];
+  const float inv = (float)(1.0 / rad);
+  const float c0 = (float)cx, c1 = (float)cy, c2 = (float)cz;
+  for (int i = t; i < n_out; i += kPrepThreads) {
+    const float4 q = raw[t0 + sample_at(i, nt, n_out)];
     Xt[i] = (isfinite(q.x) && isfinite(q.y) && isfinite(q.z))
                 ? make_float4((q.x - c0) * inv, (q.y - c1) * inv, (q.z - c2) * inv, 0.f)
                 : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 p = raw[s0 + (int64_t)i * ns / n_out];
+    const float4 p = raw[s0 + sample_at(i, ns, n_out)];
     if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
       float m[3];
       for (int r = 0; r < 3; ++r)
@@ -277,7 +292,7 @@ void proposal_guesses(ProposalImpl& P, hipStream_t s, const CloudSet& cs, const 
                       const int* d_tgt, int npairs, int npts, double* d_guess, int stride, int slot) {
   P.reserve(npairs, npts);
   P.frame.reserve((size_t)npairs * 4 * sizeof(float));
-  hipLaunchKernelGGL(k_prop_prepare, dim3(npairs), dim3(256), 0, s, cs.d_raw.as<float4>(),
+  hipLaunchKernelGGL(k_prop_prepare, dim3(npairs), dim3(kPrepThreads), 0, s, cs.d_raw.as<float4>(),
                      cs.d_cloud_off.as<int64_t>(), d_src, d_tgt, d_guess, stride, npts,
                      P.X.as<float4>(), P.frame.as<float>());
   launch_enc_forward(P.bufs(), npairs, npts, nullptr, s);
