@@ -162,9 +162,11 @@ __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const 
   static_assert(PREC == 0 || TILE == 64, "distance tiles need one wavefront per tile");
   static_assert(TILE >= NV, "one thread per value writes the tile's sums");
   __shared__ double lds[NV * (TILE / 64)];
-  const TileRange tr = xcd_tiles(b.ntiles, c.interleave);
-  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-    const int p = b.tile_pair[tile];
+  const int nt = b.it_count ? *b.it_count : b.ntiles;
+  const TileRange tr = xcd_tiles(nt, c.interleave);
+  for (int it = tr.first; it < tr.end; it += tr.step) {
+    const int2 e = b.it_list ? b.it_list[it] : make_int2(it, b.tile_pair[it]);
+    const int tile = e.x, p = e.y;
     const PairState* st = b.state + p;
     if (st->active == 0) continue;  // uniform across the block
     const PairDesc pd = b.pairs[p];
@@ -286,10 +288,13 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
   // tiles, the same source points for every hypothesis of a pair: the tiles
   // listed in gate_tiles (a pass over every tile that skipped the others spent
   // 320 us of a cycle step on skipping; over the list it takes 190 us)
+  // the full pass runs over the tiles of the instances the gate kept (fit_list)
   const bool listed = per_instance && b.gate_tiles;
-  const TileRange tr = xcd_tiles(listed ? b.ngate_tiles : b.ntiles, interleave);
+  const bool kept = !per_instance && b.fit_count;
+  const int nt = listed ? b.ngate_tiles : (kept ? *b.fit_count : b.ntiles);
+  const TileRange tr = xcd_tiles(nt, interleave);
   for (int it = tr.first; it < tr.end; it += tr.step) {
-    const int tile = listed ? b.gate_tiles[it] : it;
+    const int tile = listed ? b.gate_tiles[it] : (kept ? b.fit_list[it].x : it);
     const int p = b.tile_pair[tile];
     PairState* st = b.state + p;
     if (st->gated) continue;  // uniform across the block
@@ -376,6 +381,54 @@ __global__ void k_gate(PairState* st, int npairs, int H) {
       s.gated = 1;
     }
   }
+}
+
+// The (tile, instance) lists of the tiles whose instance takes part: mode 0 --
+// active instances (the search), mode 1 -- instances the gate kept (the final
+// fitness pass).  With the gate setting most hypotheses aside, a launch over
+// every tile left each block a different number of live tiles and waited for the
+// fullest; over the list the live tiles are dealt out evenly.  Two launches: one
+// block places each instance's run of tiles (a block scan over runs of
+// instances per thread, into off[]), then one thread per tile writes its entry.
+constexpr int kCompactThreads = 1024;
+
+__device__ __forceinline__ bool takes_part(const PairState& st, int mode) {
+  return mode == 0 ? st.active != 0 : st.gated == 0;
+}
+
+__global__ void __launch_bounds__(kCompactThreads)
+k_compact_offsets(const IcpBuffers b, int mode, int* __restrict__ off, int* __restrict__ count) {
+  __shared__ int s_off[kCompactThreads];
+  const int t = threadIdx.x;
+  const int per = (b.npairs + kCompactThreads - 1) / kCompactThreads;
+  const int p0 = min(t * per, b.npairs), p1 = min(p0 + per, b.npairs);
+  int n = 0;
+  for (int p = p0; p < p1; ++p)
+    if (takes_part(b.state[p], mode)) n += b.pairs[p].ntiles;
+  s_off[t] = n;
+  __syncthreads();
+  for (int o = 1; o < kCompactThreads; o <<= 1) {
+    const int v = t >= o ? s_off[t - o] : 0;
+    __syncthreads();
+    s_off[t] += v;
+    __syncthreads();
+  }
+  if (t == kCompactThreads - 1) *count = s_off[t];
+  int at = s_off[t] - n;
+  for (int p = p0; p < p1; ++p) {
+    const bool on = takes_part(b.state[p], mode);
+    off[p] = on ? at : -1;
+    if (on) at += b.pairs[p].ntiles;
+  }
+}
+
+__global__ void k_compact_write(const IcpBuffers b, const int* __restrict__ off,
+                                int2* __restrict__ list) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.ntiles) return;
+  const int p = b.tile_pair[t];
+  const int o = off[p];
+  if (o >= 0) list[o + (t - b.pairs[p].tile_begin)] = make_int2(t, p);
 }
 
 __global__ void k_state_init(PairState* st, const double* guesses, const PairDesc* pairs,
@@ -563,6 +616,18 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
                          interleave);
       break;
   }
+}
+
+void launch_compact_tiles(const IcpBuffers& b, int* off, int2* it_list, int* it_count,
+                          int2* fit_list, int* fit_count, bool fit, hipStream_t s) {
+  if (b.npairs <= 0 || b.ntiles <= 0) return;
+  const unsigned blocks = (unsigned)((b.ntiles + 255) / 256);
+  hipLaunchKernelGGL(k_compact_offsets, dim3(1), dim3(kCompactThreads), 0, s, b, 0, off, it_count);
+  hipLaunchKernelGGL(k_compact_write, dim3(blocks), dim3(256), 0, s, b, off, it_list);
+  if (!fit) return;
+  hipLaunchKernelGGL(k_compact_offsets, dim3(1), dim3(kCompactThreads), 0, s, b, 1, off,
+                     fit_count);
+  hipLaunchKernelGGL(k_compact_write, dim3(blocks), dim3(256), 0, s, b, off, fit_list);
 }
 
 void launch_gate(PairState* st, int npairs, int H, hipStream_t s) {

@@ -65,6 +65,13 @@ struct IcpBuffers {
   const int* gate_tiles;  // the tiles the gate's pass scores (every gate_stride-th of an
                           // instance), in tile order (nullable: all, skipping the others)
   int ngate_tiles;
+  // (tile, instance) lists the search and the final fitness pass run over, in tile
+  // order, and their lengths (device memory: rebuilt by launch_compact_tiles as
+  // instances stop or are set aside; nullable: every tile)
+  const int2* it_list;
+  const int* it_count;
+  const int2* fit_list;
+  const int* fit_count;
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)
@@ -84,6 +91,11 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
 // Hypothesis gate over n pairs of H consecutive instances, after a fitness pass
 // with each instance's radius: the best keeps running, the others are set aside.
 void launch_gate(PairState* st, int npairs, int H, hipStream_t s);
+// Rebuilds IcpBuffers::it_list (tiles of active instances) and, with `fit`,
+// fit_list (tiles of instances the gate kept) from the instance states; off holds
+// npairs ints of scratch.
+void launch_compact_tiles(const IcpBuffers& b, int* off, int2* it_list, int* it_count,
+                          int2* fit_list, int* fit_count, bool fit, hipStream_t s);
 // Initialise instance states from row-major 4x4 guesses (nullable = identity);
 // instances with an empty source or target start inactive.
 void launch_state_init(PairState* st, const double* guesses, const PairDesc* pairs,

@@ -487,6 +487,21 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
     LCR_HIP(hipMemcpy(d_pairs.p, hp.data(), npairs * sizeof(PairDesc), hipMemcpyHostToDevice));
   if (ntiles > 0)
     LCR_HIP(hipMemcpy(d_tile_pair.p, tp.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+  // live-tile lists: every tile until a run compacts them
+  d_it_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
+  d_fit_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
+  d_list_count.reserve(2 * sizeof(int));
+  d_list_off.reserve((size_t)std::max(npairs, 1) * sizeof(int));
+  {
+    std::vector<int2> all((size_t)ntiles);
+    for (int t = 0; t < ntiles; ++t) all[t] = make_int2(t, tp[t]);
+    const int counts[2] = {ntiles, ntiles};
+    if (ntiles > 0) {
+      LCR_HIP(hipMemcpy(d_it_list.p, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+      LCR_HIP(hipMemcpy(d_fit_list.p, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    LCR_HIP(hipMemcpy(d_list_count.p, counts, sizeof counts, hipMemcpyHostToDevice));
+  }
   ngate_tiles = (int)gt.size();
   d_gate_tiles.reserve(std::max<size_t>(gt.size(), 1) * sizeof(int));
   if (ngate_tiles > 0)
@@ -519,6 +534,10 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.near = cs.d_near.p ? cs.d_near.as<uint8_t>() : nullptr;
   b.gate_tiles = d_gate_tiles.as<int>();
   b.ngate_tiles = ngate_tiles;
+  b.it_list = d_it_list.as<int2>();
+  b.it_count = d_list_count.as<int>();
+  b.fit_list = d_fit_list.as<int2>();
+  b.fit_count = d_list_count.as<int>() + 1;
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;
@@ -565,6 +584,8 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   launch_state_init(d_state.as<PairState>(), d_guess.as<double>(), d_pairs.as<PairDesc>(),
                     cs.d_grids.as<GridDesc>(), npairs, s);
   launch_fill_i32(d_prev.as<int>(), (int64_t)ntiles * tile, -1, s);  // no previous matches
+  launch_compact_tiles(b, d_list_off.as<int>(), d_it_list.as<int2>(), d_list_count.as<int>(),
+                       d_fit_list.as<int2>(), d_list_count.as<int>() + 1, true, s);
   d_active.reserve(sizeof(int));
   h_active.reserve(sizeof(int));
   const bool eager = !use_graph || prof.on();
@@ -573,6 +594,8 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
     const int k = prof.begin("gate", s, &spans);
     launch_fitness(b, 0.0, nblocks_fit, s, true, c.interleave);
     launch_gate(d_state.as<PairState>(), npairs / gate_h, gate_h, s);
+    launch_compact_tiles(b, d_list_off.as<int>(), d_it_list.as<int2>(), d_list_count.as<int>(),
+                         d_fit_list.as<int2>(), d_list_count.as<int>() + 1, true, s);
     prof.end(k, s, &spans);
   }
   auto chunk = [&](std::vector<Profiler::Span>* sp) {
@@ -584,6 +607,8 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
       launch_icp_solve(b, c, s);
       prof.end(k2, s, sp);
     }
+    launch_compact_tiles(b, d_list_off.as<int>(), d_it_list.as<int2>(), d_list_count.as<int>(),
+                         nullptr, nullptr, false, s);
     hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, d_state.as<PairState>(), npairs,
                        d_active.as<int>());
     LCR_HIP(hipMemcpyAsync(h_active.p, d_active.p, sizeof(int), hipMemcpyDeviceToHost, s));
