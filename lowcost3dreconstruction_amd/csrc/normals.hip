@@ -42,7 +42,11 @@ namespace lcr {
 namespace {
 
 constexpr int kWaves = 4;
-constexpr int kCap = 496;  // staged points per wave: 4 x (496 x 16 + 64 x 4) B = 32 KiB per block
+// staged points per wave: 4 x (560 x 16 + 64 x 4) B = 36 KiB per block.  560 keeps
+// the dense Kinect-v2 captures (1.5 mm voxels) mostly inside LDS -- C5 20.2 against
+// 21.5 ms per step at 496 -- at no cost on cycle36; 600 and 624 (10 slots) cost
+// cycle36 1 %.
+constexpr int kCap = 560;
 constexpr int kSlots = (kCap + 63) / 64;
 
 __device__ __forceinline__ void wave_lds_sync() {
