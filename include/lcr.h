@@ -200,7 +200,7 @@ int lcr_cycle_autotune(lcr_cycle *c, const char *cache_path, lcr_tune_result *ou
 int lcr_transform_points(const float *xyz, const float *nrm, int64_t n, int64_t stride_bytes,
                          const double T[16], float *out_xyz, float *out_nrm);
 
-/* ---- the steps before ICP (VoxelGrid, outlier removal, depth) ---- */
+/* ---- the steps before ICP: VoxelGrid, outlier removal, depth to cloud ---- */
 
 /* pcl::VoxelGrid<PointXYZRGBNormal> with setLeafSize(leaf, leaf, leaf) and filter
  * (pcl_tools/cloud_downsampling.cpp:73-75): one point per occupied voxel of the
