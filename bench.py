@@ -162,6 +162,9 @@ def main():
                     help="let the autotuner pick tile / cell size / precision first (its choice "
                          "varies between near-equal settings from run to run; the default "
                          "measures the library's defaults, reproducibly)")
+    ap.add_argument("--autotune-cache", default=None,
+                    help="file the autotuner looks its choice up in and appends it to "
+                         "(default: none, always tune)")
     ap.add_argument("--compare-precisions", type=int, default=3, metavar="STEPS",
                     help="also time every precision mode for STEPS steps (0 = skip)")
     ap.add_argument("--cpu-sample", type=int, default=3)
@@ -214,13 +217,12 @@ def main():
     upload_s = time.perf_counter() - t_up
     del spare
     # the autotuner times the cycle under its 30 settings (tile, cell size,
-    # search precision) and keeps the fastest; its cache lives in gpurun_out/
+    # search precision) and keeps the fastest; its cache file is --autotune-cache
     tune = None
     tuned_mode = PRECISION[args.precision]
     if args.autotune:
         t_t = time.time()
-        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-        tune = cycle.autotune(os.path.join(ROOT, "gpurun_out", "autotune.txt"))
+        tune = cycle.autotune(args.autotune_cache)
         tune["seconds"] = time.time() - t_t
         tuned_mode = int(tune["precision_mode"])
         log(f"autotune: {tune}")

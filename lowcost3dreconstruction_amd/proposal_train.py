@@ -9,7 +9,7 @@ label is the inverse motion (the transform ICP converges to from the truth on th
 noise-only pairs, checked by `label_check`).  Everything except the random shape and
 motion parameters is generated, trained and evaluated on the GPU.
 
-Usage (GPU box):  python -m lowcost3dreconstruction_amd.proposal_train --pairs 200000
+Usage (on a GPU):  python -m lowcost3dreconstruction_amd.proposal_train --pairs 200000
 """
 from __future__ import annotations
 
