@@ -42,12 +42,12 @@ namespace lcr {
 namespace {
 
 constexpr int kWaves = 4;
-// staged points per wave: 4 x (560 x 16 + 64 x 4) B = 36 KiB per block.  560 keeps
-// the dense regions of Kinect-v2 captures (1.5 mm voxels, 74 clouds of 25k-88k
-// points) mostly inside LDS -- 20.2 against 21.5 ms per cycle step at 496 -- at no
-// cost on Kinect-v1 captures (2 mm voxels, 36 clouds of 37k-113k points); 600 and
-// 624 (10 slots) cost those 1 %.
-constexpr int kCap = 560;
+// staged points per wave: 4 x (576 x 16 + 64 x 4) B = 37 KiB per block, nine slots
+// of 64.  576 keeps the dense regions of Kinect-v2 captures (1.5 mm voxels, 74
+// clouds of 25k-88k points) mostly inside LDS -- 20.0 against 21.5 ms per cycle
+// step at 496 -- at no cost on Kinect-v1 captures (2 mm voxels, 36 clouds of
+// 37k-113k points); 600 and 624 (ten slots) cost those 1 %.
+constexpr int kCap = 576;
 constexpr int kSlots = (kCap + 63) / 64;
 
 __device__ __forceinline__ void wave_lds_sync() {
