@@ -160,7 +160,7 @@ def test_autotune_keeps_results_and_caches(lcr, small_cycle, tmp_path):
 
 def test_hypothesis_gate_refines_the_best_initial_pose(lcr, small_cycle):
     """With hyp_gate, each pair's hypotheses are scored by their source points
-    within r at the initial pose (on a fixed subsample of at least 64 tiles of 64
+    within r at the initial pose (on a fixed subsample of at least 32 tiles of 64
     source points) and only the best is refined: bitwise the single-pair ICP from
     that pose.  Against the oracle's full inlier counts: a clear winner (1.5x the
     runner-up) must be the one kept, otherwise the kept pose has at least 2/3 of
