@@ -18,21 +18,26 @@ paired onto capture 0), 512x424 Kinect-v2 rendering, noise 0.001 z, 1.5 mm voxel
 source centroid), plus the learned one when weights are present.
 
 With several hypotheses per pair (the learned one, or C5's 15) each pair's
-initial poses are scored by their inliers within r and only the best is refined
-(`--no-gate` refines them all and keeps the best by final fitness).
+initial poses are scored by their inliers within r: the first (the prior) is
+always refined, and the best other one too when it scores higher; the lower final
+fitness wins (`--no-gate` refines them all and keeps the best by final fitness).
 
-The launch uses the library's defaults (tile 64, cell size 2.5 point spacings);
-`--autotune` lets the autotuner pick tile, cell size and search precision on this
-device first.  A step = normals + grids + all ICP iterations (one hipGraph) + fitness
-+ selection for the whole cycle, with the clouds already resident in HBM.  `value` counts the
-pairs whose selected transform is within 0.05 deg / 0.2 mm of the FP64 oracle's
-(run from the prior), divided by the step time.
+The autotuner picks tile, cell size and search precision on this device first
+(`--no-autotune` measures the library's defaults: tile 64, cell size 2.5 point
+spacings, FP32 scan).  A step = normals + grids + all ICP iterations (hipGraph
+chunks) + fitness + selection for the whole cycle, with the clouds already
+resident in HBM.  `value` counts the pairs whose selected transform is within
+0.05 deg / 0.2 mm of the FP64 oracle's run from the same initial pose (the
+hypothesis the step kept), divided by the step time.
 
-Also reported: the plain single-pair GPU path (one pair at a time through the
-per-pair API: normals + upload + align + fitness), every correspondence-search
-precision mode on the same cycle (tile 64, checked bitwise against the FP32 scan), the dominant kernel's roofline from HIP events
-recorded around each launch, and a host CPU baseline (the PCL-semantics FP32
-restatement with an exact kd-tree, one thread, on a bounded sample of pairs).
+Also reported: the plain single-pair GPU path (warm: one resident single-pair
+cycle per pair, graphs captured beforehand; cold: the per-pair API with a new
+handle and upload per pair), every correspondence-search precision mode on the
+same cycle (checked bitwise against the FP32 scan), the roofline of every major
+kernel from HIP events recorded around each launch (dominant first), the latency
+of a fresh cycle (upload + first run), what the learned hypothesis gains when the
+seed is not the rig's prior, and a host CPU baseline (the PCL-semantics FP32
+restatement with an exact kd-tree, on a bounded sample of pairs).
 """
 import argparse
 import json
@@ -50,6 +55,7 @@ ROT_TOL = np.deg2rad(0.05)
 TRANS_TOL = 0.2e-3
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2, "fp32tiles": 3}
 WEIGHTS = os.path.join(ROOT, "lowcost3dreconstruction_amd", "models", "proposal_c4.npy")
+TRAFFIC = os.path.join(ROOT, "profiles", "kernel_traffic.json")
 
 
 def log(*a):
@@ -74,31 +80,59 @@ def icp_kw():
                 euclidean_fitness_epsilon=1e-3)
 
 
-def cpu_reference_and_baseline(cyc, sample_pairs, threads):
-    """The oracle is used here only as the checker (FP64 reference transforms for
-    every pair, from the prior) and as the timed host baseline (PCL-semantics FP32
-    mode, one thread, on `sample_pairs` pairs)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_references(cyc, threads, starts):
+    """The checker: per pair, the FP64 oracle (oracle normals) run from the initial
+    pose the GPU step kept.  starts: (P, 4, 4)."""
     import oracle
     t0 = time.time()
     normals = [oracle.normals(c, 30, nthreads=threads)[0] for c in cyc.clouds]
-    refs = oracle.icp_batch(cyc.clouds, normals, cyc.pairs, guesses=cyc.meta["prior"],
-                            nthreads=threads, point_to_plane=True, mode=oracle.FP64, **icp_kw())
+    refs = oracle.icp_batch(cyc.clouds, normals, cyc.pairs, guesses=starts, nthreads=threads,
+                            point_to_plane=True, mode=oracle.FP64, **icp_kw())
     log(f"oracle FP64 references: {time.time() - t0:.1f} s")
-    # all host cores: the whole cycle, pairs spread over `threads` threads
-    t2 = time.perf_counter()
-    tn_all = [oracle.normals(c, 30, nthreads=threads)[0] for c in cyc.clouds]
-    oracle.icp_batch(cyc.clouds, tn_all, cyc.pairs, guesses=cyc.meta["prior"], nthreads=threads,
-                     point_to_plane=True, mode=oracle.PCL32, **icp_kw())
-    all_cores_s = time.perf_counter() - t2
-    times = []
-    for i in range(sample_pairs):
-        s, t = cyc.pairs[i]
+    return refs
+
+
+def cpu_baseline(cyc, sample_pairs, threads, reps=5):
+    """The timed host baseline: the oracle's PCL-semantics FP32 mode (exact
+    kd-tree), 30-NN normals of the target + point-to-plane ICP from the prior per
+    pair.  One thread: one warm-up pass over `sample_pairs` pairs, then the median
+    of `reps` timed passes.  All host threads: the whole cycle with the pairs spread
+    over `threads` threads, one warm-up and the median of `reps` runs."""
+    import oracle
+    prior = cyc.meta["prior"]
+
+    def one_pass():
         t1 = time.perf_counter()
-        tn = oracle.normals(cyc.clouds[t], 30, nthreads=1)[0]
-        oracle.icp(cyc.clouds[s], cyc.clouds[t], tn, guess=cyc.meta["prior"][i], point_to_plane=True,
-                   mode=oracle.PCL32, **icp_kw())
-        times.append(time.perf_counter() - t1)
-    return refs, float(np.median(times)), len(cyc.pairs) / all_cores_s
+        for i in range(sample_pairs):
+            s, t = cyc.pairs[i]
+            tn = oracle.normals(cyc.clouds[t], 30, nthreads=1)[0]
+            oracle.icp(cyc.clouds[s], cyc.clouds[t], tn, guess=prior[i], point_to_plane=True,
+                       mode=oracle.PCL32, **icp_kw())
+        return (time.perf_counter() - t1) / sample_pairs
+
+    def all_cores():
+        t2 = time.perf_counter()
+        tn_all = [oracle.normals(c, 30, nthreads=threads)[0] for c in cyc.clouds]
+        oracle.icp_batch(cyc.clouds, tn_all, cyc.pairs, guesses=prior, nthreads=threads,
+                         point_to_plane=True, mode=oracle.PCL32, **icp_kw())
+        return time.perf_counter() - t2
+
+    one_pass()
+    single = float(np.median([one_pass() for _ in range(reps)]))
+    all_cores()
+    multi = float(np.median([all_cores() for _ in range(reps)]))
+    return 1.0 / single, len(cyc.pairs) / multi
 
 
 def aligned_mask(T, refs):
@@ -109,6 +143,11 @@ def aligned_mask(T, refs):
         errs.append((er, et))
         ok.append(er <= ROT_TOL and et <= TRANS_TOL)
     return np.array(ok, bool), np.array(errs).reshape(-1, 2)
+
+
+def gt_errors(T, T_gt):
+    from lowcost3dreconstruction_amd import synth
+    return np.array([synth.transform_error(a, b) for a, b in zip(T, T_gt)]).reshape(-1, 2)
 
 
 def copy_calibration(mib):
@@ -126,14 +165,13 @@ def copy_calibration(mib):
 
 
 def measured_traffic():
-    """HBM bytes per iteration launch from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy calibration), or
-    None when absent."""
-    path = os.path.join(ROOT, "profiles", "icp_iteration_traffic.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        return float(json.load(f)["traffic_bytes_per_launch"])
+    """HBM bytes per launch per kernel from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel's reading),
+    or {} when absent."""
+    if not os.path.exists(TRAFFIC):
+        return {}
+    with open(TRAFFIC) as f:
+        return json.load(f).get("kernels", {})
 
 
 def timed_steps(cycle, steps):
@@ -141,6 +179,18 @@ def timed_steps(cycle, steps):
     for _ in range(steps):
         cycle.run()
     return (time.perf_counter() - t0) / steps
+
+
+def roofline_entry(name, kernel, bytes_per_launch, launches, total_ms, steps, traffic):
+    avg_s = (total_ms / 1e3) / max(launches, 1)
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    t = traffic.get(name)
+    return {"name": name, "kernel": kernel, "bound": "hbm", "unit": "GB/s",
+            "algorithmic_bytes_per_launch": bytes_per_launch, "launches_per_step": launches / steps,
+            "avg_launch_us": avg_s * 1e6, "ms_per_step": total_ms / steps, "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "frac": achieved / HBM_PEAK_GBS,
+            "traffic": (t["traffic_bytes_per_launch"] if t else None),
+            "traffic_source": (t["source"] if t else None)}
 
 
 def main():
@@ -151,26 +201,27 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", choices=("cycle36", "c5"), default="cycle36")
     ap.add_argument("--precision", choices=sorted(PRECISION), default="fp32",
-                    help="correspondence search of the measured run: exact FP32 scan, or "
+                    help="correspondence search with --no-autotune: exact FP32 scan, or "
                          "FP16/BF16 matrix-core distance tiles with exact FP32 rerank")
     ap.add_argument("--proposal", default="auto",
                     help="'auto' (use the committed weights when present), 'none', or a path")
     ap.add_argument("--no-gate", action="store_true",
-                    help="refine every hypothesis of a pair instead of only the one with the "
-                         "most inliers at its initial pose")
-    ap.add_argument("--autotune", action="store_true",
-                    help="let the autotuner pick tile / cell size / precision first (its choice "
-                         "varies between near-equal settings from run to run; the default "
-                         "measures the library's defaults, reproducibly)")
+                    help="refine every hypothesis of a pair instead of the seed and the best "
+                         "other one by inliers at the initial pose")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="measure the library's default launch (tile 64, cell size 2.5, FP32 "
+                         "scan) instead of the autotuner's choice")
     ap.add_argument("--autotune-cache", default=None,
                     help="file the autotuner looks its choice up in and appends it to "
                          "(default: none, always tune)")
     ap.add_argument("--compare-precisions", type=int, default=3, metavar="STEPS",
                     help="also time every precision mode for STEPS steps (0 = skip)")
-    ap.add_argument("--cpu-sample", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=5)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-pairs", type=int, default=36)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the fresh-cycle latency and the perturbed-seed learned run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -185,6 +236,7 @@ def main():
         sys.exit(2)
 
     import lowcost3dreconstruction_amd as lcr
+    from lowcost3dreconstruction_amd import synth
     lcr.init(0)
     t0 = time.time()
     cyc, guesses, n_hyp = make_workload(args.workload, args.seed)
@@ -199,38 +251,46 @@ def main():
         log(f"learned proposal: {wpath}")
     elif args.proposal not in ("auto", "none"):
         raise SystemExit(f"proposal weights not found: {wpath}")
+    launch = {"mode": PRECISION[args.precision], "tile": 0, "cell_factor": 0.0}
 
-    def make_cycle(mode, with_prop=True, tile=0, cell_factor=0.0):
+    def make_cycle(with_prop=True, g=None, nh=None, **over):
         p = prop if with_prop else None
-        gate = not args.no_gate and (n_hyp > 1 or p is not None)
-        return lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses, n_hyp=n_hyp, point_to_plane=True,
-                         normals_k=30, precision_mode=mode, proposal=p, hyp_gate=gate,
-                         tile=tile, cell_factor=cell_factor, **icp_kw())
+        nh = n_hyp if nh is None else nh
+        gate = not args.no_gate and (nh > 1 or p is not None)
+        kw = dict(launch)
+        kw.update(over)
+        return lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses if g is None else g, n_hyp=nh,
+                         point_to_plane=True, normals_k=30, precision_mode=kw["mode"], proposal=p,
+                         hyp_gate=gate, tile=kw["tile"], cell_factor=kw["cell_factor"], **icp_kw())
 
     # ---------------- batched cycle (the measured path); creating a cycle copies
     # the clouds host -> device once (the PCIe-inclusive rate below adds that time)
-    cycle = make_cycle(PRECISION[args.precision])
-    # steady-state hand-over: a second cycle object over the same clouds (the first
-    # creation in a process also pays the runtime's warm-up and first allocations)
-    t_up = time.perf_counter()
-    spare = make_cycle(PRECISION[args.precision])
-    upload_s = time.perf_counter() - t_up
-    del spare
-    # the autotuner times the cycle under its 30 settings (tile, cell size,
-    # search precision) and keeps the fastest; its cache file is --autotune-cache
+    cycle = make_cycle()
     tune = None
-    tuned_mode = PRECISION[args.precision]
-    if args.autotune:
+    if not args.no_autotune:
         t_t = time.time()
         tune = cycle.autotune(args.autotune_cache)
         tune["seconds"] = time.time() - t_t
-        tuned_mode = int(tune["precision_mode"])
+        launch = {"mode": int(tune["precision_mode"]), "tile": int(tune["tile"]),
+                  "cell_factor": float(tune["cell_factor"])}
         log(f"autotune: {tune}")
+    # steady-state hand-over: a second cycle object over the same clouds (the first
+    # creation in a process also pays the runtime's warm-up and first allocations),
+    # then its first run (graph capture, per-upload geometry probe): a fresh cycle's
+    # latency
+    t_up = time.perf_counter()
+    spare = make_cycle()
+    upload_s = time.perf_counter() - t_up
+    spare.run()
+    fresh_s = time.perf_counter() - t_up
+    del spare
     for _ in range(args.warmup):
         cycle.run()
     step_s = timed_steps(cycle, args.steps)
     res = cycle.results()
     inst = cycle.instances()
+    init = cycle.initial_poses()
+    H = cycle.n_total_hyp
     # Per-kernel GPU time: HIP events cannot be recorded inside a captured graph,
     # so the same steps are replayed with the profiler on, which issues the same
     # launches eagerly with an event pair around each kernel on its stream.
@@ -238,42 +298,31 @@ def main():
     lcr.prof_enable(True)
     prof_step_s = timed_steps(cycle, args.steps)
     lcr.prof_enable(False)
-    n_iter_launches, iter_ms = lcr.prof_read("icp_iteration")
-    _, fit_ms = lcr.prof_read("fitness")
-    _, grid_ms = lcr.prof_read("grid_build")
-    _, nrm_ms = lcr.prof_read("normals")
-    _, prop_ms = lcr.prof_read("proposal")
-    _, gate_ms = lcr.prof_read("gate")
-    _, solve_ms = lcr.prof_read("icp_solve")
+    prof = {k: lcr.prof_read(k) for k in ("icp_iteration", "icp_solve", "fitness", "grid_build",
+                                          "normals", "proposal", "gate")}
     log(f"batched step {step_s * 1e3:.3f} ms (profiled replay {prof_step_s * 1e3:.3f} ms); "
-        f"per step: icp search {iter_ms / args.steps:.3f} ms ({n_iter_launches // args.steps} "
-        f"launches) + solve {solve_ms / args.steps:.3f} ms, "
-        f"fitness {fit_ms / args.steps:.3f} ms, grids {grid_ms / args.steps:.3f} ms, "
-        f"normals {nrm_ms / args.steps:.3f} ms, proposal {prop_ms / args.steps:.3f} ms")
+        + ", ".join(f"{k} {v[1] / args.steps:.3f} ms ({v[0] // args.steps} launches)"
+                    for k, v in prof.items()))
 
     # ---------------- the same cycle without the learned hypothesis, and every
     # precision mode (their transforms must agree bitwise with the measured run)
     no_prop = None
     if prop is not None:
-        c0 = (make_cycle(tuned_mode, with_prop=False, tile=tune["tile"],
-                         cell_factor=tune["cell_factor"])
-              if tune else make_cycle(PRECISION[args.precision], with_prop=False))
+        c0 = make_cycle(with_prop=False)
         c0.run()
         r0 = c0.results()
         no_prop = {"ms_per_step": timed_steps(c0, max(1, args.steps // 2)) * 1e3,
                    "iterations_mean": float(r0["iterations"].mean())}
         no_prop_T = r0["T"]
+        no_prop_start = c0.initial_poses()[np.arange(len(cyc.pairs)) * c0.n_total_hyp
+                                           + r0["best_hyp"]]
         log(f"without the learned hypothesis: {no_prop['ms_per_step']:.3f} ms/step")
         del c0
-    # every search mode at tile 64 (the distance-tile modes need one wavefront per
-    # tile) and the measured run's cell size: same reduction order, so the
-    # transforms must agree bitwise with the fp32 scan's
     modes = {}
     if args.compare_precisions > 0:
-        cf = tune["cell_factor"] if tune else 0.0
         ref_T = None
         for name, mode in sorted(PRECISION.items(), key=lambda kv: kv[1]):
-            cm = make_cycle(mode, tile=64, cell_factor=cf)
+            cm = make_cycle(mode=mode, tile=64)
             cm.run()
             lcr.prof_reset()
             lcr.prof_enable(True)
@@ -291,23 +340,56 @@ def main():
                 f"{modes[name]['icp_iteration_ms_per_step']:.3f} ms, identical results: {same}")
             del cm
 
-    # ---------------- roofline of the iteration kernel: per instance and executed
-    # iteration 12 B per source point + 24 B per target point (xyz + normal)
-    H = len(inst["iterations"]) // len(cyc.pairs)
+    # ---------------- roofline of every major kernel (algorithmic bytes per launch
+    # / average launch time from the HIP events), dominant first
     it = inst["iterations"].astype(np.int64)
     ns = np.repeat(np.array([sizes[s] for s, _ in cyc.pairs], np.int64), H)
     nt = np.repeat(np.array([sizes[t] for _, t in cyc.pairs], np.int64), H)
-    bytes_total = float(np.sum(it * (12 * ns + 24 * nt)))
-    launches = max(n_iter_launches // args.steps, 1)
-    bytes_per_launch = bytes_total / launches
-    avg_launch_s = (iter_ms / 1e3) / max(n_iter_launches, 1)
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    refined = it > 0
+    steps = args.steps
+    traffic = measured_traffic()
+    n_it, it_ms = prof["icp_iteration"]
+    n_fit, fit_ms = prof["fitness"]
+    n_nrm, nrm_ms = prof["normals"]
+    kernels = [
+        # per instance and executed iteration: 12 B per source point + 24 B per target
+        # point (xyz + normal)
+        roofline_entry("icp_search", "k_icp_iter", float(np.sum(it * (12 * ns + 24 * nt)))
+                       / max(n_it / steps, 1), n_it, it_ms, steps, traffic),
+        # per refined instance: 12 B per source point + 12 B per target point
+        roofline_entry("fitness", "k_fitness", float(np.sum((12 * ns + 12 * nt)[refined]))
+                       / max(n_fit / steps, 1), n_fit, fit_ms, steps, traffic),
+        # per cloud: 12 B read + 24 B grid + 16 B normal and curvature written
+        roofline_entry("normals", "k_normal_moments + k_normal_solve", 52.0 * sum(sizes)
+                       / max(n_nrm / steps, 1), n_nrm, nrm_ms, steps, traffic),
+    ]
+    kernels.sort(key=lambda e: -e["ms_per_step"])
 
     # ---------------- plain single-pair GPU path (prior seed, one pair at a time)
     n_plain = min(args.plain_pairs, len(cyc.pairs))
-    plain_T = []
-    plain_s_per_pair = None
+    plain_T, plain_s_per_pair, cold_T, cold_s_per_pair = [], None, [], None
+    prior = cyc.meta["prior"]
     if n_plain > 0:
+        # warm: one resident single-pair cycle per pair (its two clouds uploaded, its
+        # buffers sized and its graphs captured beforehand); each run computes the
+        # normals of its own two clouds, builds their grids, iterates and scores
+        singles = []
+        for i in range(n_plain):
+            s, t = cyc.pairs[i]
+            c1 = lcr.Cycle([cyc.clouds[s], cyc.clouds[t]], [(0, 1)], guesses=prior[i:i + 1],
+                           point_to_plane=True, normals_k=30, precision_mode=launch["mode"],
+                           tile=launch["tile"], cell_factor=launch["cell_factor"], **icp_kw())
+            c1.run()
+            singles.append(c1)
+        reps = 3
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            for c1 in singles:
+                c1.run()
+        plain_s_per_pair = (time.perf_counter() - t1) / (reps * n_plain)
+        plain_T = [c1.results()["T"][0] for c1 in singles]
+        del singles
+        # cold: the per-pair API, a new handle and upload per pair
         t1 = time.perf_counter()
         for i in range(n_plain):
             s, t = cyc.pairs[i]
@@ -320,38 +402,77 @@ def main():
             icp.setMaximumIterations(kw["max_iterations"])
             icp.setTransformationEpsilon(kw["transformation_epsilon"])
             icp.setEuclideanFitnessEpsilon(kw["euclidean_fitness_epsilon"])
-            icp.align(cyc.meta["prior"][i])
+            icp.align(prior[i])
             icp.getFitnessScore()
-            plain_T.append(icp.getFinalTransformation())
-        plain_s_per_pair = (time.perf_counter() - t1) / n_plain
+            cold_T.append(icp.getFinalTransformation())
+        cold_s_per_pair = (time.perf_counter() - t1) / n_plain
+
+    # ---------------- what the learned hypothesis does when the seed is not the
+    # rig's prior: each pair's seed is the prior moved by a random rotation of up to
+    # 20 deg about the prior-mapped source centroid
+    perturbed = None
+    if prop is not None and not args.no_extras and args.workload == "cycle36":
+        seeds = synth.hypotheses(cyc, 2, 20.0, 0.0, args.seed + 7)[1::2]
+        out = {}
+        for name, p in (("seed_only", False), ("seed_and_learned", True)):
+            cp = make_cycle(with_prop=p, g=seeds, nh=1)
+            cp.run()
+            ms = timed_steps(cp, max(1, args.steps // 2)) * 1e3
+            rp = cp.results()
+            ge = gt_errors(rp["T"], cyc.T_gt)
+            out[name] = {"ms_per_step": ms, "iterations_mean": float(rp["iterations"].mean()),
+                         "registered_pairs": int(np.sum((np.rad2deg(ge[:, 0]) < 0.5)
+                                                        & (ge[:, 1] < 2e-3))),
+                         "median_rot_err_deg": float(np.rad2deg(np.median(ge[:, 0]))),
+                         "learned_chosen": (float(np.mean(rp["best_hyp"] == 1)) if p else None)}
+            del cp
+        perturbed = {"seed": "turntable prior rotated by U[0, 20] deg about a random axis "
+                             "through the source centroid",
+                     "registered": "within 0.5 deg / 2 mm of the true motion", **out}
+        log(f"perturbed seeds: {out}")
 
     # ---------------- checker and host baseline
     cpu = None
+    starts = init[np.arange(len(cyc.pairs)) * H + res["best_hyp"]]
     if not args.no_cpu:
-        refs, cpu_per_pair, cpu_all = cpu_reference_and_baseline(cyc, args.cpu_sample, args.threads)
+        refs = oracle_references(cyc, args.threads, starts)
         ok, errs = aligned_mask(res["T"], refs)
-        ok_plain = aligned_mask(plain_T, refs[:n_plain])[0] if n_plain else np.zeros(0, bool)
+        prior_refs = refs if np.array_equal(starts, prior) else \
+            oracle_references(cyc, args.threads, prior)
+        ok_plain = aligned_mask(plain_T, prior_refs[:n_plain])[0] if n_plain else np.zeros(0, bool)
+        ok_cold = aligned_mask(cold_T, prior_refs[:n_plain])[0] if n_plain else np.zeros(0, bool)
         if no_prop is not None:
-            no_prop["aligned_pairs"] = int(aligned_mask(no_prop_T, refs)[0].sum())
-        cpu = {"value": 1.0 / cpu_per_pair, "unit": "pairs/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_sample} pairs of the same cycle: 30-NN normals of the target "
-                         f"+ point-to-plane ICP from the prior, PCL-semantics FP32 restatement "
-                         f"with an exact kd-tree (not PCL), median per pair",
-               "all_cores": {"value": cpu_all, "unit": "pairs/s", "cores": args.threads,
-                             "sample": "the whole cycle, pairs spread over the threads"}}
+            np_refs = prior_refs if np.array_equal(no_prop_start, prior) else \
+                oracle_references(cyc, args.threads, no_prop_start)
+            no_prop["aligned_pairs"] = int(aligned_mask(no_prop_T, np_refs)[0].sum())
+        t_c = time.time()
+        single, multi = cpu_baseline(cyc, args.cpu_sample, args.threads)
+        log(f"cpu baseline: {time.time() - t_c:.1f} s")
+        cpu = {"value": single, "unit": "pairs/s", "cores": 1, "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"{args.cpu_sample} pairs of the same cycle per pass (30-NN normals of "
+                         f"the target + point-to-plane ICP from the prior), PCL-semantics FP32 "
+                         f"restatement with an exact kd-tree (not PCL); 1 warm-up pass, median "
+                         f"of 5 passes",
+               "all_cores": {"value": multi, "unit": "pairs/s", "cores": args.threads,
+                             "sample": "the whole cycle, pairs spread over the threads; 1 "
+                                       "warm-up, median of 5"}}
     else:
         ok = np.ones(len(cyc.pairs), bool)
         ok_plain = np.ones(n_plain, bool)
+        ok_cold = np.ones(n_plain, bool)
         errs = np.zeros((len(cyc.pairs), 2))
     n_aligned = int(ok.sum())
     value = n_aligned / step_s
-    # aligned pairs by the hypothesis the step kept (0: the turntable prior)
     kept_prior = np.asarray(res["best_hyp"]) == 0
     by_kept = {"prior": [int(ok[kept_prior].sum()), int(kept_prior.sum())],
                "other": [int(ok[~kept_prior].sum()), int((~kept_prior).sum())]}
+    gte = gt_errors(res["T"], cyc.T_gt)
     plain_value = float(ok_plain.mean()) / plain_s_per_pair if n_plain else None
+    cold_value = float(ok_cold.mean()) / cold_s_per_pair if n_plain else None
     if no_prop is not None:
         no_prop["value"] = no_prop.get("aligned_pairs", len(cyc.pairs)) / (no_prop["ms_per_step"] / 1e3)
+    dom = kernels[0]
 
     out = {
         "metric": "aligned pairs/sec at <=0.05deg/0.2mm vs FP64 oracle",
@@ -377,35 +498,48 @@ def main():
             "hypothesis_gate": (not args.no_gate and (n_hyp > 1 or prop is not None)),
             "max_iterations": 60, "max_corr_dist_m": 0.01, "normals_k": 30, "seed": args.seed,
             "initial_pose": "turntable prior (rig yaw step about its axis)",
-            "correspondence_search": {v: k for k, v in PRECISION.items()}[tuned_mode],
+            "correspondence_search": {v: k for k, v in PRECISION.items()}[launch["mode"]],
             "launch": ({"tile": tune["tile"], "cell_factor": tune["cell_factor"], "tuned": True}
                        if tune else {"tile": "default", "cell_factor": "default", "tuned": False}),
             "parallelism": "single GPU",
         },
         "aligned_pairs": n_aligned,
+        "aligned_reference": "FP64 oracle on oracle normals, run from the hypothesis each pair kept",
         "aligned_by_kept_hypothesis": by_kept,
-        "upload_ms": upload_s * 1e3,
-        "pcie_inclusive_pairs_per_s": n_aligned / (step_s + upload_s),
         "max_rot_err_deg": float(np.rad2deg(errs[:, 0].max())),
         "max_trans_err_m": float(errs[:, 1].max()),
+        "vs_true_motion": {"median_rot_err_deg": float(np.rad2deg(np.median(gte[:, 0]))),
+                           "max_rot_err_deg": float(np.rad2deg(gte[:, 0].max())),
+                           "median_trans_err_m": float(np.median(gte[:, 1])),
+                           "max_trans_err_m": float(gte[:, 1].max())},
+        "upload_ms": upload_s * 1e3,
+        "fresh_cycle_ms": fresh_s * 1e3,
+        "pcie_inclusive_pairs_per_s": n_aligned / (step_s + upload_s),
         "iterations_mean": float(res["iterations"].mean()),
+        "instances_refined": int(refined.sum()),
         "learned_chosen": (float(np.mean(res["best_hyp"] == n_hyp)) if prop is not None else None),
         "without_learned": no_prop,
+        "learned_on_perturbed_seeds": perturbed,
         "plain_gpu": ({"value": plain_value, "unit": "pairs/s", "pairs": n_plain,
-                       "note": "one pair at a time through the per-pair API "
-                               "(normals + upload + align + fitness)"} if n_plain else None),
+                       "note": "warm: one resident single-pair cycle per pair (graphs captured "
+                               "beforehand); each run computes its two clouds' normals, grids, "
+                               "ICP and fitness"} if n_plain else None),
+        "plain_gpu_cold": ({"value": cold_value, "unit": "pairs/s", "pairs": n_plain,
+                            "note": "per-pair API: new handle, upload, normals, align, fitness"}
+                           if n_plain else None),
         "speedup_vs_plain_gpu": (value / plain_value if plain_value else None),
+        "speedup_vs_plain_gpu_cold": (value / cold_value if cold_value else None),
         "speedup_vs_cpu_baseline": (value / cpu["value"] if cpu else None),
+        "speedup_vs_cpu_all_cores": (value / cpu["all_cores"]["value"] if cpu else None),
         "autotune": tune,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "icp_iteration", "avg_launch_us": avg_launch_s * 1e6,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": dom["traffic"],
+                     "traffic_source": dom["traffic_source"], "kernel": dom["name"],
+                     "avg_launch_us": dom["avg_launch_us"],
+                     "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
+                     "kernels": kernels},
         "precision_modes": modes,
-        "kernel_ms_per_step": {"icp_iteration": iter_ms / args.steps,
-                               "icp_solve": solve_ms / args.steps, "fitness": fit_ms / args.steps,
-                               "grid_build": grid_ms / args.steps, "normals": nrm_ms / args.steps,
-                               "proposal": prop_ms / args.steps, "gate": gate_ms / args.steps,
+        "kernel_ms_per_step": {**{k: v[1] / steps for k, v in prof.items()},
                                "profiled_step": prof_step_s * 1e3},
         "cpu_baseline": cpu,
     }

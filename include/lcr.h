@@ -146,8 +146,11 @@ typedef struct {
   int32_t hyp_gate;             /* 0: refine every hypothesis, keep the best by final
                                    fitness; 1: first score every hypothesis's initial
                                    pose by its source points within max_corr_dist
-                                   (ties: smaller mean d2) and refine only the best
-                                   one per pair */
+                                   (ties: smaller mean d2) on a subsample of the
+                                   source, then refine the pair's first hypothesis
+                                   (the plain seed) and, when another one scores
+                                   higher, that one too; the lower final fitness
+                                   of the two wins (ties: the first) */
 } lcr_cycle_desc;
 
 typedef struct {
@@ -171,6 +174,11 @@ int lcr_cycle_results(lcr_cycle *c, lcr_cycle_result *r);
  * n_pairs * (n_hyp + 1 if a proposal is set, else n_hyp) entries (each nullable) */
 int lcr_cycle_instances(lcr_cycle *c, double *T, int32_t *iterations, int32_t *state,
                         double *fitness);
+/* The initial pose of every instance of the last run, in the same order as
+ * lcr_cycle_instances (row-major 4x4 each): the given guesses and, with a
+ * proposal, the learned pose the network predicted.  Lets a caller rerun any
+ * instance through the single-pair path (alignment.sh:105-113 from that pose). */
+int lcr_cycle_initial_poses(lcr_cycle *c, double *T);
 void lcr_cycle_destroy(lcr_cycle *c);
 
 /* Autotuner (no counterpart in the reference, which has no launch shape to
@@ -191,6 +199,28 @@ typedef struct {
   int32_t from_cache;
 } lcr_tune_result;
 int lcr_cycle_autotune(lcr_cycle *c, const char *cache_path, lcr_tune_result *out);
+
+/* ---- the cycle driver's pose bookkeeping (host arithmetic, no device work) ---- */
+
+/* In-memory pose chaining, replacing alignment.sh:105-113's in-place rewrites of
+ * every later capture: poses[j] = T_1 T_2 ... T_j maps capture j into capture 0's
+ * frame (poses[0] = I), T (row-major 4x4 each, n_captures - 1 of them) holding
+ * T_i, the transform of capture i onto capture i - 1.  With T_close (nullable:
+ * capture 0 onto capture n_captures - 1) the loop residual E = poses[n-1] T_close
+ * -- the identity for a consistent cycle -- is spread evenly over the loop's
+ * n_captures edges: poses[j] <- E^(-j/n) poses[j], E^s = exp(s log E) in SE(3);
+ * every edge of the loop, the closing one included, then carries E^(1/n). */
+int lcr_chain_poses(int32_t n_captures, const double *T, const double *T_close, double *poses);
+/* out = T^s = exp(s log T) for a rigid row-major 4x4 T (the screw motion scaled by s). */
+int lcr_se3_pow(const double T[16], double s, double out[16]);
+/* centroid_align's transform (pcl_tools/centroid_align.cpp:104-130), the seed
+ * alignment.sh:117-118 gives the top and bottom views: move the source's centroid
+ * to the origin, rotate by Rx(roll) Ry(pitch) Rz(yaw) (degrees), move it to the
+ * target's centroid, then by `elevation` along y.  Packed xyz; centroids over the
+ * finite points. */
+int lcr_centroid_align(const float *src, int64_t n_src, const float *tgt, int64_t n_tgt,
+                       double roll_deg, double pitch_deg, double yaw_deg, double elevation,
+                       double T[16]);
 
 /* Rigid transform of a cloud on the device: xyz' = T xyz (FP64, rounded once to
  * float) and normal' = R normal, non-finite entries passed through.  Replaces

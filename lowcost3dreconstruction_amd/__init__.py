@@ -40,13 +40,15 @@ EXPORTS = (
     "lcr_icp_final_transform", "lcr_icp_has_converged", "lcr_icp_fitness",
     "lcr_icp_iterations", "lcr_icp_convergence_state", "lcr_icp_correspondences",
     "lcr_normals", "lcr_cycle_create", "lcr_cycle_run", "lcr_cycle_results",
-    "lcr_cycle_instances", "lcr_cycle_destroy", "lcr_icp_batch_align", "lcr_prof_enable",
+    "lcr_cycle_instances", "lcr_cycle_initial_poses", "lcr_cycle_destroy",
+    "lcr_icp_batch_align", "lcr_prof_enable",
     "lcr_prof_reset", "lcr_prof_read", "lcr_last_error", "lcr_version",
     "lcr_proposal_create", "lcr_proposal_destroy", "lcr_proposal_set_weights",
     "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
     "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
     "lcr_proposal_predict", "lcr_cycle_autotune", "lcr_transform_points", "lcr_voxel_grid",
     "lcr_outlier_removal", "lcr_depth_to_cloud",
+    "lcr_chain_poses", "lcr_se3_pow", "lcr_centroid_align",
 )
 
 
@@ -141,6 +143,7 @@ def _load():
     L.lcr_cycle_run.argtypes = [vp]
     L.lcr_cycle_results.argtypes = [vp, C.POINTER(CycleResult)]
     L.lcr_cycle_instances.argtypes = [vp, dp, ip, ip, dp]
+    L.lcr_cycle_initial_poses.argtypes = [vp, dp]
     L.lcr_cycle_destroy.argtypes = [vp]
     L.lcr_cycle_destroy.restype = None
     L.lcr_icp_batch_align.argtypes = [C.POINTER(CycleDesc), C.POINTER(CycleResult)]
@@ -167,6 +170,9 @@ def _load():
     L.lcr_outlier_removal.argtypes = [fp, C.c_int64, C.c_int32, C.c_double, u8, fp, dp, i64p]
     L.lcr_depth_to_cloud.argtypes = [C.POINTER(C.c_uint16), C.c_int32, C.c_int32, u8, dp,
                                      C.c_int32, C.c_int32, C.POINTER(C.c_int32), fp, fp, u8, i64p]
+    L.lcr_chain_poses.argtypes = [C.c_int32, dp, dp, dp]
+    L.lcr_se3_pow.argtypes = [dp, C.c_double, dp]
+    L.lcr_centroid_align.argtypes = [fp, C.c_int64, fp, C.c_int64] + [C.c_double] * 4 + [dp]
     L.lcr_last_error.restype = C.c_char_p
     L.lcr_version.restype = C.c_char_p
     return L
@@ -329,6 +335,39 @@ def transform_points(xyz, T, nrm=None):
     _check(lib.lcr_transform_points(_fp(x), _fp(n) if n is not None else None, len(x), 12, _dp(M),
                                     _fp(out), _fp(out_n) if out_n is not None else None))
     return out, out_n
+
+
+def chain_poses(T, T_close=None):
+    """Poses of the captures of a cycle in capture 0's frame from the pair
+    transforms T[i - 1] (capture i onto i - 1), with the closing pair's residual
+    (T_close: capture 0 onto the last capture) spread evenly over the loop.
+    Replaces alignment.sh:105-113's in-place chaining.  Returns (n, 4, 4)."""
+    Ts = np.ascontiguousarray(np.asarray(T, np.float64).reshape(-1, 16))
+    n = len(Ts) + 1
+    out = np.empty((n, 4, 4), np.float64)
+    tc = np.ascontiguousarray(np.asarray(T_close, np.float64).reshape(16)) \
+        if T_close is not None else None
+    _check(lib.lcr_chain_poses(n, _dp(Ts), _dp(tc) if tc is not None else None, _dp(out)))
+    return out
+
+
+def se3_pow(T, s):
+    """T^s = exp(s log T) for a rigid 4x4."""
+    M = np.ascontiguousarray(np.asarray(T, np.float64).reshape(16))
+    out = np.empty((4, 4), np.float64)
+    _check(lib.lcr_se3_pow(_dp(M), float(s), _dp(out)))
+    return out
+
+
+def centroid_align(src, tgt, roll=0.0, pitch=0.0, yaw=0.0, elevation=0.0):
+    """centroid_align's transform (centroid_align.cpp:104-130): source centroid to
+    the origin, Rx(roll) Ry(pitch) Rz(yaw) in degrees, onto the target centroid,
+    then `elevation` along y.  Returns the 4x4."""
+    s, t = _f32(src), _f32(tgt)
+    out = np.empty((4, 4), np.float64)
+    _check(lib.lcr_centroid_align(_fp(s), len(s), _fp(t), len(t), float(roll), float(pitch),
+                                  float(yaw), float(elevation), _dp(out)))
+    return out
 
 
 def _u8p(a):
@@ -520,6 +559,13 @@ class Cycle:
         ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
         _check(lib.lcr_cycle_instances(self._h, _dp(T), ip(it), ip(st), _dp(fit)))
         return dict(T=T, iterations=it, state=st, fitness=fit)
+
+    def initial_poses(self):
+        """(n_pairs * hypotheses, 4, 4): every instance's initial pose in the last
+        run, the learned one included."""
+        T = np.empty((self.n_pairs * self.n_total_hyp, 4, 4), np.float64)
+        _check(lib.lcr_cycle_initial_poses(self._h, _dp(T)))
+        return T
 
 
 ENC_NPARAM = 174470

@@ -8,7 +8,12 @@
 // capture 0, all in one lcr_cycle run.  Poses are chained as the script's
 // in-place rewrites do (alignment.sh:109-110 applies each pair's transform to all
 // later captures): capture j ends in the frame of capture 0 by
-// T_1 T_2 ... T_j, with T_i the transform of capture i onto i-1.
+// T_1 T_2 ... T_j, with T_i the transform of capture i onto i-1.  With --closing
+// the closing pair's loop residual is spread evenly over the cycle's edges
+// (lcr_chain_poses).  The top and bottom pairs start from centroid_align's
+// transform (pcl_tools/centroid_align.cpp:104-130) with the script's roll, pitch,
+// yaw and elevation (alignment.sh:117-118: top -90, 0, 180, +1; bottom 90, 0, 0,
+// -1; --top_seed / --bottom_seed change them).
 //
 // Output: OUT/<name>.ply (binary PLY, points and normals moved by the chained
 // pose), OUT/<name>.txt (the 4x4 in the row-major text the transform tool reads,
@@ -64,6 +69,16 @@ void check(int rc) {
   if (rc != LCR_OK) throw std::string("liblcr: ") + lcr_last_error();
 }
 
+// "roll,pitch,yaw,elevation"
+void parse_seed(const std::string& text, const std::string& flag, double* v) {
+  std::string c = text;
+  for (char& ch : c)
+    if (ch == ',') ch = ' ';
+  std::istringstream is(c);
+  if (!(is >> v[0] >> v[1] >> v[2] >> v[3]))
+    throw std::string(flag + " must be roll,pitch,yaw,elevation");
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
@@ -95,6 +110,10 @@ int main(int argc, char* argv[]) {
         .add("yaw_step", "Turntable step in degrees used as the initial pose (0 = identity)",
              true, "0", true)
         .add("centre", "Point on the turntable axis, x,y,z", true, "0,0,0", true)
+        .add("top_seed", "centroid_align roll,pitch,yaw,elevation of top.ply onto 0.ply", true,
+             "-90,0,180,1", true)
+        .add("bottom_seed", "centroid_align roll,pitch,yaw,elevation of bottom.ply onto 0.ply",
+             true, "90,0,0,-1", true)
         .add("precision", "Correspondence search: fp32, fp16 or bf16", true, "fp32", true);
     desc.parse(argc, argv);
     if (desc.count("help")) {
@@ -173,12 +192,22 @@ int main(int argc, char* argv[]) {
     const int npairs = (int)src.size();
     std::vector<double> guesses(16 * (size_t)npairs);
     const double yaw = desc.as<double>("yaw_step");
+    double seeds[2][4];
+    parse_seed(desc.str("top_seed"), "top_seed", seeds[0]);
+    parse_seed(desc.str("bottom_seed"), "bottom_seed", seeds[1]);
     for (int p = 0; p < npairs; ++p) {
-      const bool turntable_pair = src[p] < ncap;
-      if (yaw != 0.0 && turntable_pair)
-        turntable_prior(yaw, centre, &guesses[16 * (size_t)p]);
-      else
-        identity(&guesses[16 * (size_t)p]);
+      double* g = &guesses[16 * (size_t)p];
+      if (src[p] >= ncap) {
+        // top / bottom onto capture 0: centroid_align's transform
+        const double* v = seeds[src[p] - ncap];
+        check(lcr_centroid_align(xyz[src[p]].data(), (int64_t)clouds[src[p]].size(),
+                                 xyz[0].data(), (int64_t)clouds[0].size(), v[0], v[1], v[2], v[3],
+                                 g));
+      } else if (yaw != 0.0) {
+        turntable_prior(yaw, centre, g);
+      } else {
+        identity(g);
+      }
     }
 
     std::vector<const float*> xp(names.size()), np_(names.size());
@@ -228,10 +257,11 @@ int main(int argc, char* argv[]) {
     }
     lcr_cycle_destroy(cyc);
 
-    // chained poses into the frame of capture 0
+    // chained poses into the frame of capture 0, the closing pair's residual spread
+    // over the loop
     std::vector<double> pose(16 * names.size());
-    identity(&pose[0]);
-    for (int i = 1; i < ncap; ++i) mat_mul(&pose[16 * (i - 1)], &T[16 * (i - 1)], &pose[16 * i]);
+    check(lcr_chain_poses(ncap, T.data(), closing ? &T[16 * (size_t)(ncap - 1)] : nullptr,
+                          pose.data()));
     if (tb) {
       const int base = npairs - 2;
       mat_mul(&pose[0], &T[16 * base], &pose[16 * ncap]);

@@ -17,6 +17,7 @@
 
 #include "../../include/lcr.h"
 #include "runtime.hpp"
+#include "tune_cache.hpp"
 
 using namespace lcr;
 
@@ -168,7 +169,6 @@ struct lcr_icp {
 struct lcr_cycle {
   lcr_cycle_desc desc;
   Stream stream;
-  std::vector<std::vector<float>> xyz, nrm;
   std::vector<int32_t> src, tgt;
   std::vector<double> guesses;
   std::vector<float> vps;
@@ -612,6 +612,22 @@ int lcr_cycle_instances(lcr_cycle* c, double* T, int32_t* iterations, int32_t* s
   return LCR_OK;
 }
 
+int lcr_cycle_initial_poses(lcr_cycle* c, double* T) {
+  return guarded([&] {
+    if (!c || !T) return fail(LCR_E_ARG, "NULL argument");
+    if (!c->ran) return fail(LCR_E_STATE, "lcr_cycle_run has not run");
+    c->stream.bind();
+    // d_guess still holds the last run's poses (the proposal wrote its slot there)
+    const size_t n = c->states.size();
+    if (n > 0) {
+      LCR_HIP(hipMemcpyAsync(T, c->eng.d_guess.p, n * 16 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream.s));
+      LCR_HIP(hipStreamSynchronize(c->stream.s));
+    }
+    return LCR_OK;
+  });
+}
+
 int lcr_cycle_results(lcr_cycle* c, lcr_cycle_result* r) {
   if (!c || !r) return fail(LCR_E_ARG, "NULL argument");
   if (!c->ran) return fail(LCR_E_STATE, "lcr_cycle_run has not run");
@@ -865,40 +881,14 @@ int lcr_depth_to_cloud(const uint16_t* depth, int32_t width, int32_t height, con
 
 namespace {
 
-std::string tune_key(const lcr_cycle* c) {
+std::string cycle_tune_key(const lcr_cycle* c) {
   hipDeviceProp_t prop;
   std::string dev = "unknown";
   if (hipGetDeviceProperties(&prop, c->stream.device) == hipSuccess)
     dev = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount);
-  int64_t total = 0;
-  for (const auto& x : c->xyz) total += (int64_t)(x.size() / 3);
-  char buf[512];
-  snprintf(buf, sizeof buf, "dev=%s clouds=%d pairs=%d hyp=%d points=%.2e r=%.6g p2pl=%d k=%d",
-           dev.c_str(), c->desc.n_clouds, c->desc.n_pairs, c->nhyp_total, (double)total,
-           c->desc.params.max_corr_dist, c->desc.params.point_to_plane ? 1 : 0, c->desc.normals_k);
-  return buf;
-}
-
-bool tune_lookup(const char* path, const std::string& key, lcr_tune_result* r) {
-  FILE* f = fopen(path, "r");
-  if (!f) return false;
-  char line[1024];
-  bool found = false;
-  while (fgets(line, sizeof line, f)) {
-    char* tab = strchr(line, '\t');
-    if (!tab || std::string(line, tab - line) != key) continue;
-    int tile = 0, prec = 0;
-    double cf = 0.0;
-    if (sscanf(tab + 1, "%d %d %lf", &tile, &prec, &cf) == 3 &&
-        (tile == 64 || tile == 128 || tile == 256) && prec >= 0 && prec <= 2 && cf > 0.0) {
-      r->tile = tile;
-      r->precision_mode = prec;
-      r->cell_factor = cf;
-      found = true;  // the last entry for a key wins
-    }
-  }
-  fclose(f);
-  return found;
+  return tune_key(dev, c->desc.n_clouds, c->desc.n_pairs, c->nhyp_total, c->cs.total_raw,
+                  c->desc.params.max_corr_dist, c->desc.params.point_to_plane,
+                  c->desc.normals_k > 0 ? c->desc.normals_k : 30);
 }
 
 }  // namespace
@@ -911,8 +901,8 @@ int lcr_cycle_autotune(lcr_cycle* c, const char* cache_path, lcr_tune_result* ou
     lcr_tune_result r;
     memset(&r, 0, sizeof r);
     lcr_icp_params& P = c->desc.params;
-    const std::string key = tune_key(c);
-    if (cache_path && tune_lookup(cache_path, key, &r)) {
+    const std::string key = cycle_tune_key(c);
+    if (cache_path && tune_lookup_file(cache_path, key, &r.tile, &r.precision_mode, &r.cell_factor)) {
       P.tile = r.tile;
       P.precision_mode = r.precision_mode;
       P.cell_factor = r.cell_factor;

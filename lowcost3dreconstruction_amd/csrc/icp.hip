@@ -355,28 +355,33 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
   }
 }
 
-// Per pair: keep the hypothesis with the most source points within its radius
-// (ties: the smaller mean d2 of those, then the lower index), set the others aside.
+// Per pair: the first hypothesis (the plain seed) is always refined.  Of the
+// others, the one with the most source points within its radius (ties: the
+// smaller mean d2 of those, then the lower index) is refined as well when it has
+// more such points than the seed; every other hypothesis is set aside.  The
+// selection after the final fitness pass then picks the lower fitness of the
+// (at most two) refined poses, so a hypothesis that wins the inlier count at its
+// initial pose but ends in a worse basin never replaces the seed's refinement.
+__device__ __forceinline__ bool gate_better(const PairState& a, const PairState& b) {
+  return a.fit_count > b.fit_count || (a.fit_count == b.fit_count && a.fitness < b.fitness);
+}
+
 __global__ void k_gate(PairState* st, int npairs, int H) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
-  int best = -1;
-  for (int h = 0; h < H; ++h) {
+  int alt = -1;
+  for (int h = 1; h < H; ++h) {
     const PairState& s = st[p * H + h];
     if (!s.active) continue;
-    if (best < 0) {
-      best = h;
-      continue;
-    }
-    const PairState& b = st[p * H + best];
-    if (s.fit_count > b.fit_count || (s.fit_count == b.fit_count && s.fitness < b.fitness))
-      best = h;
+    if (alt < 0 || gate_better(s, st[p * H + alt])) alt = h;
   }
+  const PairState& s0 = st[p * H];
+  if (alt >= 0 && s0.active && !gate_better(st[p * H + alt], s0)) alt = -1;
   for (int h = 0; h < H; ++h) {
     PairState& s = st[p * H + h];
     s.fitness = DBL_MAX;
     s.fit_count = 0;
-    if (h != best && best >= 0) {
+    if (h != 0 && h != alt) {
       s.active = 0;
       s.gated = 1;
     }

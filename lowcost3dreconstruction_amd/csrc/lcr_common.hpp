@@ -93,11 +93,9 @@ struct IcpConst {
 };
 
 // The hypothesis gate scores each pose on every s-th work tile of the source,
-// s = clamp(ntiles / kGateMinTiles, 1, kGateStride): at most 1/32 of the points,
-// but at least kGateMinTiles tiles where the source has them.  32 tiles (2048
-// points) halve the gate's pass against 64 on 74 pairs x 15 hypotheses, with as
-// many pairs (61 of 74) within 0.05 deg / 0.2 mm of the oracle run from the
-// turntable prior.
+// s = clamp(ntiles / kGateMinTiles, 1, kGateStride): at least 1/32 of the tiles,
+// and at least kGateMinTiles tiles where the source has them.  32 tiles (2048
+// points) halve the gate's pass against 64 on 74 pairs x 15 hypotheses.
 constexpr int kGateStride = 32;
 constexpr int kGateMinTiles = 32;
 __host__ __device__ inline int gate_stride(int ntiles) {

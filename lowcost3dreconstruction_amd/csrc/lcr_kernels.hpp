@@ -120,7 +120,7 @@ int fitness_max_blocks_per_cu(int tile);
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
                     int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
-                    int* deferred, float4* nrm_sorted, hipStream_t s);
+                    int* deferred, uint8_t* flags, float4* nrm_sorted, hipStream_t s);
 
 // ---- K11-K13 filters before ICP (filters.hip)
 // VoxelGrid lattice: voxel of p = floor(p * inv) - minb (FP32, per axis); index

@@ -7,6 +7,7 @@
 #include "lcr_common.hpp"
 #include "lcr_math.hpp"
 #include "lcr_search.hpp"
+#include "tune_cache.hpp"
 
 #include <math.h>
 #include <algorithm>
@@ -172,5 +173,20 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
 
 // C = A * B (4x4 row-major)
 void lcrm_mat4_mul(const double* A, const double* B, double* C) { mat4_mul(A, B, C); }
+
+// The autotuner's cache key and lookup (tune_cache.hpp).  key: buffer of `len`
+// bytes; returns the key's length.
+int lcrm_tune_key(const char* device, int clouds, int pairs, int hyp, int64_t points, double r,
+                  int p2pl, int k, char* key, int len) {
+  const std::string s = tune_key(device, clouds, pairs, hyp, points, r, p2pl, k);
+  snprintf(key, (size_t)len, "%s", s.c_str());
+  return (int)s.size();
+}
+
+// 1 and the entry's (tile, precision_mode, cell_factor) when `key` is cached in
+// the file at `path` (the last valid entry wins), else 0.
+int lcrm_tune_lookup(const char* path, const char* key, int* tile, int* prec, double* cf) {
+  return tune_lookup_file(path, key, tile, prec, cf) ? 1 : 0;
+}
 
 }  // extern "C"

@@ -590,16 +590,183 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
   return true;
 }
 
+// ---------------------------------------------------------------- lane path
+//
+// One query per lane (64 neighbouring cell-sorted queries per wavefront, each
+// lane on its own): the candidates are the points of the 3x3x3 cells around the
+// query's cell, read from memory (the wavefront's queries share most of them,
+// so the reads hit the caches).  The kk-th smallest FLANN d2 is certified when
+// it lies below D2, the squared distance from the query to the block's
+// boundary (minus the rounding slack), so only d2 < D2 counts.  Pass 1 builds a
+// per-lane histogram of d2 over [0, D2) in kLaneBuckets equal buckets -- on a
+// surface the number of points within a radius grows with its square, so the
+// buckets fill about evenly -- and finds the bucket B that holds the kk-th.
+// Pass 2 lists every point of a lower bucket and keeps the (d2, original index)
+// of bucket B's points in LDS; a sorting network orders those and the first
+// kk - (count below B) complete the list.  Bucket index = floor(d2 * scale) is
+// monotone in d2, so the result is exactly the kk smallest (d2, original index).
+// A query goes to the staged wavefront path instead (flag set) when its block
+// covers the grid, holds fewer than kk points under D2 (not certified), more
+// than kLaneMaxCand candidates, or more than kLaneList points in bucket B.
+//
+// Against the staged path (the whole wavefront on one query: a 64-lane d2 scan
+// of the staged block, ballot counts and a bisection per query), each lane here
+// does the same d2 work for its own query and the selection costs a histogram
+// update per candidate: about 470 VALU and 320 SALU wavefront instructions per
+// query there.
+constexpr int kLaneBuckets = 32;
+constexpr int kLaneList = 8;
+constexpr int kLaneMaxCand = 4096;
+constexpr int kLaneThreads = 256;
+
+// Calls f(p) for every point of the 3x3x3 cells around cp (clipped to the grid),
+// four loads in flight at a time.
+template <class F>
+__device__ __forceinline__ void for_block1(const GridDesc& g, const int* cs, const float4* pts,
+                                           const CellPos& cp, F&& f) {
+  const int x0 = max(cp.cx - 1, 0), x1 = min(cp.cx + 1, g.nx - 1);
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cp.cz + dz;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cp.cy + dy;
+      if (z < 0 || z >= g.nz || y < 0 || y >= g.ny || x0 > x1) continue;
+      const int row = (z * g.ny + y) * g.nx;
+      const int s = cs[row + x0], e = cs[row + x1 + 1];
+      for (int i = s; i < e; i += 4) {
+        float4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p[u] = pts[min(i + u, e - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i + u < e) f(p[u]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b) {
+  const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+__global__ void __launch_bounds__(kLaneThreads)
+k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
+              const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
+              int64_t total, int k, int* __restrict__ nbr, int* __restrict__ nsel,
+              uint8_t* __restrict__ flags) {
+  // per lane: kLaneBuckets 16-bit counters, two per word; bucket B's entries
+  __shared__ uint32_t s_hist[kLaneThreads / 64][kLaneBuckets / 2][64];
+  __shared__ uint64_t s_list[kLaneThreads / 64][kLaneList][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = (int64_t)blockIdx.x * kLaneThreads + threadIdx.x;
+  if (j >= total) return;  // every lane works on its own LDS words: no barrier below
+  const float4 q = pts[j];
+  const GridDesc g = grids[cloud_of_sorted[j]];
+  const int kk = min(k, g.npts);
+  if (kk < 3) {
+    nsel[j] = kk;
+    flags[j] = 0;
+    return;
+  }
+  const int* cs = cell_start + g.cell_off;
+  const CellPos cp = cell_pos(g, q.x, q.y, q.z);
+  const float Lc = block_gap(g, cp, 1);
+  const float L = (Lc - cp.slack) * g.h;
+  // the staged path takes blocks that cover the grid (no boundary to certify
+  // against) and gaps inside the rounding slack
+  const float D2 = L * L * 0.99998f;
+  const float scale = (float)kLaneBuckets / D2;
+  // (a finite scale keeps every bucket index of a d2 < D2 inside [0, kLaneBuckets])
+  bool ok = Lc < INFINITY && L > 0.0f && D2 > 1e-30f && scale < INFINITY;
+  if (ok) {
+    int m = 0;
+    const int x0 = max(cp.cx - 1, 0), x1 = min(cp.cx + 1, g.nx - 1);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int z = cp.cz + dz, y = cp.cy + dy;
+        if (z < 0 || z >= g.nz || y < 0 || y >= g.ny || x0 > x1) continue;
+        const int row = (z * g.ny + y) * g.nx;
+        m += cs[row + x1 + 1] - cs[row + x0];
+      }
+    ok = m >= kk && m <= kLaneMaxCand;
+  }
+  int B = -1, below = 0, nB = 0;
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < kLaneBuckets / 2; ++c) s_hist[w][c][lane] = 0u;
+    for_block1(g, cs, pts, cp, [&](const float4& p) {
+      const float d = flann_d2(q.x, q.y, q.z, p);
+      if (d < D2) {
+        const int b = min((int)(d * scale), kLaneBuckets - 1);
+        atomicAdd(&s_hist[w][b >> 1][lane], 1u << ((b & 1) * 16));
+      }
+    });
+    int cum = 0;
+#pragma unroll
+    for (int c = 0; c < kLaneBuckets / 2; ++c) {
+      const uint32_t v = s_hist[w][c][lane];
+      const int lo = (int)(v & 0xffffu), hi = (int)(v >> 16);
+      if (B < 0 && cum + lo >= kk) {
+        B = 2 * c;
+        below = cum;
+        nB = lo;
+      } else if (B < 0 && cum + lo + hi >= kk) {
+        B = 2 * c + 1;
+        below = cum + lo;
+        nB = hi;
+      }
+      cum += lo + hi;
+    }
+    ok = B >= 0 && nB <= kLaneList;
+  }
+  if (!ok) {
+    flags[j] = 1;
+    return;
+  }
+  int* out = nbr + j * k;
+  int pos = 0, nl = 0;
+  for_block1(g, cs, pts, cp, [&](const float4& p) {
+    const float d = flann_d2(q.x, q.y, q.z, p);
+    if (d < D2) {
+      const int b = min((int)(d * scale), kLaneBuckets - 1);
+      if (b < B) {
+        out[pos++] = __float_as_int(p.w);
+      } else if (b == B && nl < kLaneList) {
+        s_list[w][nl++][lane] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)__float_as_int(p.w);
+      }
+    }
+  });
+  uint64_t e[kLaneList];
+#pragma unroll
+  for (int t = 0; t < kLaneList; ++t) e[t] = t < nl ? s_list[w][t][lane] : ~0ull;
+  // 19-comparator sorting network for 8 keys (d2 bits, original index)
+  cswap(e[0], e[2]); cswap(e[1], e[3]); cswap(e[4], e[6]); cswap(e[5], e[7]);
+  cswap(e[0], e[4]); cswap(e[1], e[5]); cswap(e[2], e[6]); cswap(e[3], e[7]);
+  cswap(e[0], e[1]); cswap(e[2], e[3]); cswap(e[4], e[5]); cswap(e[6], e[7]);
+  cswap(e[2], e[4]); cswap(e[3], e[5]);
+  cswap(e[1], e[4]); cswap(e[3], e[6]);
+  cswap(e[1], e[2]); cswap(e[3], e[4]); cswap(e[5], e[6]);
+  const int need = kk - below;
+#pragma unroll
+  for (int t = 0; t < kLaneList; ++t)
+    if (t < need) out[pos + t] = (int)(uint32_t)e[t];
+  nsel[j] = kk;
+  flags[j] = 0;
+}
+
 // Search pass, per point in cell-sorted order: the neighbour count and either
 // the original indices of its K nearest neighbours (kMoments: nbr holds k per
 // point, the moments are summed by the solve pass) or the FP64 sum of their
-// distances (kSqrtDist: mom holds 1 per point).
+// distances (kSqrtDist: mom holds 1 per point).  With flags (nullable) only the
+// points whose flag is set take part (the queries the lane path left).
 template <int MODE>
 __global__ void __launch_bounds__(kWaves * 64)
 k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
                  const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
                  int64_t total, int k, double* __restrict__ mom, int* __restrict__ nbr,
-                 int* __restrict__ nsel, int* __restrict__ deferred) {
+                 int* __restrict__ nsel, int* __restrict__ deferred,
+                 const uint8_t* __restrict__ flags) {
   __shared__ float4 s_p[kWaves][kCap];
   __shared__ int s_cb[kWaves][64];
   const int lane = threadIdx.x & 63;
@@ -609,6 +776,9 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
   const int64_t jq = base + lane;
   const bool own = jq < total;
   const int nq = (int)min<int64_t>(64, total - base);
+  const bool take = own && (!flags || flags[jq] != 0);
+  const uint64_t todo = __ballot(take);
+  if (todo == 0) return;
   Staged st;
   st.cell = -1;
   st.K = 0;
@@ -618,6 +788,7 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
   st.lastT = 0u;
   int mine_n = 0;
   for (int qi = 0; qi < nq; ++qi) {
+    if (!((todo >> qi) & 1ull)) continue;  // wave-uniform
     // wave-uniform addresses: scalar loads, so no vector load issued before the
     // loop stays pending into it (a pending one makes every iteration wait for
     // the previous iteration's stores)
@@ -641,7 +812,7 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
     }
     if (lane == qi) mine_n = kk;
   }
-  if (own) nsel[jq] = mine_n;
+  if (take) nsel[jq] = mine_n;
 }
 
 // Rescan pass for the deferred queries (candidate blocks too large for LDS), one
@@ -752,7 +923,7 @@ void launch_sor_distances(const GridDesc* grids, const int* cell_start, const fl
   launch_fill_i32(deferred, 1, 0, s);
   hipLaunchKernelGGL(k_normal_moments<kSqrtDist>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
                      cell_start, pts, cloud_of_sorted, npts_total, mean_k + 1, sums, (int*)nullptr, nsel,
-                     deferred);
+                     deferred, (const uint8_t*)nullptr);
   hipLaunchKernelGGL(k_normal_rescan<kSqrtDist>, dim3(1024), dim3(256), 0, s, grids, cell_start,
                      pts, cloud_of_sorted, mean_k + 1, deferred, sums, (int*)nullptr);
   hipLaunchKernelGGL(k_sor_dist, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s, pts,
@@ -762,14 +933,18 @@ void launch_sor_distances(const GridDesc* grids, const int* cell_start, const fl
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
                     int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
-                    int* deferred, float4* nrm_sorted, hipStream_t s) {
+                    int* deferred, uint8_t* flags, float4* nrm_sorted, hipStream_t s) {
   if (npts_total <= 0) return;
   const int64_t per_block = (int64_t)kWaves * 64;
   const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
   launch_fill_i32(deferred, 1, 0, s);
+  // the lane path first; the staged wavefront path takes the queries it flags
+  hipLaunchKernelGGL(k_normal_lane, dim3((unsigned)((npts_total + kLaneThreads - 1) / kLaneThreads)),
+                     dim3(kLaneThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, npts_total, k,
+                     nbr, nsel, flags);
   hipLaunchKernelGGL(k_normal_moments<kMoments>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
                      cell_start, pts, cloud_of_sorted, npts_total, k, (double*)nullptr, nbr, nsel,
-                     deferred);
+                     deferred, (const uint8_t*)flags);
   hipLaunchKernelGGL(k_normal_rescan<kMoments>, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
                      cloud_of_sorted, k, deferred, (double*)nullptr, nbr);
   hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s,

@@ -407,10 +407,11 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
   d_nbr.reserve(np * (size_t)k * sizeof(int));
   d_nsel.reserve(np * sizeof(int));
   d_defer.reserve((np + 1) * sizeof(int));
+  d_nflag.reserve(np);
   launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
                  d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
                  d_raw.as<float4>(), d_cloud_off.as<int64_t>(), d_nbr.as<int>(), d_nsel.as<int>(),
-                 d_defer.as<int>(), d_nrm.as<float4>(), s);
+                 d_defer.as<int>(), d_nflag.as<uint8_t>(), d_nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
 }

@@ -141,6 +141,7 @@ struct CloudSet {
   DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel, d_defer;
   DevBuf d_tpts, d_tpos, d_tkeys, d_tkeys_sorted, d_tvals, d_ttmp;  // tree order
   DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
+  DevBuf d_nflag;  // normals: 1 where the lane path left the query to the staged path
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
 
   // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;

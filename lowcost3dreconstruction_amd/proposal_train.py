@@ -5,8 +5,9 @@ random shape (a union of 3 ellipsoids, semi-axes 0.05-0.15 m, as the ellipsoid
 pair generator in synth.py), two independent 8192-point area-uniform samples with
 1 mm noise, and the source moved by an initial error of U[0, 45 deg] about a
 uniform axis plus a translation of length U[0, 0.1 m] in a uniform direction.  The
-label is the inverse motion (the transform ICP converges to from the truth on these
-noise-only pairs, checked by `label_check`).  Everything except the random shape and
+label is the inverse motion; `label_check` runs plain point-to-plane ICP from each
+label on its own pair and measures how far the converged plain-ICP transform (the
+label the configuration names) lies from it.  Everything except the random shape and
 motion parameters is generated, trained and evaluated on the GPU.
 
 Usage (on a GPU):  python -m lowcost3dreconstruction_amd.proposal_train --pairs 200000
@@ -130,12 +131,45 @@ def make_eval_pairs(lcr, prop, n, npts=8192, seed=1234, min_angle_deg=0.0, max_a
     return raw[:, 0], raw[:, 1], T_gt
 
 
+def label_check(lcr, prop, n=64, npts=8192, seed=777, max_corr_dist=0.01):
+    """The training labels are the inverse synthetic motions; the learned-proposal
+    configuration asks for converged plain-ICP transforms.  On `n` pairs drawn as
+    the training pairs are, plain point-to-plane ICP (30-NN normals on the device)
+    is run from each label on its own pair: the transform it converges to is the
+    plain-ICP label.  Returns per pair (rotation difference in degrees, translation
+    difference in metres) between the two labels, in the clouds' raw frame, the ICP
+    iteration counts, and the network's own error against the label (degrees,
+    metres; with the weights `prop` holds)."""
+    rng = np.random.default_rng(seed)
+    shapes, shares = random_shapes(rng, n)
+    motions = random_motions(rng, n)
+    raw, frame = prop.synthesize(shapes, shares, motions, npts, noise=0.001, seed=seed, raw=True)
+    pred, lab = prop.predict(0, n)
+
+    def rotvec(v):
+        a = float(np.linalg.norm(v))
+        return synth.rot_axis_angle(v, a) if a > 0 else np.eye(3)
+
+    net_rot = np.array([np.rad2deg(synth.rotation_error(rotvec(p[:3]), rotvec(q[:3])))
+                        for p, q in zip(pred.astype(np.float64), lab.astype(np.float64))])
+    net_tr = np.linalg.norm(pred[:, 3:] - lab[:, 3:], axis=1) * frame[:, 3]
+    T_lab = np.stack([np.linalg.inv(M) for M in motions])
+    clouds = [c for i in range(n) for c in (raw[i, 0], raw[i, 1])]
+    pairs = [(2 * i, 2 * i + 1) for i in range(n)]
+    c = lcr.Cycle(clouds, pairs, guesses=T_lab, point_to_plane=True, normals_k=30,
+                  max_corr_dist=max_corr_dist, max_iterations=60)
+    c.run()
+    r = c.results()
+    err = np.array([synth.transform_error(r["T"][i], T_lab[i]) for i in range(n)])
+    return np.rad2deg(err[:, 0]), err[:, 1], r["iterations"], net_rot, net_tr
+
+
 def ablation(lcr, prop, src, tgt, T_gt, max_corr_dist=0.1, max_iterations=60,
              ok_rot_deg=1.0, ok_trans=0.005, hyp_gate=False):
     """Plain point-to-plane ICP from the identity vs the same ICP with the learned
-    hypothesis added (both refined and the best by final fitness kept, or with
-    hyp_gate only the one with more inliers at its initial pose refined).  Returns
-    per-pair success, errors and iterations."""
+    hypothesis added (both refined and the best by final fitness kept; with hyp_gate
+    the learned pose is refined only when it has more inliers at its initial pose
+    than the identity).  Returns per-pair success, errors and iterations."""
     n = len(src)
     clouds = [c for pair in zip(src, tgt) for c in pair]
     pairs = [(2 * i, 2 * i + 1) for i in range(n)]

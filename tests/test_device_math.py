@@ -200,3 +200,39 @@ def test_grid_ring_search_equals_bruteforce(m, h, r):
             idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded, seeds, ring_limit)
             np.testing.assert_array_equal(idx, ridx)
             np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
+
+
+def test_tune_cache_key_and_lookup(m, tmp_path):
+    """The autotuner's cache: the key carries the workload's real point total, a
+    choice of any precision mode (the FP32-staged tiles, mode 3, included) is found
+    again, the last valid entry for a key wins and malformed or impossible entries
+    (a wavefront-tile mode at tile 128, an unknown mode) are skipped."""
+    m.lcrm_tune_key.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double,
+                                C.c_int, C.c_int, C.c_char_p, C.c_int]
+    m.lcrm_tune_lookup.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_double)]
+
+    def key(points, r=0.01):
+        buf = C.create_string_buffer(512)
+        n = m.lcrm_tune_key(b"gfx950:256", 36, 36, 2, points, r, 1, 30, buf, 512)
+        assert n == len(buf.value)
+        return buf.value
+
+    k = key(2467636)
+    assert b"points=2467636" in k and b"clouds=36" in k and b"hyp=2" in k
+    assert key(2467636) != key(2467637) and key(2467636) != key(2467636, r=0.02)
+    other = key(1000)
+    path = tmp_path / "tune.txt"
+    path.write_bytes(k + b"\t64 3 2.5 1.0\n" + other + b"\t128 0 3 1.0\n"
+                     + k + b"\t128 2 2.0 1.0\n" + k + b"\t64 7 2.0 1.0\n" + k + b"\tgarbage\n")
+    t, p, cf = C.c_int(), C.c_int(), C.c_double()
+    assert m.lcrm_tune_lookup(str(path).encode(), k, C.byref(t), C.byref(p), C.byref(cf)) == 1
+    assert (t.value, p.value, cf.value) == (64, 3, 2.5)  # later entries are invalid
+    with open(path, "ab") as f:
+        f.write(k + b"\t256 0 4 1.0\n")
+    assert m.lcrm_tune_lookup(str(path).encode(), k, C.byref(t), C.byref(p), C.byref(cf)) == 1
+    assert (t.value, p.value, cf.value) == (256, 0, 4.0)
+    assert m.lcrm_tune_lookup(str(path).encode(), key(5), C.byref(t), C.byref(p),
+                              C.byref(cf)) == 0
+    assert m.lcrm_tune_lookup(str(tmp_path / "none.txt").encode(), k, C.byref(t), C.byref(p),
+                              C.byref(cf)) == 0

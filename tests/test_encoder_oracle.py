@@ -70,6 +70,30 @@ def test_maxpool_ties_pick_lowest_index_and_relu_zero():
     assert feat[7] == 0.0 and arg[7] == 0
 
 
+def test_forward_matches_torch_fixture():
+    """The forward pass against torch's float64 linear / relu / max on seeded
+    pairs (tests/golden/make_encoder_golden.py), plain and with the device's
+    bfloat16 rounding points: outputs and features to 1e-12 relative (the FP64 sums
+    run in another order), argmax equal.  A transposed weight block, a dropped bias
+    or a rounding at the wrong point fails this."""
+    import importlib.util
+    import os
+
+    from conftest import GOLDEN
+    spec = importlib.util.spec_from_file_location(
+        "make_encoder_golden", os.path.join(GOLDEN, "make_encoder_golden.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    g = np.load(os.path.join(GOLDEN, "encoder.npz"), allow_pickle=False)
+    w = gen.weights(int(g["seed"]))
+    for q in (0, 1):
+        for i in range(len(g["src"])):
+            out, feat, arg = oracle.enc_forward(w, g["src"][i], g["tgt"][i], quantize=bool(q))
+            np.testing.assert_allclose(out, g[f"out_q{q}"][i], rtol=1e-12, atol=1e-13)
+            np.testing.assert_allclose(feat, g[f"feat_q{q}"][i], rtol=1e-12, atol=1e-13)
+            np.testing.assert_array_equal(arg, g[f"arg_q{q}"][i])
+
+
 def test_quantized_forward_close_to_fp64():
     rng = np.random.default_rng(2)
     w = init_weights(rng)

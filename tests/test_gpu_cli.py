@@ -128,37 +128,102 @@ def test_transform_points_matches_oracle(lcr):
 
 def test_align_cycle_cli_chains_poses(lcr, tmp_path):
     """alignment.sh's naming (0.ply .. N-1.ply, top.ply, bottom.ply): one batched
-    run, every pair equal to the Cycle binding, poses chained into capture 0's frame."""
-    cyc = synth.turntable_cycle(8, n_captures=4, yaw_step_deg=10.0, scale=1.5,
-                                centre=(0.0, 0.0, -0.8), top_bottom=True, closing_pair=False)
+    run, every pair equal to the Cycle binding from the same seeds (the turntable
+    prior; centroid_align's transform for top and bottom), poses chained into
+    capture 0's frame with the closing pair's residual spread over the loop."""
+    centre = (0.0, 0.0, -0.8)
+    cyc = synth.turntable_cycle(8, n_captures=4, yaw_step_deg=10.0, scale=1.5, centre=centre,
+                                top_bottom=True, closing_pair=True)
     names = ["0", "1", "2", "3", "top", "bottom"]
     for name, c in zip(names, cyc.clouds):
         write_ascii_ply(tmp_path / f"{name}.ply", c)
     out = tmp_path / "out"
     out.mkdir()
     print_case(test="cli_align_cycle")
+    # the synthetic top / bottom views are tilted by +-45 deg about the camera x axis
     r = subprocess.run([os.path.join(BIN, "align_cycle"), "-d", str(tmp_path), "-n", "4", "-o",
-                        str(out), "--top_bottom", "--point_to_plane", "--distance_threshold",
-                        "0.01", "--max_iterations", "60", "--yaw_step", "10", "--centre",
-                        "0,0,-0.8"], capture_output=True, text=True, timeout=300)
+                        str(out), "--top_bottom", "--closing", "--point_to_plane",
+                        "--distance_threshold", "0.01", "--max_iterations", "60", "--yaw_step",
+                        "10", "--centre", "0,0,-0.8", "--top_seed", "-45,0,0,0", "--bottom_seed",
+                        "45,0,0,0"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert r.stdout.count("Has converged:") == len(cyc.pairs)
-    guesses = np.stack([synth.turntable_prior(10.0, (0.0, 0.0, -0.8))] * 3 +
-                       [np.eye(4), np.eye(4)])
+    prior = synth.turntable_prior(10.0, centre)
+    guesses = np.stack([prior] * 4 + [lcr.centroid_align(cyc.clouds[4], cyc.clouds[0], -45, 0, 0, 0),
+                                      lcr.centroid_align(cyc.clouds[5], cyc.clouds[0], 45, 0, 0, 0)])
     ref = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=guesses, point_to_plane=True,
                     max_corr_dist=0.01, max_iterations=60, normals_k=30)
     ref.run()
     T = ref.results()["T"]
-    pose = {"0": np.eye(4)}
-    for i in (1, 2, 3):
-        pose[str(i)] = pose[str(i - 1)] @ T[i - 1]
-    pose["top"], pose["bottom"] = T[3], T[4]
+    chain = lcr.chain_poses(T[:3], T[3])
+    pose = {str(i): chain[i] for i in range(4)}
+    pose["top"], pose["bottom"] = T[4], T[5]
     for name, c in zip(names, cyc.clouds):
         M = np.loadtxt(out / f"{name}.txt")
         np.testing.assert_allclose(M, pose[name], rtol=0, atol=1e-12)
         got, _ = read_binary_ply(out / f"{name}.ply")
         exp = oracle.transform_points(c, M)
         np.testing.assert_array_equal(got[:, :3].astype(np.float32), exp.astype(np.float32))
+    # the turntable captures against the true motion: capture j in capture 0's frame
+    # is poses[0] poses[j]^-1 (top and bottom: the seed test below; centroids of
+    # differently cropped partial views are too far apart for a 1 cm radius)
+    for i, name in enumerate(names[:4]):
+        truth = cyc.poses[0] @ np.linalg.inv(cyc.poses[i])
+        er, et = synth.transform_error(pose[name], truth)
+        assert er < np.deg2rad(0.5) and et < 3e-3, (name, np.rad2deg(er), et)
+
+
+def test_align_cycle_cli_top_bottom_seed_recovers_true_pose(tmp_path):
+    """Top and bottom views held the way alignment.sh:117-118 expects (turned by the
+    script's roll / yaw about their centroid): centroid_align seeds with the
+    script's angles (-90,0,180 and 90,0,0; elevation 0 here, as every view samples
+    the whole surface) bring them within reach of ICP, which recovers the true
+    pose, not just the binding's.  Moved onto the centroid without the rotation,
+    ICP cannot."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(21)
+    shape = synth.make_shape(rng)
+
+    def sample():
+        return synth.sample_surface(shape, 20000, rng) + rng.normal(scale=1e-3, size=(20000, 3))
+
+    cap0 = sample()
+    c = cap0.mean(0)
+    yaw = synth.turntable_prior(10.0, c)
+    views = {"0": cap0, "1": synth.apply(np.linalg.inv(yaw), sample())}
+    truth = {"0": np.eye(4), "1": yaw}
+    for name, (roll, ryaw), d in (("top", (-90.0, 180.0), (2.0, 0.003)),
+                                  ("bottom", (90.0, 0.0), (-2.5, -0.002))):
+        R = Rotation.from_euler("XYZ", [roll, 0.0, ryaw], degrees=True).as_matrix()
+        delta = synth.rigid(synth.rot_axis_angle([1, 1, 0], np.deg2rad(d[0])), [d[1], 0, d[1]])
+        C = synth.rigid(np.eye(3), c)
+        T_gt = C @ delta @ synth.rigid(R, np.zeros(3)) @ np.linalg.inv(C)
+        views[name] = synth.apply(np.linalg.inv(T_gt), sample())
+        truth[name] = T_gt
+    for name, v in views.items():
+        write_ascii_ply(tmp_path / f"{name}.ply", v.astype(np.float32))
+    print_case(test="cli_top_bottom_seed")
+    for seeded in (True, False):
+        out = tmp_path / ("out_seeded" if seeded else "out_identity")
+        out.mkdir()
+        extra = (["--top_seed", "-90,0,180,0", "--bottom_seed", "90,0,0,0"] if seeded else
+                 ["--top_seed", "0,0,0,0", "--bottom_seed", "0,0,0,0"])
+        # run to convergence (the CLI's default 1e-3 relative MSE stop leaves a
+        # slowly sliding rotation a few tenths of a degree short)
+        r = subprocess.run([os.path.join(BIN, "align_cycle"), "-d", str(tmp_path), "-n", "2",
+                            "-o", str(out), "--top_bottom", "--distance_threshold", "0.05",
+                            "--max_iterations", "300", "--euclidean_fitness_epsilon", "1e-9",
+                            "--transformation_epsilon", "1e-14", "--yaw_step", "10", "--centre",
+                            ",".join(f"{x:.9g}" for x in c)] + extra,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        for name in ("1", "top", "bottom"):
+            er, et = synth.transform_error(np.loadtxt(out / f"{name}.txt"), truth[name])
+            print(f"seeded={seeded} {name}: {np.rad2deg(er):.4f} deg {et * 1e3:.4f} mm")
+            if seeded:
+                assert er < np.deg2rad(0.2) and et < 1e-3, (name, np.rad2deg(er), et)
+            elif name != "1":
+                assert er > np.deg2rad(10), (name, np.rad2deg(er))
 
 
 def test_cloud_downsampling_and_outlier_removal_cli(tmp_path):
