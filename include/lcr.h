@@ -267,6 +267,25 @@ int lcr_depth_to_cloud(const uint16_t *depth, int32_t width, int32_t height, con
                        const int32_t crop[4], float *out_xyz, float *out_nrm, uint8_t *out_rgb,
                        int64_t *out_n);
 
+/* lcr_depth_to_cloud for callers that cannot allocate device memory.  No
+ * counterpart in the reference: it is for code that may not call hipMalloc --
+ * work under HIP graph capture, a per-frame capture loop, harnesses that forbid
+ * allocation.  Same computation and outputs as lcr_depth_to_cloud, but every
+ * array is a DEVICE pointer (depth, color, out_xyz, out_nrm, out_rgb; color,
+ * out_nrm and out_rgb nullable), *out_n is written on the device (one int64), and
+ * the work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * without waiting for it; intrinsics and crop stay host arrays.  Nothing is
+ * allocated: the kernels' scratch is a fixed array of 8 MiB per device, taken by
+ * lcr_init or on the first call on a device.  width * height must be at most
+ * LCR_NOALLOC_MAX_PIXELS (else LCR_E_LIMIT).  Every call on a device shares its
+ * scratch, so calls that overlap on one device (other streams, other threads)
+ * race and must be serialised by the caller. */
+#define LCR_NOALLOC_MAX_PIXELS (1 << 20)
+int lcr_depth_to_cloud_noalloc(const uint16_t *depth, int32_t width, int32_t height,
+                               const uint8_t *color, const double intrinsics[4], int32_t depth_min,
+                               int32_t depth_max, const int32_t crop[4], float *out_xyz,
+                               float *out_nrm, uint8_t *out_rgb, int64_t *out_n, void *stream);
+
 /* One-shot: create, run, results, destroy. */
 int lcr_icp_batch_align(const lcr_cycle_desc *d, lcr_cycle_result *r);
 

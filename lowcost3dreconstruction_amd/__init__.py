@@ -47,7 +47,7 @@ EXPORTS = (
     "lcr_proposal_get_weights", "lcr_proposal_forward", "lcr_proposal_loss_grad",
     "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
     "lcr_proposal_predict", "lcr_cycle_autotune", "lcr_transform_points", "lcr_voxel_grid",
-    "lcr_outlier_removal", "lcr_depth_to_cloud",
+    "lcr_outlier_removal", "lcr_depth_to_cloud", "lcr_depth_to_cloud_noalloc",
     "lcr_chain_poses", "lcr_se3_pow", "lcr_centroid_align",
 )
 
@@ -170,6 +170,9 @@ def _load():
     L.lcr_outlier_removal.argtypes = [fp, C.c_int64, C.c_int32, C.c_double, u8, fp, dp, i64p]
     L.lcr_depth_to_cloud.argtypes = [C.POINTER(C.c_uint16), C.c_int32, C.c_int32, u8, dp,
                                      C.c_int32, C.c_int32, C.POINTER(C.c_int32), fp, fp, u8, i64p]
+    L.lcr_depth_to_cloud_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, dp, C.c_int32,
+                                             C.c_int32, C.POINTER(C.c_int32), vp, vp, vp, vp,
+                                             vp]
     L.lcr_chain_poses.argtypes = [C.c_int32, dp, dp, dp]
     L.lcr_se3_pow.argtypes = [dp, C.c_double, dp]
     L.lcr_centroid_align.argtypes = [fp, C.c_int64, fp, C.c_int64] + [C.c_double] * 4 + [dp]

@@ -203,6 +203,7 @@ int lcr_init(int device) {
     if (device < 0 || device >= n) return fail(LCR_E_ARG, "no such HIP device");
     LCR_HIP(hipSetDevice(device));
     g_device = device;
+    noalloc_init(device);
     return LCR_OK;
   });
 }

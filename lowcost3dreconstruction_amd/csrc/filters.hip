@@ -249,6 +249,11 @@ __global__ void k_depth_points(const uint16_t* __restrict__ depth, int w, int h,
   }
 }
 
+__global__ void k_depth_count(const int* __restrict__ pos, const int* __restrict__ flags, int64_t n,
+                              int64_t* __restrict__ out_n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out_n = (int64_t)pos[n - 1] + flags[n - 1];
+}
+
 }  // namespace
 
 void launch_voxel_grid(const float4* raw, const float4* raw_nrm, const float* curv,
@@ -303,6 +308,11 @@ void launch_depth_points(const uint16_t* depth, int w, int h, const uint8_t* col
   if (np <= 0) return;
   hipLaunchKernelGGL(k_depth_points, dim3(blocks_of(np, 256)), dim3(256), 0, s, depth, w, h, color,
                      P, pos, out_xyz, out_nrm, out_rgb);
+}
+
+void launch_depth_count(const int* pos, const int* flags, int64_t n, int64_t* out_n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_depth_count, dim3(1), dim3(64), 0, s, pos, flags, n, out_n);
 }
 
 }  // namespace lcr

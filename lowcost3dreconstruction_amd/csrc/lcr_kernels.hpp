@@ -163,6 +163,8 @@ void launch_depth_flags(const uint16_t* depth, int w, int h, const DepthParams& 
 void launch_depth_points(const uint16_t* depth, int w, int h, const uint8_t* color,
                          const DepthParams& P, const int* pos, float* out_xyz, float* out_nrm,
                          uint8_t* out_rgb, hipStream_t s);
+// *out_n = pos[n - 1] + flags[n - 1]: the number of points (n = w * h > 0)
+void launch_depth_count(const int* pos, const int* flags, int64_t n, int64_t* out_n, hipStream_t s);
 
 // StatisticalOutlierRemoval distances (normals.hip's selection with mean_k + 1
 // neighbours): dist / finite per input point (the caller zeroes both first);

@@ -221,6 +221,10 @@ int device_cu_count();
 // Records the message returned by lcr_last_error() on this thread; returns code.
 int set_last_error(int code, const std::string& msg);
 
+// Takes the address of the fixed scratch array of lcr_depth_to_cloud_noalloc on
+// `device`, which must be current (noalloc.hip).
+void noalloc_init(int device);
+
 // The pose proposal behind an lcr_proposal handle, and its per-cycle inference:
 // writes each pair's learned guess into slot `slot` of d_guess (pair-major,
 // `stride` slots per pair, slot 0 = the plain seed), on stream s.
