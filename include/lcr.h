@@ -267,6 +267,18 @@ int lcr_depth_to_cloud(const uint16_t *depth, int32_t width, int32_t height, con
                        const int32_t crop[4], float *out_xyz, float *out_nrm, uint8_t *out_rgb,
                        int64_t *out_n);
 
+/* Burst depth fusion (capture/super_resolution/super_resolution.cpp:283-314) for
+ * frames already aligned (the ECC registration and warp of :255-279 are the
+ * identity for a static rig): each frame upsampled by `factor` with nearest
+ * neighbours, the zero-eliminating mean per pixel (the mean of the positive
+ * depths, in double), the Lanczos-4 resize back to width x height (source position
+ * (x + 0.5) factor - 0.5, 8 taps per axis clamped to the image, horizontal then
+ * vertical, float) and the 16-bit conversion (round to nearest even, clamped, NaN
+ * -> 0).  frames: n_frames x height x width uint16 millimetres, row-major; out:
+ * height x width.  factor in 1..16. */
+int lcr_depth_fuse(const uint16_t *frames, int32_t n_frames, int32_t width, int32_t height,
+                   int32_t factor, uint16_t *out);
+
 /* lcr_depth_to_cloud for callers that cannot allocate device memory.  No
  * counterpart in the reference: it is for code that may not call hipMalloc --
  * work under HIP graph capture, a per-frame capture loop, harnesses that forbid

@@ -48,7 +48,7 @@ EXPORTS = (
     "lcr_proposal_dataset", "lcr_proposal_train", "lcr_proposal_synthesize",
     "lcr_proposal_predict", "lcr_cycle_autotune", "lcr_transform_points", "lcr_voxel_grid",
     "lcr_outlier_removal", "lcr_depth_to_cloud", "lcr_depth_to_cloud_noalloc",
-    "lcr_chain_poses", "lcr_se3_pow", "lcr_centroid_align",
+    "lcr_chain_poses", "lcr_se3_pow", "lcr_centroid_align", "lcr_depth_fuse",
 )
 
 
@@ -173,6 +173,8 @@ def _load():
     L.lcr_depth_to_cloud_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, dp, C.c_int32,
                                              C.c_int32, C.POINTER(C.c_int32), vp, vp, vp, vp,
                                              vp]
+    L.lcr_depth_fuse.argtypes = [C.POINTER(C.c_uint16), C.c_int32, C.c_int32, C.c_int32,
+                                 C.c_int32, C.POINTER(C.c_uint16)]
     L.lcr_chain_poses.argtypes = [C.c_int32, dp, dp, dp]
     L.lcr_se3_pow.argtypes = [dp, C.c_double, dp]
     L.lcr_centroid_align.argtypes = [fp, C.c_int64, fp, C.c_int64] + [C.c_double] * 4 + [dp]
@@ -338,6 +340,18 @@ def transform_points(xyz, T, nrm=None):
     _check(lib.lcr_transform_points(_fp(x), _fp(n) if n is not None else None, len(x), 12, _dp(M),
                                     _fp(out), _fp(out_n) if out_n is not None else None))
     return out, out_n
+
+
+def depth_fuse(frames, factor=2):
+    """Burst depth fusion of super_resolution.cpp:283-314 on the device, for aligned
+    frames: nearest upsampling by `factor`, zero-eliminating mean, Lanczos-4 resize
+    back, 16 bits.  frames: (K, H, W) uint16 millimetres.  Returns (H, W) uint16."""
+    f = np.ascontiguousarray(frames, dtype=np.uint16)
+    k, h, w = f.shape
+    out = np.empty((h, w), np.uint16)
+    u16 = C.POINTER(C.c_uint16)
+    _check(lib.lcr_depth_fuse(f.ctypes.data_as(u16), k, w, h, int(factor), out.ctypes.data_as(u16)))
+    return out
 
 
 def chain_poses(T, T_close=None):

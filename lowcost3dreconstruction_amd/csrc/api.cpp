@@ -85,12 +85,15 @@ bool params_ok(const lcr_icp_params* p, std::string* why) {
   return true;
 }
 
-// Tile order of the persistent kernels: grid stride unless LCR_TILE_ORDER=xcd
-// asks for XCD-contiguous ranges (read once).
+// Tile order of the ICP search launches: grid stride (1) unless LCR_TILE_ORDER
+// asks for XCD-contiguous ranges ("xcd", 0) or the switch between them by the
+// live-tile count ("auto", 2) (read once).
 int tile_order_interleave() {
   static const int v = [] {
     const char* e = getenv("LCR_TILE_ORDER");
-    return (e && strcmp(e, "xcd") == 0) ? 0 : 1;
+    if (e && strcmp(e, "xcd") == 0) return 0;
+    if (e && strcmp(e, "auto") == 0) return 2;
+    return 1;
   }();
   return v;
 }

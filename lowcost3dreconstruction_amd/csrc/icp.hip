@@ -156,14 +156,22 @@ __device__ __forceinline__ void tile_moments(bool has, float qx, float qy, float
 // cycle's iterations than the unconstrained five (87 VGPRs), six 9 %, eight
 // (64 VGPRs, more spills) 8 %: the search is bound by the latency of its
 // dependent loads, and more resident waves hide more of it.
-template <bool P2PL, int TILE, int PREC>
+//
+// RING: every instance's radius spans at most kRingOnlyCells target cells
+// (IcpConst::ring_only), so the bounded search ends inside the ring loop and the
+// tree walk is never needed; this variant carries none of its code or registers.
+template <bool P2PL, int TILE, int PREC, bool RING = false>
 __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const IcpConst c) {
   constexpr int NV = P2PL ? kP2LVals : kP2PVals;
   static_assert(PREC == 0 || TILE == 64, "distance tiles need one wavefront per tile");
   static_assert(TILE >= NV, "one thread per value writes the tile's sums");
   __shared__ double lds[NV * (TILE / 64)];
   const int nt = b.it_count ? *b.it_count : b.ntiles;
-  const TileRange tr = xcd_tiles(nt, c.interleave);
+  // order 2: XCD-contiguous while at least half of the tiles are live (each
+  // XCD's L2 then holds the targets of its own eighth of the pairs), grid stride
+  // once fewer instances are left (they would crowd a few XCDs)
+  const int inter = c.interleave == 2 ? (2 * nt < b.ntiles ? 1 : 0) : c.interleave;
+  const TileRange tr = xcd_tiles(nt, inter);
   for (int it = tr.first; it < tr.end; it += tr.step) {
     const int2 e = b.it_list ? b.it_list[it] : make_int2(it, b.tile_pair[it]);
     const int tile = e.x, p = e.y;
@@ -200,15 +208,21 @@ __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const 
         if (nn.j >= 0) seed = nn.j;
       }
     }
-    if (search)
-      nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, bounded, seed,
-                    2, true);
-    const float rlim = r * 1.00001f;
-    // tree walks happen only for radii of more than kRingOnlyCells cells: kept out
-    // of the hot path's code layout
-    if (__builtin_expect(__ballot(nn.tree) != 0, 0))
-      bvh_nn_wave(gt, b.tree(), nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
-                  nn.id, nn.j);
+    if constexpr (RING) {
+      if (search)
+        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, Tree{nullptr, nullptr, nullptr}, qx,
+                      qy, qz, r, bounded, seed, 2, true);
+    } else {
+      if (search)
+        nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, bounded,
+                      seed, 2, true);
+      const float rlim = r * 1.00001f;
+      // tree walks happen only for radii of more than kRingOnlyCells cells: kept out
+      // of the hot path's code layout
+      if (__builtin_expect(__ballot(nn.tree) != 0, 0))
+        bvh_nn_wave(gt, b.tree(), nn.tree, qx, qy, qz, bounded ? rlim * rlim : INFINITY, nn.d2,
+                    nn.id, nn.j);
+    }
     bool has = false;
     float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
     if (own) {
@@ -557,7 +571,18 @@ __global__ void k_transform_points(const float4* raw, const float4* raw_nrm, int
 
 template <bool P2PL, int TILE>
 void launch_iter_t(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((k_icp_iter<P2PL, TILE, 0>), dim3(nblocks), dim3(TILE), 0, s, b, c);
+  if (c.ring_only)
+    hipLaunchKernelGGL((k_icp_iter<P2PL, TILE, 0, true>), dim3(nblocks), dim3(TILE), 0, s, b, c);
+  else
+    hipLaunchKernelGGL((k_icp_iter<P2PL, TILE, 0>), dim3(nblocks), dim3(TILE), 0, s, b, c);
+}
+
+template <bool P2PL, int TILE, bool RING>
+int iter_blocks_per_cu() {
+  int nb = 0;
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<P2PL, TILE, 0, RING>, TILE, 0);
+  return (e != hipSuccess || nb < 1) ? 1 : nb;
 }
 
 template <int PREC>
@@ -670,9 +695,19 @@ void launch_transform_points(const float4* raw, const float4* raw_nrm, int64_t n
                      raw_nrm, n, M, out_xyz, out_nrm);
 }
 
-int icp_max_blocks_per_cu(int tile, bool p2pl, int precision) {
+int icp_max_blocks_per_cu(int tile, bool p2pl, int precision, bool ring_only) {
   int nb = 0;
   hipError_t e = hipSuccess;
+  if (ring_only && precision == 0) {
+    switch (tile) {
+      case 64:
+        return p2pl ? iter_blocks_per_cu<true, 64, true>() : iter_blocks_per_cu<false, 64, true>();
+      case 128:
+        return p2pl ? iter_blocks_per_cu<true, 128, true>() : iter_blocks_per_cu<false, 128, true>();
+      default:
+        return p2pl ? iter_blocks_per_cu<true, 256, true>() : iter_blocks_per_cu<false, 256, true>();
+    }
+  }
   if (precision >= 1 && precision <= 3) {
     if (precision == 1)
       e = p2pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp_iter<true, 64, 1>, 64, 0)

@@ -89,7 +89,9 @@ struct IcpConst {
   double mse_rel;                // euclidean_fitness_epsilon
   double mse_abs;                // 1e-12
   int precision;                 // correspondence search: 0 FP32, 1 FP16 / 2 BF16 tiles
-  int interleave;                // tile order: 0 XCD-contiguous, 1 grid stride
+  int interleave;                // tile order: 0 XCD-contiguous, 1 grid stride, 2 by live tiles
+  int ring_only;                 // 1: every instance's radius spans at most kRingOnlyCells
+                                 // target cells, so the search never needs the tree
 };
 
 // The hypothesis gate scores each pose on every s-th work tile of the source,

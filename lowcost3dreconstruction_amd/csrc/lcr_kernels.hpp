@@ -112,15 +112,17 @@ void launch_transform_output(const float4* raw, const float4* raw_nrm, int64_t n
 // x' = T x (xyz, FP64, rounded once) and n' = R n for a host-given row-major T.
 void launch_transform_points(const float4* raw, const float4* raw_nrm, int64_t n, const double* T,
                              float* out_xyz, float* out_nrm, hipStream_t s);
-int icp_max_blocks_per_cu(int tile, bool p2pl, int precision);
+int icp_max_blocks_per_cu(int tile, bool p2pl, int precision, bool ring_only = false);
 // Resident blocks per CU of the fitness pass at `tile` points per block.
 int fitness_max_blocks_per_cu(int tile);
 
 // ---- K7 normals (normals.hip)
+// Ints of the normals pass's workspace for n points (its two query lists).
+int64_t normals_work_ints(int64_t n);
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
                     int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
-                    int* deferred, uint8_t* flags, float4* nrm_sorted, hipStream_t s);
+                    int* work, float4* nrm_sorted, hipStream_t s);
 
 // ---- K11-K13 filters before ICP (filters.hip)
 // VoxelGrid lattice: voxel of p = floor(p * inv) - minb (FP32, per axis); index

@@ -593,119 +593,172 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
 // ---------------------------------------------------------------- lane path
 //
 // One query per lane (64 neighbouring cell-sorted queries per wavefront, each
-// lane on its own): the candidates are the points of the 3x3x3 cells around the
-// query's cell, read from memory (the wavefront's queries share most of them,
-// so the reads hit the caches).  The kk-th smallest FLANN d2 is certified when
-// it lies below D2, the squared distance from the query to the block's
+// lane on its own): the candidates are the points of the (2K+1)^3 cells around
+// the query's cell, read from memory (the wavefront's queries share most of
+// them, so the reads hit the caches).  The kk-th smallest FLANN d2 is certified
+// when it lies below D2, the squared distance from the query to the block's
 // boundary (minus the rounding slack), so only d2 < D2 counts.  Pass 1 builds a
-// per-lane histogram of d2 over [0, D2) in kLaneBuckets equal buckets -- on a
-// surface the number of points within a radius grows with its square, so the
-// buckets fill about evenly -- and finds the bucket B that holds the kk-th.
-// Pass 2 lists every point of a lower bucket and keeps the (d2, original index)
-// of bucket B's points in LDS; a sorting network orders those and the first
-// kk - (count below B) complete the list.  Bucket index = floor(d2 * scale) is
-// monotone in d2, so the result is exactly the kk smallest (d2, original index).
-// A query goes to the staged wavefront path instead (flag set) when its block
+// per-lane histogram of d2 over [0, D2) in NB equal buckets -- on a surface the
+// number of points within a radius grows with its square, so the buckets fill
+// about evenly -- and finds the bucket B that holds the kk-th.  Pass 2 lists
+// every point of a lower bucket and keeps (d2, original index, position) of
+// bucket B's points in LDS; a sorting network orders those by (d2, original
+// index) and the first kk - (count below B) complete the list.  Bucket index =
+// floor(d2 * scale) is monotone in d2, so the result is exactly the kk smallest
+// (d2, original index).  A query is flagged for the next path when its block
 // covers the grid, holds fewer than kk points under D2 (not certified), more
-// than kLaneMaxCand candidates, or more than kLaneList points in bucket B.
+// than kLaneMaxCand candidates, or more than NL points in bucket B.
 //
-// Against the staged path (the whole wavefront on one query: a 64-lane d2 scan
-// of the staged block, ballot counts and a bisection per query), each lane here
-// does the same d2 work for its own query and the selection costs a histogram
-// update per candidate: about 470 VALU and 320 SALU wavefront instructions per
-// query there.
-constexpr int kLaneBuckets = 32;
-constexpr int kLaneList = 8;
+// The lane pass runs K = 1 (3x3x3 cells) over every point and appends the
+// points it flags (a few percent: near crop edges and silhouettes the 3x3x3
+// block holds too few; dense spots overflow bucket B) to a list, which the
+// wavefront pass below serves.
 constexpr int kLaneMaxCand = 4096;
 constexpr int kLaneThreads = 256;
 
-// Calls f(p) for every point of the 3x3x3 cells around cp (clipped to the grid),
-// four loads in flight at a time.
-template <class F>
-__device__ __forceinline__ void for_block1(const GridDesc& g, const int* cs, const float4* pts,
-                                           const CellPos& cp, F&& f) {
-  const int x0 = max(cp.cx - 1, 0), x1 = min(cp.cx + 1, g.nx - 1);
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cp.cz + dz;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cp.cy + dy;
-      if (z < 0 || z >= g.nz || y < 0 || y >= g.ny || x0 > x1) continue;
-      const int row = (z * g.ny + y) * g.nx;
-      const int s = cs[row + x0], e = cs[row + x1 + 1];
-      for (int i = s; i < e; i += 4) {
-        float4 p[4];
+// Sorted-array start and length of the (2K+1)^2 x-runs ((2K+1) cells each, clipped
+// to the grid) of the block of radius K around cp: the cell-table loads are
+// issued together.
+template <int K>
+struct BlockRuns {
+  static constexpr int R = (2 * K + 1) * (2 * K + 1);
+  int s[R], n[R];
+};
+
+template <int K>
+__device__ __forceinline__ BlockRuns<K> block_runs(const GridDesc& g, const int* cs, const CellPos& cp) {
+  BlockRuns<K> b;
+  const int x0 = max(cp.cx - K, 0), x1 = min(cp.cx + K, g.nx - 1);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) p[u] = pts[min(i + u, e - 1)];
+  for (int r = 0; r < BlockRuns<K>::R; ++r) {
+    const int z = cp.cz + r / (2 * K + 1) - K, y = cp.cy + r % (2 * K + 1) - K;
+    const bool in = z >= 0 && z < g.nz && y >= 0 && y < g.ny && x0 <= x1;
+    const int row = in ? (z * g.ny + y) * g.nx : 0;
+    const int s = in ? cs[row + x0] : 0;
+    const int e = in ? cs[row + x1 + 1] : 0;
+    b.s[r] = s;
+    b.n[r] = e - s;
+  }
+  return b;
+}
+
+// Calls f(p, i) for every point i of the block's runs, four loads in flight.
+template <int K, class F>
+__device__ __forceinline__ void for_block(const BlockRuns<K>& b, const float4* pts, F&& f) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (i + u < e) f(p[u]);
-      }
+  for (int r = 0; r < BlockRuns<K>::R; ++r) {
+    const int s = b.s[r], e = s + b.n[r];
+    for (int i = s; i < e; i += 4) {
+      float4 p[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) p[u] = pts[min(i + u, e - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i + u < e) f(p[u], i + u);
     }
   }
 }
 
-__device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b) {
-  const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+// (key, payload) compare-exchange: smaller key first.
+__device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b, int& pa, int& pb) {
+  const bool sw = b < a;
+  const uint64_t lo = sw ? b : a, hi = sw ? a : b;
+  const int plo = sw ? pb : pa, phi = sw ? pa : pb;
   a = lo;
   b = hi;
+  pa = plo;
+  pb = phi;
 }
 
-__global__ void __launch_bounds__(kLaneThreads)
-k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
-              const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
-              int64_t total, int k, int* __restrict__ nbr, int* __restrict__ nsel,
-              uint8_t* __restrict__ flags) {
-  // per lane: kLaneBuckets 16-bit counters, two per word; bucket B's entries
-  __shared__ uint32_t s_hist[kLaneThreads / 64][kLaneBuckets / 2][64];
-  __shared__ uint64_t s_list[kLaneThreads / 64][kLaneList][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t j = (int64_t)blockIdx.x * kLaneThreads + threadIdx.x;
-  if (j >= total) return;  // every lane works on its own LDS words: no barrier below
+// Sorts NL (key, payload) pairs: the 19-comparator network for 8, odd-even
+// transposition (NL rounds) otherwise.
+template <int NL>
+__device__ __forceinline__ void sort_keys(uint64_t (&e)[NL], int (&ei)[NL]) {
+  if constexpr (NL == 8) {
+    cswap(e[0], e[2], ei[0], ei[2]); cswap(e[1], e[3], ei[1], ei[3]);
+    cswap(e[4], e[6], ei[4], ei[6]); cswap(e[5], e[7], ei[5], ei[7]);
+    cswap(e[0], e[4], ei[0], ei[4]); cswap(e[1], e[5], ei[1], ei[5]);
+    cswap(e[2], e[6], ei[2], ei[6]); cswap(e[3], e[7], ei[3], ei[7]);
+    cswap(e[0], e[1], ei[0], ei[1]); cswap(e[2], e[3], ei[2], ei[3]);
+    cswap(e[4], e[5], ei[4], ei[5]); cswap(e[6], e[7], ei[6], ei[7]);
+    cswap(e[2], e[4], ei[2], ei[4]); cswap(e[3], e[5], ei[3], ei[5]);
+    cswap(e[1], e[4], ei[1], ei[4]); cswap(e[3], e[6], ei[3], ei[6]);
+    cswap(e[1], e[2], ei[1], ei[2]); cswap(e[3], e[4], ei[3], ei[4]); cswap(e[5], e[6], ei[5], ei[6]);
+  } else {
+#pragma unroll
+    for (int round = 0; round < NL; ++round)
+#pragma unroll
+      for (int t = round & 1; t + 1 < NL; t += 2) cswap(e[t], e[t + 1], ei[t], ei[t + 1]);
+  }
+}
+
+// Block b of n runs on XCD b % 8 (the hardware's round-robin); this returns the
+// logical block such that each XCD takes one contiguous range of them, so that
+// neighbouring cell-sorted queries -- and the cells they read -- share an L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int x = b & 7, slot = b >> 3, q = n >> 3, rem = n & 7;
+  return x * q + min(x, rem) + slot;
+}
+
+template <int NB, int NL>
+constexpr int lane_words() {
+  return 3 * NL > NB / 2 ? 3 * NL : NB / 2;
+}
+
+// nsel bit: the query's list holds original indices into the cloud's raw points
+// (the rescan path), not positions in the cell-sorted copy.
+constexpr int kNselOrig = 1 << 30;
+
+// One query of the lane path. K: block radius in cells; NB: histogram buckets;
+// NL: bucket-B capacity; buf: the lane's LDS column (lane_words words).
+// Returns true when the query is left to the next path (nothing written).
+template <int K, int NB, int NL>
+__device__ __forceinline__ bool lane_query(const GridDesc* __restrict__ grids,
+                                           const int* __restrict__ cell_start,
+                                           const float4* __restrict__ pts,
+                                           const int* __restrict__ cloud_of_sorted, int k, int64_t j,
+                                           uint32_t (*buf)[64], int lane, int* __restrict__ nbr,
+                                           int* __restrict__ nsel) {
   const float4 q = pts[j];
   const GridDesc g = grids[cloud_of_sorted[j]];
   const int kk = min(k, g.npts);
   if (kk < 3) {
     nsel[j] = kk;
-    flags[j] = 0;
-    return;
+    return false;
   }
   const int* cs = cell_start + g.cell_off;
   const CellPos cp = cell_pos(g, q.x, q.y, q.z);
-  const float Lc = block_gap(g, cp, 1);
+  const float Lc = block_gap(g, cp, K);
   const float L = (Lc - cp.slack) * g.h;
-  // the staged path takes blocks that cover the grid (no boundary to certify
-  // against) and gaps inside the rounding slack
   const float D2 = L * L * 0.99998f;
-  const float scale = (float)kLaneBuckets / D2;
-  // (a finite scale keeps every bucket index of a d2 < D2 inside [0, kLaneBuckets])
+  const float scale = (float)NB / D2;
+  // blocks that cover the grid (no boundary to certify against) and gaps inside
+  // the rounding slack go on; a finite scale keeps every bucket index of a
+  // d2 < D2 inside [0, NB]
   bool ok = Lc < INFINITY && L > 0.0f && D2 > 1e-30f && scale < INFINITY;
+  BlockRuns<K> blk;
   if (ok) {
+    blk = block_runs<K>(g, cs, cp);
     int m = 0;
-    const int x0 = max(cp.cx - 1, 0), x1 = min(cp.cx + 1, g.nx - 1);
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int z = cp.cz + dz, y = cp.cy + dy;
-        if (z < 0 || z >= g.nz || y < 0 || y >= g.ny || x0 > x1) continue;
-        const int row = (z * g.ny + y) * g.nx;
-        m += cs[row + x1 + 1] - cs[row + x0];
-      }
+#pragma unroll
+    for (int r = 0; r < BlockRuns<K>::R; ++r) m += blk.n[r];
     ok = m >= kk && m <= kLaneMaxCand;
   }
   int B = -1, below = 0, nB = 0;
   if (ok) {
 #pragma unroll
-    for (int c = 0; c < kLaneBuckets / 2; ++c) s_hist[w][c][lane] = 0u;
-    for_block1(g, cs, pts, cp, [&](const float4& p) {
+    for (int c = 0; c < NB / 2; ++c) buf[c][lane] = 0u;
+    for_block<K>(blk, pts, [&](const float4& p, int) {
       const float d = flann_d2(q.x, q.y, q.z, p);
       if (d < D2) {
-        const int b = min((int)(d * scale), kLaneBuckets - 1);
-        atomicAdd(&s_hist[w][b >> 1][lane], 1u << ((b & 1) * 16));
+        const int b = min((int)(d * scale), NB - 1);
+        atomicAdd(&buf[b >> 1][lane], 1u << ((b & 1) * 16));
       }
     });
     int cum = 0;
 #pragma unroll
-    for (int c = 0; c < kLaneBuckets / 2; ++c) {
-      const uint32_t v = s_hist[w][c][lane];
+    for (int c = 0; c < NB / 2; ++c) {
+      const uint32_t v = buf[c][lane];
       const int lo = (int)(v & 0xffffu), hi = (int)(v >> 16);
       if (B < 0 && cum + lo >= kk) {
         B = 2 * c;
@@ -718,67 +771,356 @@ k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
       }
       cum += lo + hi;
     }
-    ok = B >= 0 && nB <= kLaneList;
+    ok = B >= 0 && nB <= NL;
   }
-  if (!ok) {
-    flags[j] = 1;
-    return;
-  }
+  if (!ok) return true;
+  // the list holds sorted positions (the solve pass reads the cell-sorted copy)
   int* out = nbr + j * k;
   int pos = 0, nl = 0;
-  for_block1(g, cs, pts, cp, [&](const float4& p) {
+  for_block<K>(blk, pts, [&](const float4& p, int i) {
     const float d = flann_d2(q.x, q.y, q.z, p);
     if (d < D2) {
-      const int b = min((int)(d * scale), kLaneBuckets - 1);
+      const int b = min((int)(d * scale), NB - 1);
       if (b < B) {
-        out[pos++] = __float_as_int(p.w);
-      } else if (b == B && nl < kLaneList) {
-        s_list[w][nl++][lane] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)__float_as_int(p.w);
+        out[pos++] = i;
+      } else if (b == B && nl < NL) {
+        buf[3 * nl][lane] = __float_as_uint(d);
+        buf[3 * nl + 1][lane] = (uint32_t)__float_as_int(p.w);
+        buf[3 * nl + 2][lane] = (uint32_t)i;
+        ++nl;
       }
     }
   });
-  uint64_t e[kLaneList];
+  uint64_t e[NL];
+  int ei[NL];
 #pragma unroll
-  for (int t = 0; t < kLaneList; ++t) e[t] = t < nl ? s_list[w][t][lane] : ~0ull;
-  // 19-comparator sorting network for 8 keys (d2 bits, original index)
-  cswap(e[0], e[2]); cswap(e[1], e[3]); cswap(e[4], e[6]); cswap(e[5], e[7]);
-  cswap(e[0], e[4]); cswap(e[1], e[5]); cswap(e[2], e[6]); cswap(e[3], e[7]);
-  cswap(e[0], e[1]); cswap(e[2], e[3]); cswap(e[4], e[5]); cswap(e[6], e[7]);
-  cswap(e[2], e[4]); cswap(e[3], e[5]);
-  cswap(e[1], e[4]); cswap(e[3], e[6]);
-  cswap(e[1], e[2]); cswap(e[3], e[4]); cswap(e[5], e[6]);
+  for (int t = 0; t < NL; ++t) {
+    e[t] = t < nl ? ((uint64_t)buf[3 * t][lane] << 32) | buf[3 * t + 1][lane] : ~0ull;
+    ei[t] = t < nl ? (int)buf[3 * t + 2][lane] : 0;
+  }
+  sort_keys<NL>(e, ei);
   const int need = kk - below;
 #pragma unroll
-  for (int t = 0; t < kLaneList; ++t)
-    if (t < need) out[pos + t] = (int)(uint32_t)e[t];
+  for (int t = 0; t < NL; ++t)
+    if (t < need) out[pos + t] = ei[t];
   nsel[j] = kk;
-  flags[j] = 0;
+  return false;
+}
+
+// Appends the lanes where take holds to list (list[*count ...]), one atomic per
+// wavefront; every lane of the wavefront calls it.  The order of the list
+// depends on scheduling; every consumer's result per query does not.
+__device__ __forceinline__ void wave_append(bool take, int64_t j, int* __restrict__ list,
+                                            int* __restrict__ count) {
+  const uint64_t b = __ballot(take);
+  if (b == 0) return;
+  const int lead = __ffsll((unsigned long long)b) - 1;
+  int base = 0;
+  if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(count, __popcll(b));
+  base = __builtin_amdgcn_readlane(base, lead);
+  if (take) list[base + lane_prefix(b)] = (int)j;
+}
+
+// The lane pass over every point: one lane per point, blocks mapped
+// XCD-contiguous; the points it leaves are appended to list.  No barrier: each
+// lane uses only its own LDS column.
+template <int K, int NB, int NL>
+__global__ void __launch_bounds__(kLaneThreads)
+k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
+              const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
+              int64_t total, int k, int* __restrict__ nbr, int* __restrict__ nsel,
+              int* __restrict__ list, int* __restrict__ count) {
+  __shared__ uint32_t s_buf[kLaneThreads / 64][lane_words<NB, NL>()][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kLaneThreads + threadIdx.x;
+  const bool left = j < total && lane_query<K, NB, NL>(grids, cell_start, pts, cloud_of_sorted, k,
+                                                       j, s_buf[w], lane, nbr, nsel);
+  wave_append(left, j, list, count);
+}
+
+// ---------------------------------------------------------------- wavefront pass
+//
+// The queries the lane pass leaves, one wavefront per query.  The candidates are
+// the (2K+1)^2 x-runs of the block of radius K around the query's cell, taken 64
+// runs at a time; the lanes stride through each chunk's points together
+// (consecutive lanes read consecutive points of a run).
+//
+// Pass 1 counts d2 < D2 (the block's certified range, as in the lane path) in
+// kWaveBuckets equal buckets of a histogram the wavefront shares in LDS.  Fewer
+// than kk below D2 means the kk-th neighbour may lie outside the block; any kk
+// of the block's points bound its distance from above (the kk-th smallest of the
+// per-lane minima, distinct points, or the largest d2 when fewer lanes hold one),
+// and the next block is the smallest whose certified range holds that bound, so
+// it certifies.  A prefix over the buckets gives bucket B of the kk-th; more
+// than 64 points in B split it into 256 sub-buckets with one more pass (the
+// bucket key floor(256 * d2 * scale) at that level, still monotone in d2).
+// Pass 2 lists the points of lower buckets (ballot order: the run order of the
+// block) and keeps bucket B's (d2, original index, position) in LDS, where each
+// one's rank among them picks the first kk - (count below B).  The selected set
+// is exactly the kk smallest (d2, original index).  A block that covers the
+// grid, a radius above kWaveMaxK, or more than 64 equal keys sends the query to
+// the rescan pass.
+constexpr int kWaveBuckets = 256;
+constexpr int kWaveMaxK = 48;
+constexpr int kWaveThreads = 256;
+
+struct WaveLds {
+  int hist[kWaveBuckets];
+  int pref[65];  // points before run r of the chunk
+  int start[64]; // run r's first sorted position
+  uint32_t key[64], idx[64];
+  int pos[64];
+};
+
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// The kk-th smallest of the lanes' values (ties by lane), kk <= 64.
+__device__ __forceinline__ float wave_kth_smallest(float v, int kk) {
+  const int lane = threadIdx.x & 63;
+  int rank = 0;
+  for (int u = 0; u < 64; ++u) {
+    const float w = __shfl(v, u, 64);
+    rank += (w < v || (w == v && u < lane)) ? 1 : 0;
+  }
+  const int at = __ffsll((unsigned long long)__ballot(rank == kk - 1)) - 1;
+  return __shfl(v, at, 64);
+}
+
+// Calls f(valid, p, i) for this lane's share of the points of the block of radius
+// K (i: the sorted position), run chunk by run chunk; every lane takes part in
+// every call (valid = false past a chunk's end), so f may use ballots.
+template <class F>
+__device__ __forceinline__ void for_wave_block(WaveLds& L, const GridDesc& g, const int* cs,
+                                               const CellPos& cp, int K, const float4* pts,
+                                               F&& f) {
+  const int lane = threadIdx.x & 63;
+  const int side = 2 * K + 1, R = side * side;
+  const int x0 = max(cp.cx - K, 0), x1 = min(cp.cx + K, g.nx - 1);
+  for (int r0 = 0; r0 < R; r0 += 64) {
+    const int r = r0 + lane;
+    int st = 0, len = 0;
+    if (r < R && x0 <= x1) {
+      const int z = cp.cz + r / side - K, y = cp.cy + r % side - K;
+      if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) {
+        const int row = (z * g.ny + y) * g.nx;
+        st = cs[row + x0];
+        len = cs[row + x1 + 1] - st;
+      }
+    }
+    const int incl = wave_incl_scan(len);
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0) continue;
+    wave_lds_sync();  // the previous chunk's reads are done
+    L.start[lane] = st;
+    L.pref[lane + 1] = incl;
+    if (lane == 0) L.pref[0] = 0;
+    wave_lds_sync();
+    int rr = 0;
+    for (int base = 0; base < total; base += 64) {
+      const int t = base + lane;
+      const bool valid = t < total;
+      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+      int i = 0;
+      if (valid) {
+        while (t >= L.pref[rr + 1]) ++rr;  // runs may be empty
+        i = L.start[rr] + (t - L.pref[rr]);
+        p = pts[i];
+      }
+      f(valid, p, i);
+    }
+  }
+}
+
+// Bucket B of the kk-th among the counts in L.hist (lane l holds buckets 4l ..
+// 4l + 3) plus `below` points before them: returns false when the counts hold
+// fewer than kk; otherwise B, the count before B (below updated) and B's count.
+__device__ __forceinline__ bool wave_find_bucket(const WaveLds& L, int kk, int& below, int& B,
+                                                 int& nB) {
+  const int lane = threadIdx.x & 63;
+  int h[4], sum = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    h[c] = L.hist[4 * lane + c];
+    sum += h[c];
+  }
+  const int incl = wave_incl_scan(sum) + below;
+  if (__shfl(incl, 63, 64) < kk) return false;
+  const bool cross = incl >= kk && incl - sum < kk;
+  const int cl = __ffsll((unsigned long long)__ballot(cross)) - 1;
+  int b = 0, bl = 0, n = 0;
+  if (cross) {
+    int cum = incl - sum;
+    b = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (b < 0 && cum + h[c] >= kk) {
+        b = 4 * lane + c;
+        bl = cum;
+        n = h[c];
+      }
+      cum += h[c];
+    }
+  }
+  B = __builtin_amdgcn_readlane(b, cl);
+  below = __builtin_amdgcn_readlane(bl, cl);
+  nB = __builtin_amdgcn_readlane(n, cl);
+  return true;
+}
+
+__global__ void __launch_bounds__(kWaveThreads)
+k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
+              const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted, int k,
+              const int* __restrict__ list, const int* __restrict__ count, int* __restrict__ nbr,
+              int* __restrict__ nsel, int* __restrict__ deferred) {
+  __shared__ WaveLds s_w[kWaveThreads / 64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  WaveLds& L = s_w[w];
+  const int n = *count;
+  const int nwaves = (int)gridDim.x * (kWaveThreads / 64);
+  for (int qi = (int)blockIdx.x * (kWaveThreads / 64) + w; qi < n; qi += nwaves) {
+    const int64_t j = __builtin_amdgcn_readfirstlane(list[qi]);
+    const float4 q = pts[j];
+    const GridDesc g = grids[cloud_of_sorted[j]];
+    const int kk = min(k, g.npts);  // at least 3: the lane pass serves the others
+    const int* cs = cell_start + g.cell_off;
+    const CellPos cp = cell_pos(g, q.x, q.y, q.z);
+    bool done = false;
+    int K = 1;
+    for (int attempt = 0; attempt < 8 && !done && K <= kWaveMaxK; ++attempt) {
+      const float Lc = block_gap(g, cp, K);
+      if (Lc == INFINITY) break;  // covers the grid: the rescan pass
+      const float Lm = (Lc - cp.slack) * g.h;
+      const float D2 = Lm * Lm * 0.99998f;
+      const float scale = (float)kWaveBuckets / D2;
+      if (!(Lm > 0.0f && D2 > 1e-30f && scale < INFINITY)) {
+        ++K;
+        continue;
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int c = lane; c < kWaveBuckets; c += 64) L.hist[c] = 0;
+      // pass 1: the histogram of d2 < D2, and the lane's count, least and largest d2
+      int mc = 0;
+      float dmin = INFINITY, dmax = 0.0f;
+      for_wave_block(L, g, cs, cp, K, pts, [&](bool valid, const float4& p, int) {
+        if (!valid) return;
+        const float d = flann_d2(q.x, q.y, q.z, p);
+        ++mc;
+        dmin = fminf(dmin, d);
+        dmax = fmaxf(dmax, d);
+        if (d < D2) atomicAdd(&L.hist[min((int)(d * scale), kWaveBuckets - 1)], 1);
+      });
+      wave_lds_sync();
+      if (wave_sum_i(mc) < kk) {
+        K = 2 * K + 1;
+        continue;
+      }
+      int below = 0, B, nB;
+      if (!wave_find_bucket(L, kk, below, B, nB)) {
+        // not certified: a block whose certified range holds an upper bound U of
+        // the kk-th d2 certifies (its gap is at least its radius in cells)
+        const float U = (kk <= 64 && popc_ballot(mc > 0) >= kk) ? wave_kth_smallest(dmin, kk)
+                                                                 : wave_max_f(dmax);
+        const float need = sqrtf(U) * g.inv_h * 1.0001f + cp.slack + 1e-3f;
+        K = max(K + 1, (int)ceilf(fminf(need, (float)(kWaveMaxK + 1))));
+        continue;
+      }
+      // level 1 inside bucket B when it holds more than 64 points
+      int SB = 0;
+      const bool split = nB > 64;
+      if (split) {
+        wave_lds_sync();
+#pragma unroll
+        for (int c = lane; c < kWaveBuckets; c += 64) L.hist[c] = 0;
+        wave_lds_sync();
+        for_wave_block(L, g, cs, cp, K, pts, [&](bool valid, const float4& p, int) {
+          if (!valid) return;
+          const float d = flann_d2(q.x, q.y, q.z, p);
+          if (!(d < D2)) return;
+          const float x = d * scale;
+          if (min((int)x, kWaveBuckets - 1) != B) return;
+          atomicAdd(&L.hist[min((int)((x - (float)B) * 256.0f), 255)], 1);
+        });
+        wave_lds_sync();
+        wave_find_bucket(L, kk, below, SB, nB);
+        if (nB > 64) break;  // more than 64 equal keys: the rescan pass
+      }
+      // pass 2: list the lower keys, keep the kk-th's key in LDS
+      int* out = nbr + j * k;
+      int pos = 0, nl = 0;
+      for_wave_block(L, g, cs, cp, K, pts, [&](bool valid, const float4& p, int i) {
+        const float d = valid ? flann_d2(q.x, q.y, q.z, p) : INFINITY;
+        const float x = d * scale;
+        const int b = d < D2 ? min((int)x, kWaveBuckets - 1) : kWaveBuckets;
+        const int sb = (split && b == B) ? min((int)((x - (float)B) * 256.0f), 255) : 0;
+        const bool lt = b < B || (b == B && sb < SB);
+        const bool eq = b == B && sb == SB;
+        const uint64_t blt = __ballot(lt), beq = __ballot(eq);
+        if (lt) out[pos + lane_prefix(blt)] = i;
+        if (eq) {
+          const int slot = nl + lane_prefix(beq);
+          L.key[slot] = __float_as_uint(d);
+          L.idx[slot] = (uint32_t)__float_as_int(p.w);
+          L.pos[slot] = i;
+        }
+        pos += __popcll(blt);
+        nl += __popcll(beq);
+      });
+      wave_lds_sync();
+      // rank of each kept entry by (d2, original index): unique keys
+      if (lane < nl) {
+        const uint64_t mine = ((uint64_t)L.key[lane] << 32) | L.idx[lane];
+        int rank = 0;
+        for (int u = 0; u < nl; ++u) rank += (((uint64_t)L.key[u] << 32) | L.idx[u]) < mine ? 1 : 0;
+        if (rank < kk - below) out[pos + rank] = L.pos[lane];
+      }
+      if (lane == 0) nsel[j] = kk;
+      done = true;
+    }
+    if (!done && lane == 0) {
+      // deferred[0] counts the list deferred[1..]
+      nsel[j] = kk | kNselOrig;
+      deferred[1 + atomicAdd(deferred, 1)] = (int)j;
+    }
+    wave_lds_sync();  // L is reused by the next query
+  }
 }
 
 // Search pass, per point in cell-sorted order: the neighbour count and either
 // the original indices of its K nearest neighbours (kMoments: nbr holds k per
 // point, the moments are summed by the solve pass) or the FP64 sum of their
-// distances (kSqrtDist: mom holds 1 per point).  With flags (nullable) only the
-// points whose flag is set take part (the queries the lane path left).
+// distances (kSqrtDist: mom holds 1 per point).
 template <int MODE>
 __global__ void __launch_bounds__(kWaves * 64)
 k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
                  const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
                  int64_t total, int k, double* __restrict__ mom, int* __restrict__ nbr,
-                 int* __restrict__ nsel, int* __restrict__ deferred,
-                 const uint8_t* __restrict__ flags) {
+                 int* __restrict__ nsel, int* __restrict__ deferred) {
   __shared__ float4 s_p[kWaves][kCap];
   __shared__ int s_cb[kWaves][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 64;
+  const int64_t base = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kWaves + w) * 64;
   if (base >= total) return;
   const int64_t jq = base + lane;
   const bool own = jq < total;
   const int nq = (int)min<int64_t>(64, total - base);
-  const bool take = own && (!flags || flags[jq] != 0);
-  const uint64_t todo = __ballot(take);
-  if (todo == 0) return;
   Staged st;
   st.cell = -1;
   st.K = 0;
@@ -788,7 +1130,6 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
   st.lastT = 0u;
   int mine_n = 0;
   for (int qi = 0; qi < nq; ++qi) {
-    if (!((todo >> qi) & 1ull)) continue;  // wave-uniform
     // wave-uniform addresses: scalar loads, so no vector load issued before the
     // loop stays pending into it (a pending one makes every iteration wait for
     // the previous iteration's stores)
@@ -812,7 +1153,7 @@ k_normal_moments(const GridDesc* __restrict__ grids, const int* __restrict__ cel
     }
     if (lane == qi) mine_n = kk;
   }
-  if (take) nsel[jq] = mine_n;
+  if (own) nsel[jq] = mine_n;
 }
 
 // Rescan pass for the deferred queries (candidate blocks too large for LDS), one
@@ -846,20 +1187,25 @@ k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell
 // Solve pass, one point per lane: FP64 moments of its neighbours relative to it
 // (in the order the search listed them), covariance, FP64 eigen-decomposition,
 // normal (eigenvector of the smallest eigenvalue) oriented towards the viewpoint.
+// The lane and wavefront passes list sorted positions (neighbouring queries read
+// neighbouring points of the cell-sorted copy); the rescan pass (kNselOrig set)
+// lists original indices into the cloud's raw points.  Both hold the same
+// coordinates.
 __global__ void __launch_bounds__(256)
 k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
                const float4* __restrict__ raw, const int64_t* __restrict__ cloud_off,
                const int* __restrict__ nbr, int k, const int* __restrict__ nsel, int64_t total,
                const float* __restrict__ vp, int flip_all, float4* __restrict__ out) {
-  const int64_t jq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t jq = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (jq >= total) return;
-  const int cnt = nsel[jq];
+  const int ns = nsel[jq];
+  const int cnt = ns & ~kNselOrig;
   if (cnt < 3) {
     out[jq] = make_float4(NAN, NAN, NAN, NAN);
     return;
   }
   const float4 q = pts[jq];
-  const float4* R = raw + cloud_off[cloud_of_sorted[jq]];
+  const float4* R = (ns & kNselOrig) ? raw + cloud_off[cloud_of_sorted[jq]] : pts;
   const int* list = nbr + jq * k;
   Moments acc;
 #pragma unroll
@@ -923,33 +1269,40 @@ void launch_sor_distances(const GridDesc* grids, const int* cell_start, const fl
   launch_fill_i32(deferred, 1, 0, s);
   hipLaunchKernelGGL(k_normal_moments<kSqrtDist>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
                      cell_start, pts, cloud_of_sorted, npts_total, mean_k + 1, sums, (int*)nullptr, nsel,
-                     deferred, (const uint8_t*)nullptr);
+                     deferred);
   hipLaunchKernelGGL(k_normal_rescan<kSqrtDist>, dim3(1024), dim3(256), 0, s, grids, cell_start,
                      pts, cloud_of_sorted, mean_k + 1, deferred, sums, (int*)nullptr);
   hipLaunchKernelGGL(k_sor_dist, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s, pts,
                      cloud_of_sorted, cloud_off, sums, npts_total, mean_k, dist, finite);
 }
 
+int64_t normals_work_ints(int64_t n) { return 2 * n + 2; }
+
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
                     int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
-                    int* deferred, uint8_t* flags, float4* nrm_sorted, hipStream_t s) {
+                    int* work, float4* nrm_sorted, hipStream_t s) {
   if (npts_total <= 0) return;
-  const int64_t per_block = (int64_t)kWaves * 64;
-  const unsigned blocks = (unsigned)((npts_total + per_block - 1) / per_block);
-  launch_fill_i32(deferred, 1, 0, s);
-  // the lane path first; the staged wavefront path takes the queries it flags
-  hipLaunchKernelGGL(k_normal_lane, dim3((unsigned)((npts_total + kLaneThreads - 1) / kLaneThreads)),
-                     dim3(kLaneThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, npts_total, k,
-                     nbr, nsel, flags);
-  hipLaunchKernelGGL(k_normal_moments<kMoments>, dim3(blocks), dim3(kWaves * 64), 0, s, grids,
-                     cell_start, pts, cloud_of_sorted, npts_total, k, (double*)nullptr, nbr, nsel,
-                     deferred, (const uint8_t*)flags);
+  const int64_t n = npts_total;
+  // work: [0] the lane pass's list count, [1] the rescan count with its list at
+  // [2, 2 + n), the lane pass's list at [2 + n, 2 + 2n)
+  int* count = work;
+  int* deferred = work + 1;
+  int* list = work + 2 + n;
+  launch_fill_i32(work, 2, 0, s);
+  hipLaunchKernelGGL((k_normal_lane<1, 32, 8>), dim3((unsigned)((n + kLaneThreads - 1) / kLaneThreads)),
+                     dim3(kLaneThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, n, k, nbr,
+                     nsel, list, count);
+  // persistent over the list (a few percent of the points): at most eight blocks
+  // of four wavefronts per CU
+  const int64_t wave_blocks = (n + kWaveThreads / 64 - 1) / (kWaveThreads / 64);
+  hipLaunchKernelGGL(k_normal_wave, dim3((unsigned)(wave_blocks < 2048 ? wave_blocks : 2048)),
+                     dim3(kWaveThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, k,
+                     (const int*)list, (const int*)count, nbr, nsel, deferred);
   hipLaunchKernelGGL(k_normal_rescan<kMoments>, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
                      cloud_of_sorted, k, deferred, (double*)nullptr, nbr);
-  hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((npts_total + 255) / 256)), dim3(256), 0, s,
-                     pts, cloud_of_sorted, raw, cloud_off, nbr, k, nsel, npts_total, vp, flip_all,
-                     nrm_sorted);
+  hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts,
+                     cloud_of_sorted, raw, cloud_off, nbr, k, nsel, n, vp, flip_all, nrm_sorted);
 }
 
 }  // namespace lcr

@@ -1,6 +1,8 @@
 // runtime.cpp -- host runtime of liblcr (see runtime.hpp).
 #include "runtime.hpp"
 
+#include "lcr_search.hpp"
+
 #include <atomic>
 #include <thread>
 
@@ -406,12 +408,11 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
   const size_t np = (size_t)std::max<int64_t>(total_pts, 1);
   d_nbr.reserve(np * (size_t)k * sizeof(int));
   d_nsel.reserve(np * sizeof(int));
-  d_defer.reserve((np + 1) * sizeof(int));
-  d_nflag.reserve(np);
+  d_nwork.reserve((size_t)normals_work_ints((int64_t)np) * sizeof(int));
   launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
                  d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
                  d_raw.as<float4>(), d_cloud_off.as<int64_t>(), d_nbr.as<int>(), d_nsel.as<int>(),
-                 d_defer.as<int>(), d_nflag.as<uint8_t>(), d_nrm.as<float4>(), s);
+                 d_nwork.as<int>(), d_nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
 }
@@ -508,11 +509,13 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   if (ngate_tiles > 0)
     LCR_HIP(hipMemcpy(d_gate_tiles.p, gt.data(), gt.size() * sizeof(int), hipMemcpyHostToDevice));
   const int cus = device_cu_count();
-  const int per_cu = icp_max_blocks_per_cu(tile, c.point_to_plane != 0, c.precision);
   setup_precision = c.precision;
-  int nb = std::min(std::max(ntiles, 1), cus * per_cu);
-  nb = std::max(8, (nb + 7) / 8 * 8);
-  nblocks = nb;
+  for (int ring = 0; ring < 2; ++ring) {
+    const int per_cu = icp_max_blocks_per_cu(tile, c.point_to_plane != 0, c.precision, ring != 0);
+    int nb = std::min(std::max(ntiles, 1), cus * per_cu);
+    nb = std::max(8, (nb + 7) / 8 * 8);
+    (ring ? nblocks_ring : nblocks) = nb;
+  }
   int nf = std::min(std::max(ntiles, 1), cus * fitness_max_blocks_per_cu(tile));
   nblocks_fit = std::max(8, (nf + 7) / 8 * 8);
 }
@@ -567,10 +570,23 @@ __global__ void k_count_active(const PairState* __restrict__ st, int n, int* __r
 // captured hipGraph; HIP events recorded inside a captured graph cannot be timed
 // afterwards, so with the profiler on the same launches are issued eagerly with
 // an event pair around each kernel.
+bool Engine::ring_only(const CloudSet& cs) const {
+  if (c.precision != 0 || npairs == 0) return false;
+  for (const PairDesc& d : hp) {
+    // the same float arithmetic as nn_query's per-query test
+    const float r = (float)sqrt(d.max_corr_d2);
+    const float rlim = r * 1.00001f;
+    if (!std::isfinite(r) || !(rlim * cs.grids[d.tgt].inv_h <= kRingOnlyCells)) return false;
+  }
+  return true;
+}
+
 void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool use_graph,
                  const std::function<void()>& pre) {
   if (npairs == 0) return;
   Profiler& prof = Profiler::get();
+  c.ring_only = ring_only(cs) ? 1 : 0;
+  const int nb_search = c.ring_only ? nblocks_ring : nblocks;
   const IcpBuffers b = buffers(cs);
   if (guesses) {
     memcpy(h_guess.p, guesses, (size_t)npairs * 16 * sizeof(double));
@@ -605,7 +621,7 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   auto chunk = [&](std::vector<Profiler::Span>* sp) {
     for (int it = 0; it < kChunk; ++it) {
       const int k = prof.begin("icp_iteration", s, sp);
-      launch_icp_search(b, c, nblocks, s);
+      launch_icp_search(b, c, nb_search, s);
       prof.end(k, s, sp);
       const int k2 = prof.begin("icp_solve", s, sp);
       launch_icp_solve(b, c, s);
@@ -678,7 +694,7 @@ Engine::GraphKey Engine::graph_key(const IcpBuffers& b) const {
   memset(&k, 0, sizeof k);
   k.b = b;
   k.c = c;
-  k.nblocks = nblocks;
+  k.nblocks = c.ring_only ? nblocks_ring : nblocks;
   k.nblocks_fit = nblocks_fit;
   k.guess = d_guess.p;
   k.hstate = h_state.p;

@@ -138,10 +138,10 @@ struct CloudSet {
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
   DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
-  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel, d_defer;
+  DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel;
   DevBuf d_tpts, d_tpos, d_tkeys, d_tkeys_sorted, d_tvals, d_ttmp;  // tree order
   DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
-  DevBuf d_nflag;  // normals: 1 where the lane path left the query to the staged path
+  DevBuf d_nwork;  // normals: lane-pass flags, their scan and the second pass's list
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
 
   // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;
@@ -169,6 +169,10 @@ struct CloudSet {
 // A batch of ICP instances over a CloudSet.
 struct Engine {
   int npairs = 0, ntiles = 0, tile = 256, nblocks = 8;
+  int nblocks_ring = 8;  // persistent blocks of the ring-only search variant
+  // IcpConst::ring_only for the instances over these grids: the FP32 scan, and
+  // every radius finite and spanning at most kRingOnlyCells target cells
+  bool ring_only(const CloudSet& cs) const;
   int nblocks_fit = 8;  // persistent blocks of the fitness passes (their own occupancy)
   IcpConst c{};
   std::vector<PairDesc> hp;

@@ -123,6 +123,11 @@ def lib():
         L.oracle_depth_to_cloud.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_int, u8] + \
             [C.c_double] * 4 + [C.c_int] * 6 + [fp, fp, u8]
         L.oracle_depth_to_cloud.restype = C.c_int64
+        L.oracle_depth_fuse.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(C.c_uint16)]
+        L.oracle_depth_fuse.restype = None
+        L.oracle_lanczos4_weights.argtypes = [C.c_double, fp]
+        L.oracle_lanczos4_weights.restype = None
         _lib = L
     return _lib
 
@@ -387,6 +392,22 @@ def sor(xyz, mean_k, mul):
     lib().oracle_sor(_p(x, C.c_float), len(x), int(mean_k), float(mul), _p(keep, C.c_uint8),
                      _p(dist, C.c_float), _p(stats, C.c_double))
     return keep.astype(bool), dist, tuple(stats)
+
+
+def lanczos4_weights(fr):
+    """The 8 normalised Lanczos-4 tap weights (float32) at fractional offset fr."""
+    w = np.empty(8, np.float32)
+    lib().oracle_lanczos4_weights(float(fr), _p(w, C.c_float))
+    return w
+
+
+def depth_fuse(frames, factor):
+    """super_resolution.cpp:283-314 restated for aligned frames: (H, W) uint16."""
+    f = np.ascontiguousarray(frames, dtype=np.uint16)
+    k, h, w = f.shape
+    out = np.empty((h, w), np.uint16)
+    lib().oracle_depth_fuse(_p(f, C.c_uint16), k, w, h, int(factor), _p(out, C.c_uint16))
+    return out
 
 
 def depth_to_cloud(depth, intrinsics, depth_min, depth_max, crop=None, color=None):

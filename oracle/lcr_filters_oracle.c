@@ -253,3 +253,81 @@ int64_t oracle_depth_to_cloud(const uint16_t* depth, int w, int h, const uint8_t
   }
   return m;
 }
+
+/* Lanczos-4 weights of the 8 taps around a sample point at fractional offset fr
+ * (taps at -3..4 from the integer part): the windowed sinc
+ * L(d) = sin(pi d) sin(pi d / 4) / (pi^2 d^2 / 4), d = fr + 3 - i, normalised to sum
+ * 1 in double and rounded to float; fr == 0 gives the unit tap. */
+void oracle_lanczos4_weights(double fr, float w[8]) {
+  const double pi = 3.14159265358979323846;
+  if (fr == 0.0) {
+    for (int i = 0; i < 8; i++) w[i] = i == 3 ? 1.0f : 0.0f;
+    return;
+  }
+  double c[8], s = 0.0;
+  for (int i = 0; i < 8; i++) {
+    const double d = fr + 3.0 - i;
+    c[i] = sin(pi * d) * sin(pi * d / 4.0) / (pi * pi * d * d / 4.0);
+    s += c[i];
+  }
+  for (int i = 0; i < 8; i++) w[i] = (float)(c[i] / s);
+}
+
+/* The burst fusion of capture/super_resolution/super_resolution.cpp:283-314 for
+ * frames already aligned (identity registration): each frame upsampled by `factor`
+ * with nearest neighbours (:283), the zero-eliminating mean per pixel of the
+ * upsampled frames (:295-306: the mean of the positive values, accumulated in
+ * double, NaN when there are none), the Lanczos-4 resize back to width x height
+ * (:311: source position (x + 0.5) f - 0.5, 8 taps clamped to the image, the
+ * horizontal pass then the vertical, in float) and the conversion to 16 bits
+ * (:312: round to nearest even, clamped to [0, 65535], NaN -> 0).  frames:
+ * n_frames x height x width, row-major. */
+void oracle_depth_fuse(const uint16_t *frames, int n_frames, int w, int h, int factor,
+                       uint16_t *out) {
+  const int64_t np = (int64_t)w * h;
+  float *mean = (float *)malloc(sizeof(float) * (size_t)(np > 0 ? np : 1));
+  for (int64_t p = 0; p < np; p++) {
+    double s = 0.0;
+    int c = 0;
+    for (int f = 0; f < n_frames; f++) {
+      const uint16_t v = frames[(int64_t)f * np + p];
+      if (v > 0) {
+        s += (double)(float)v;
+        c++;
+      }
+    }
+    mean[p] = c > 0 ? (float)(s / c) : NAN;
+  }
+  /* the upsampled image (factor w x factor h) repeats each mean; sample it */
+  const int W = w * factor, H = h * factor;
+  const double pos = (factor - 1) / 2.0; /* (x + 0.5) f - 0.5 = f x + (f - 1) / 2 */
+  const int base = (int)floor(pos);
+  float wx[8];
+  oracle_lanczos4_weights(pos - base, wx);
+  for (int y = 0; y < h; y++) {
+    const int sy = factor * y + base;
+    for (int x = 0; x < w; x++) {
+      const int sx = factor * x + base;
+      float acc = 0.0f;
+      for (int ky = 0; ky < 8; ky++) {
+        int r = sy - 3 + ky;
+        r = r < 0 ? 0 : (r >= H ? H - 1 : r);
+        const float *mr = mean + (int64_t)(r / factor) * w;
+        float hs = 0.0f;
+        for (int kx = 0; kx < 8; kx++) {
+          int q = sx - 3 + kx;
+          q = q < 0 ? 0 : (q >= W ? W - 1 : q);
+          hs = hs + wx[kx] * mr[q / factor];
+        }
+        acc = acc + wx[ky] * hs;
+      }
+      uint16_t v = 0;
+      if (!isnan(acc)) {
+        const float rv = rintf(acc);
+        v = rv <= 0.0f ? 0 : (rv >= 65535.0f ? 65535 : (uint16_t)rv);
+      }
+      out[(int64_t)y * w + x] = v;
+    }
+  }
+  free(mean);
+}

@@ -201,6 +201,12 @@ int64_t oracle_depth_to_cloud(const uint16_t *depth, int w, int h, const uint8_t
                               int top, int bottom, int left, int right, float *out_xyz,
                               float *out_nrm, uint8_t *out_rgb);
 
+/* Burst depth fusion of super_resolution.cpp:283-314 (aligned frames): nearest
+ * upsampling by `factor`, zero-eliminating mean, Lanczos-4 resize back, 16 bits. */
+void oracle_lanczos4_weights(double fr, float w[8]);
+void oracle_depth_fuse(const uint16_t *frames, int n_frames, int w, int h, int factor,
+                       uint16_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,3 +77,26 @@ def test_depth_to_cloud_matches_oracle(lcr, crop):
     np.testing.assert_array_equal(xyz, rxyz)
     np.testing.assert_array_equal(nrm, rnrm)
     np.testing.assert_array_equal(rgb, rrgb)
+
+
+@pytest.mark.parametrize("k,h,w,factor", [(16, 424, 512, 2), (8, 480, 640, 3), (4, 120, 160, 4),
+                                          (5, 31, 47, 1)])
+def test_depth_fuse_matches_oracle(lcr, k, h, w, factor):
+    """Burst fusion of aligned frames (super_resolution.cpp:283-314): bitwise equal
+    to the oracle, holes (all-zero pixels) included."""
+    rng = np.random.default_rng(k * 1000 + factor)
+    shape = synth.make_shape(rng, scale=1.5)
+    cam = synth.KINECT_V2 if (h, w) == (424, 512) else synth.KINECT_V1
+    d = synth.render_depth(shape, synth.turntable_pose(0.3, (0.0, 0.0, -0.9)), cam)
+    # the middle h x w of the rendered depth (the object sits in the image centre)
+    base = np.zeros((h, w))
+    hh, ww = min(h, d.shape[0]), min(w, d.shape[1])
+    y0, x0 = (d.shape[0] - hh) // 2, (d.shape[1] - ww) // 2
+    base[:hh, :ww] = d[y0:y0 + hh, x0:x0 + ww] * 1000.0
+    frames = np.round(base[None] + rng.normal(scale=2.0, size=(k, h, w)) * (base[None] > 0))
+    frames = np.clip(frames, 0, 65535).astype(np.uint16)
+    frames[rng.random(frames.shape) < 0.1] = 0
+    print_case(test="depth_fuse", k=k, h=h, w=w, factor=factor)
+    out = lcr.depth_fuse(frames, factor)
+    np.testing.assert_array_equal(out, oracle.depth_fuse(frames, factor))
+    assert (out > 0).mean() > 0.05

@@ -1,7 +1,8 @@
-"""The oracle's VoxelGrid, StatisticalOutlierRemoval and depth back-projection
-(oracle/lcr_filters_oracle.c) pinned by closed forms, brute force and planted
-structure (CPU only)."""
+"""The oracle's VoxelGrid, StatisticalOutlierRemoval, depth back-projection and
+burst depth fusion (oracle/lcr_filters_oracle.c) pinned by closed forms, brute
+force, numpy restatements and planted structure (CPU only)."""
 import numpy as np
+import pytest
 
 import oracle
 
@@ -108,3 +109,83 @@ def test_depth_to_cloud_closed_forms():
     xyz_c, _, _ = oracle.depth_to_cloud(depth, (fx, fy, cx, cy), 1010, 1020, crop=(2, 9, 3, 12))
     zz = depth[2:10, 3:13]
     assert len(xyz_c) == int(((zz >= 1010) & (zz <= 1020)).sum())
+
+
+def test_lanczos4_weights_are_the_windowed_sinc():
+    """Tap weights against numpy's sinc: sinc(d) sinc(d / 4), d = fr + 3 - i,
+    normalised; fr = 0 is the unit tap; fr = 0.5 is symmetric."""
+    for fr in (0.5, 0.25, 0.1, 0.9, 1e-3):
+        d = fr + 3.0 - np.arange(8)
+        c = np.sinc(d) * np.sinc(d / 4.0)
+        ref = (c / c.sum()).astype(np.float32)
+        w = oracle.lanczos4_weights(fr)
+        np.testing.assert_allclose(w, ref, rtol=2e-7, atol=1e-9)
+        assert abs(float(w.astype(np.float64).sum()) - 1.0) < 1e-6
+    np.testing.assert_array_equal(oracle.lanczos4_weights(0.0), [0, 0, 0, 1, 0, 0, 0, 0])
+    w = oracle.lanczos4_weights(0.5)
+    np.testing.assert_array_equal(w, w[::-1])
+
+
+def fuse_numpy(frames, factor):
+    """super_resolution.cpp:283-314 written with numpy: ZE mean (double sums of the
+    positive values), the nearest-upsampled image by np.repeat, then each output
+    pixel as the separable 8-tap sum over the clamped upsampled indices, horizontal
+    first, float32 throughout, then round half to even, clamp, NaN -> 0."""
+    f = frames.astype(np.float32)
+    pos = f > 0
+    cnt = pos.sum(0)
+    s = np.where(pos, f.astype(np.float64), 0.0).sum(0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mean = np.where(cnt > 0, (s / np.maximum(cnt, 1)).astype(np.float32), np.float32(np.nan))
+    up = np.repeat(np.repeat(mean, factor, axis=0), factor, axis=1).astype(np.float32)
+    H, W = up.shape
+    h, w = mean.shape
+    p = (factor - 1) / 2.0
+    base = int(np.floor(p))
+    wt = oracle.lanczos4_weights(p - base)
+    ys = factor * np.arange(h) + base
+    xs = factor * np.arange(w) + base
+    acc = np.zeros((h, w), np.float32)
+    for ky in range(8):
+        rows = np.clip(ys - 3 + ky, 0, H - 1)
+        hs = np.zeros((h, w), np.float32)
+        for kx in range(8):
+            cols = np.clip(xs - 3 + kx, 0, W - 1)
+            hs = (hs + wt[kx] * up[rows][:, cols]).astype(np.float32)
+        acc = (acc + wt[ky] * hs).astype(np.float32)
+    out = np.where(np.isnan(acc), 0.0, np.clip(np.rint(acc), 0, 65535))
+    return out.astype(np.uint16)
+
+
+def test_depth_fuse_closed_forms():
+    rng = np.random.default_rng(12)
+    # a constant burst stays constant; zeros are eliminated from the mean
+    f = np.full((6, 20, 24), 1234, np.uint16)
+    f[rng.random(f.shape) < 0.3] = 0
+    f[:, 0, 0] = 1234
+    np.testing.assert_array_equal(oracle.depth_fuse(f, 2), np.full((20, 24), 1234, np.uint16))
+    # the mean of the positive values: frames 1000 and 1003 where present -> 1001.5,
+    # rounded half to even -> 1002 with the unit taps of factor 1; 1000 and 1004 ->
+    # 1002 with the normalised float taps of factor 2 (they sum to 1 within 1e-7)
+    g = np.zeros((4, 16, 16), np.uint16)
+    g[0], g[2] = 1000, 1003
+    np.testing.assert_array_equal(oracle.depth_fuse(g, 1), np.full((16, 16), 1002, np.uint16))
+    g[2] = 1004
+    np.testing.assert_array_equal(oracle.depth_fuse(g, 2), np.full((16, 16), 1002, np.uint16))
+    # a pixel without any depth poisons the taps that reach it (NaN -> 0)
+    h_ = np.full((3, 16, 16), 900, np.uint16)
+    h_[:, 8, 8] = 0
+    out = oracle.depth_fuse(h_, 2)
+    assert out[8, 8] == 0 and out[0, 0] == 900 and out[15, 15] == 900
+    hole = np.argwhere(out == 0)
+    assert hole[:, 0].min() >= 6 and hole[:, 0].max() <= 10
+
+
+@pytest.mark.parametrize("factor", [1, 2, 3, 4])
+def test_depth_fuse_matches_numpy(factor):
+    rng = np.random.default_rng(factor)
+    base = 800 + 200 * np.sin(np.linspace(0, 3, 40))[:, None] * np.cos(np.linspace(0, 2, 56))[None, :]
+    f = (base[None] + rng.normal(scale=4.0, size=(8, 40, 56))).round().astype(np.uint16)
+    f[rng.random(f.shape) < 0.2] = 0
+    f[:, 30:33, 10:12] = 0
+    np.testing.assert_array_equal(oracle.depth_fuse(f, factor), fuse_numpy(f, factor))
