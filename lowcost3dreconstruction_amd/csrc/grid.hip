@@ -303,43 +303,92 @@ __device__ __forceinline__ int grid_of_cell(const GridDesc* __restrict__ g, int 
   return lo;
 }
 
-// One axis of the dilation: out[c] = OR of in over the cells within kd along
-// `axis` (pass 0 reads occupancy from cell_start instead of `in`).
-// One pass of the near mask's separable dilation along `axis`, one row of
-// blocks per grid (blockIdx.y): a cell is marked when a cell within kd along the
-// axis is (pass 0 reads occupancy from the cell-start table).  32-bit local
-// indices; the trailing sentinel cell of each grid is cleared.
-__global__ void k_near_pass(const GridDesc* __restrict__ grids, const int* __restrict__ cs,
-                            const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int axis,
-                            float r) {
+// The near mask, one bit per cell, each x-row of a grid padded to whole 32-bit
+// words (near_row_words); one row of blocks per grid (blockIdx.y), grid-stride
+// over its words.  Cells within kd cells of an occupied cell along every axis are
+// marked, kd = ceil(r / h) + 1 (at most 24, so an x-dilation reaches only the
+// neighbouring words).
+
+// Occupancy: bit b of word w of row (y, z) is set when cell x = 32 w + b holds a
+// point (padding bits past nx stay clear).  One wavefront per 64 cells of a row:
+// a lane per cell reads its two cell-table entries (coalesced), a ballot forms
+// two words.
+__global__ void k_near_occ(const GridDesc* __restrict__ grids, const int* __restrict__ cs,
+                           uint32_t* __restrict__ out) {
   const GridDesc& gd = grids[blockIdx.y];
-  const int nx = gd.nx, ny = gd.ny, nz = gd.nz, npts = gd.npts;
-  const int64_t off = gd.cell_off;
-  const int ncells = nx * ny * nz;
-  const int kd = (int)ceilf(r * 1.0001f * gd.inv_h) + 1;
-  const int dim = axis == 0 ? nx : (axis == 1 ? ny : nz);
-  const int stride = axis == 0 ? 1 : (axis == 1 ? nx : nx * ny);
-  for (int local = blockIdx.x * blockDim.x + threadIdx.x; local <= ncells;
-       local += gridDim.x * blockDim.x) {
-    const int64_t c = off + local;
-    if (npts == 0 || local == ncells) {
-      out[c] = 0;
-      continue;
+  const int lane = threadIdx.x & 63;
+  const int W = near_row_words(gd.nx);
+  const int segs = (gd.nx + 63) >> 6;
+  const int64_t items = (int64_t)gd.ny * gd.nz * segs;
+  const int* c = cs + gd.cell_off;
+  uint32_t* o = out + gd.near_off;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t it = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < items;
+       it += nwaves) {
+    const int64_t row = it / segs;
+    const int x = (int)(it - row * segs) * 64 + lane;
+    bool occ = false;
+    if (x < gd.nx) {
+      const int64_t cell = row * gd.nx + x;
+      occ = c[cell + 1] > c[cell];
     }
-    const int pos = axis == 0 ? local % nx : (axis == 1 ? (local / nx) % ny : local / (nx * ny));
-    const int lo = max(pos - kd, 0), hi = min(pos + kd, dim - 1);
-    uint8_t v = 0;
-    for (int p = lo; p <= hi && !v; ++p) {
-      const int64_t idx = c + (int64_t)(p - pos) * stride;
-      v = in ? in[idx] : (uint8_t)(cs[idx + 1] > cs[idx]);
-    }
-    out[c] = v;
+    const uint64_t m = __ballot(occ);
+    const int wx = (x - lane) / 32 + lane;  // lanes 0 and 1 write the segment's two words
+    if (lane < 2 && wx < W) o[row * W + wx] = (uint32_t)(m >> (32 * lane));
   }
 }
 
-__global__ void k_set_near_r(GridDesc* grids, int n, float r) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) grids[i].near_r = r;
+// Dilation along x: out bit x = OR of in bits x - kd .. x + kd of the row.
+__global__ void k_near_x(const GridDesc* __restrict__ grids, const uint32_t* __restrict__ in,
+                         uint32_t* __restrict__ out, float r) {
+  const GridDesc& gd = grids[blockIdx.y];
+  const int W = near_row_words(gd.nx);
+  const int64_t nwords = (int64_t)W * gd.ny * gd.nz;
+  const int kd = (int)ceilf(r * 1.0001f * gd.inv_h) + 1;
+  const uint32_t* i0 = in + gd.near_off;
+  uint32_t* o = out + gd.near_off;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    const int wx = (int)(w % W);
+    const uint32_t m = i0[w];
+    const uint32_t L = wx > 0 ? i0[w - 1] : 0u;
+    const uint32_t R = wx + 1 < W ? i0[w + 1] : 0u;
+    uint32_t acc = m;
+    for (int d = 1; d <= kd; ++d) acc |= (m << d) | (L >> (32 - d)) | (m >> d) | (R << (32 - d));
+    o[w] = acc;
+  }
+}
+
+// Dilation along y (axis 1) or z (axis 2): whole words of the rows within kd.
+__global__ void k_near_yz(const GridDesc* __restrict__ grids, const uint32_t* __restrict__ in,
+                          uint32_t* __restrict__ out, int axis, float r) {
+  const GridDesc& gd = grids[blockIdx.y];
+  const int W = near_row_words(gd.nx);
+  const int64_t nwords = (int64_t)W * gd.ny * gd.nz;
+  const int kd = (int)ceilf(r * 1.0001f * gd.inv_h) + 1;
+  const int dim = axis == 1 ? gd.ny : gd.nz;
+  const int64_t stride = axis == 1 ? (int64_t)W : (int64_t)W * gd.ny;
+  const uint32_t* i0 = in + gd.near_off;
+  uint32_t* o = out + gd.near_off;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    const int pos = axis == 1 ? (int)((w / W) % gd.ny) : (int)(w / ((int64_t)W * gd.ny));
+    const int lo = max(pos - kd, 0), hi = min(pos + kd, dim - 1);
+    uint32_t acc = 0u;
+    for (int p = lo; p <= hi; ++p) acc |= i0[w + (int64_t)(p - pos) * stride];
+    o[w] = acc;
+  }
+}
+
+// Every grid's near_r and the first word of its mask (the host computes the same).
+__global__ void k_set_near(GridDesc* grids, int n, float r) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    grids[i].near_r = r;
+    grids[i].near_off = off;
+    off += (int64_t)near_row_words(grids[i].nx) * grids[i].ny * grids[i].nz;
+  }
 }
 
 // sorted-order normals back to input order: raw[cloud_off[c] + id] = nrm[j]
@@ -400,23 +449,21 @@ void launch_unsort_normals(const float4* pts, const float4* nrm, const int* clou
                        cloud_of_sorted, cloud_off, n_sorted, raw_nrm);
 }
 
-void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_cells,
-                      float r, uint8_t* out, uint8_t* tmp, hipStream_t s) {
-  if (total_cells <= 0 || ngrids <= 0) return;
-  // per grid a row of blocks, grid-stride over its cells
-  const unsigned bx = (unsigned)std::min<int64_t>(blocks_for(total_cells / ngrids + 1, 256), 512);
+void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_words,
+                      float r, uint32_t* out, uint32_t* tmp, hipStream_t s) {
+  if (total_words <= 0 || ngrids <= 0) return;
+  // per grid a row of blocks, grid-stride over its words
+  const unsigned bx = (unsigned)std::min<int64_t>(blocks_for(total_words / ngrids + 1, 256), 256);
   const dim3 grid(bx, (unsigned)ngrids);
-  hipLaunchKernelGGL(k_near_pass, grid, dim3(256), 0, s, grids, cell_start, (const uint8_t*)nullptr,
-                     out, 0, r);
-  hipLaunchKernelGGL(k_near_pass, grid, dim3(256), 0, s, grids, cell_start, (const uint8_t*)out, tmp,
-                     1, r);
-  hipLaunchKernelGGL(k_near_pass, grid, dim3(256), 0, s, grids, cell_start, (const uint8_t*)tmp, out,
-                     2, r);
+  hipLaunchKernelGGL(k_near_occ, grid, dim3(256), 0, s, grids, cell_start, tmp);
+  hipLaunchKernelGGL(k_near_x, grid, dim3(256), 0, s, grids, (const uint32_t*)tmp, out, r);
+  hipLaunchKernelGGL(k_near_yz, grid, dim3(256), 0, s, grids, (const uint32_t*)out, tmp, 1, r);
+  hipLaunchKernelGGL(k_near_yz, grid, dim3(256), 0, s, grids, (const uint32_t*)tmp, out, 2, r);
 }
 
-void launch_set_near_r(GridDesc* grids, int ngrids, float r, hipStream_t s) {
+void launch_set_near(GridDesc* grids, int ngrids, float r, hipStream_t s) {
   if (ngrids <= 0) return;
-  hipLaunchKernelGGL(k_set_near_r, dim3((ngrids + 63) / 64), dim3(64), 0, s, grids, ngrids, r);
+  hipLaunchKernelGGL(k_set_near, dim3(1), dim3(64), 0, s, grids, ngrids, r);
 }
 
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {

@@ -63,11 +63,13 @@ __device__ __forceinline__ void xform(const double* T, const float4& s, float& q
 // False only when the query lies in a grid cell that the near mask marks as
 // holding no point within GridDesc::near_r; queries outside the grid return true
 // (the search handles them).
-__device__ __forceinline__ bool near_cell(const GridDesc& g, const uint8_t* __restrict__ near,
+__device__ __forceinline__ bool near_cell(const GridDesc& g, const uint32_t* __restrict__ near,
                                           float qx, float qy, float qz) {
   const CellPos c = cell_pos(g, qx, qy, qz);
   if (c.cx < 0 || c.cy < 0 || c.cz < 0 || c.cx >= g.nx || c.cy >= g.ny || c.cz >= g.nz) return true;
-  return near[g.cell_off + ((int64_t)c.cz * g.ny + c.cy) * g.nx + c.cx] != 0;
+  const uint32_t w =
+      near[g.near_off + ((int64_t)c.cz * g.ny + c.cy) * near_row_words(g.nx) + (c.cx >> 5)];
+  return ((w >> (c.cx & 31)) & 1u) != 0;
 }
 
 struct TileRange {

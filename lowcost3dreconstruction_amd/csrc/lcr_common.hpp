@@ -37,7 +37,11 @@ struct GridDesc {
   int64_t node_off;   // first node of this cloud's bounding-volume tree (2 float4 each)
   int depth;          // tree depth: leaves are nodes [2^depth, 2^(depth+1)), 1-based
   float near_r;       // radius of the cell mask "some point within near_r" (0 = no mask)
+  int64_t near_off;   // first 32-bit word of this grid's cell mask (near_row_words per x-row)
 };
+
+// 32-bit words per x-row of a grid's cell mask: bit x & 31 of word x >> 5.
+__host__ __device__ inline int near_row_words(int nx) { return (nx + 31) >> 5; }
 
 // Points per leaf of the bounding-volume tree over a cloud's points.
 constexpr int kLeaf = 16;

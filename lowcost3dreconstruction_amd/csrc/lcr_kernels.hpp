@@ -25,11 +25,13 @@ void launch_place_points(const float4* raw, const float4* raw_nrm, const int* ci
                          int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
                          int* slot_cell, const int64_t* cloud_off, const int* point_cloud,
                          float4* pts, float4* nrm, hipStream_t s);
-// Dilated occupancy per cell (1 = some point within r of the cell), three
-// separable passes; tmp and out hold total_cells bytes.
-void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_cells,
-                      float r, uint8_t* out, uint8_t* tmp, hipStream_t s);
-void launch_set_near_r(GridDesc* grids, int ngrids, float r, hipStream_t s);
+// Dilated occupancy per cell, one bit (set: some point within r of the cell):
+// occupancy, then three separable passes; tmp and out hold total_words words
+// (the grids' near_off must be set first, launch_set_near).
+void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, int64_t total_words,
+                      float r, uint32_t* out, uint32_t* tmp, hipStream_t s);
+// Sets every grid's near_r and its mask's first word (near_off).
+void launch_set_near(GridDesc* grids, int ngrids, float r, hipStream_t s);
 void launch_unsort_normals(const float4* pts, const float4* nrm, const int* cloud_of_sorted,
                            const int64_t* cloud_off, int64_t n_sorted, float4* raw_nrm,
                            int64_t n_raw, hipStream_t s);
@@ -61,7 +63,8 @@ struct IcpBuffers {
   double* partials;       // ntiles * kMaxVals
   int* prev;              // ntiles * tile: last match (sorted target position) per source
                           // point, -1 = none; seeds the next search (nullable)
-  const uint8_t* near;    // per target cell: some point within GridDesc::near_r (nullable)
+  const uint32_t* near;   // per target cell, one bit: some point within GridDesc::near_r
+                          // (nullable; GridDesc::near_off, near_row_words)
   const int* gate_tiles;  // the tiles the gate's pass scores (every gate_stride-th of an
                           // instance), in tile order (nullable: all, skipping the others)
   int ngate_tiles;
@@ -121,8 +124,8 @@ int fitness_max_blocks_per_cu(int tile);
 int64_t normals_work_ints(int64_t n);
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
-                    int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
-                    int* work, float4* nrm_sorted, hipStream_t s);
+                    int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, double* mom,
+                    int* nsel, int* work, float4* nrm_sorted, hipStream_t s);
 
 // ---- K11-K13 filters before ICP (filters.hip)
 // VoxelGrid lattice: voxel of p = floor(p * inv) - minb (FP32, per axis); index

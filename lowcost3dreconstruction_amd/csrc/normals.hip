@@ -597,7 +597,8 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
 // the query's cell, read from memory (the wavefront's queries share most of
 // them, so the reads hit the caches).  The kk-th smallest FLANN d2 is certified
 // when it lies below D2, the squared distance from the query to the block's
-// boundary (minus the rounding slack), so only d2 < D2 counts.  Pass 1 builds a
+// boundary (minus the rounding slack), so only d2 < D2 counts, and each row of
+// the block is trimmed to the cells that can hold such a point (ball_row).  Pass 1 builds a
 // per-lane histogram of d2 over [0, D2) in NB equal buckets -- on a surface the
 // number of points within a radius grows with its square, so the buckets fill
 // about evenly -- and finds the bucket B that holds the kk-th.  Pass 2 lists
@@ -614,47 +615,79 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
 // block holds too few; dense spots overflow bucket B) to a list, which the
 // wavefront pass below serves.
 constexpr int kLaneMaxCand = 4096;
+
 constexpr int kLaneThreads = 256;
 
-// Sorted-array start and length of the (2K+1)^2 x-runs ((2K+1) cells each, clipped
-// to the grid) of the block of radius K around cp: the cell-table loads are
-// issued together.
+// Sorted-array start and length of the (2K+1)^2 x-runs (at most 2K+1 cells each)
+// of the block of radius K around cp: the cell-table loads are issued together.
 template <int K>
 struct BlockRuns {
   static constexpr int R = (2 * K + 1) * (2 * K + 1);
   int s[R], n[R];
 };
 
+// The cells of row (z, y) of the block of radius K that may hold a point with d2
+// below D2: [xa, xb] (empty when xa > xb).  Each cell's lower-bound distance per
+// axis is axis_gap (cell units, less the rounding slack), and the bound is
+// widened by 2e-5 for the float rounding of d2, as the ring search prunes.
+__device__ __forceinline__ void ball_row(const GridDesc& g, const CellPos& cp, int K, int z, int y,
+                                         float D2, int& xa, int& xb) {
+  const float gz = axis_gap(cp.fz, z, cp.cz, cp.slack), gy = axis_gap(cp.fy, y, cp.cy, cp.slack);
+  const float thr = D2 * 1.00002f;
+  const float lyz = (gz * gz + gy * gy) * (g.h * g.h);
+  const float dxc = sqrtf(fmaxf(thr - lyz, 0.0f)) * g.inv_h * 1.00001f + cp.slack;
+  xa = max(max(cp.cx - K, (int)ceilf(cp.fx - dxc - 1.0f)), 0);
+  xb = min(min(cp.cx + K, (int)floorf(cp.fx + dxc)), g.nx - 1);
+  if (z < 0 || z >= g.nz || y < 0 || y >= g.ny || lyz > thr) xb = xa - 1;
+}
+
+// The runs of the block of radius K trimmed to the cells that may hold a point
+// with d2 below D2 (ball_row): only those can count towards the certificate.
 template <int K>
-__device__ __forceinline__ BlockRuns<K> block_runs(const GridDesc& g, const int* cs, const CellPos& cp) {
+__device__ __forceinline__ BlockRuns<K> block_runs(const GridDesc& g, const int* cs, const CellPos& cp,
+                                                   float D2) {
   BlockRuns<K> b;
-  const int x0 = max(cp.cx - K, 0), x1 = min(cp.cx + K, g.nx - 1);
 #pragma unroll
   for (int r = 0; r < BlockRuns<K>::R; ++r) {
     const int z = cp.cz + r / (2 * K + 1) - K, y = cp.cy + r % (2 * K + 1) - K;
-    const bool in = z >= 0 && z < g.nz && y >= 0 && y < g.ny && x0 <= x1;
+    int xa, xb;
+    ball_row(g, cp, K, z, y, D2, xa, xb);
+    const bool in = xa <= xb;
     const int row = in ? (z * g.ny + y) * g.nx : 0;
-    const int s = in ? cs[row + x0] : 0;
-    const int e = in ? cs[row + x1 + 1] : 0;
+    const int s = in ? cs[row + xa] : 0;
+    const int e = in ? cs[row + xb + 1] : 0;
     b.s[r] = s;
     b.n[r] = e - s;
   }
   return b;
 }
 
-// Calls f(p, i) for every point i of the block's runs, four loads in flight.
-template <int K, class F>
-__device__ __forceinline__ void for_block(const BlockRuns<K>& b, const float4* pts, F&& f) {
+// The cell-sorted points as a raw buffer (gfx9 resource word 3): a load is one
+// 32-bit offset add, and a load past the end returns zeros instead of faulting.
+// The caller keeps n * 16 below 2^31 bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t points_rsrc(const float4* pts, int64_t n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)pts, (short)0, (int)(n * 16), 0x00020000);
+}
+
+__device__ __forceinline__ float4 load_point(__amdgpu_buffer_rsrc_t r, int i) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, i * 16, 0, 0));
+}
+
+// Calls f(p, i, valid) for the points i of the block's runs, NB loads in
+// flight; the loads of a run's last batch read on past its end (the next points,
+// or zeros past the buffer) with valid = false.
+template <int K, int NBATCH, class F>
+__device__ __forceinline__ void for_block(const BlockRuns<K>& b, __amdgpu_buffer_rsrc_t rs,
+                                          F&& f) {
 #pragma unroll
   for (int r = 0; r < BlockRuns<K>::R; ++r) {
     const int s = b.s[r], e = s + b.n[r];
-    for (int i = s; i < e; i += 4) {
-      float4 p[4];
+    for (int i = s; i < e; i += NBATCH) {
+      float4 p[NBATCH];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) p[u] = pts[min(i + u, e - 1)];
+      for (int u = 0; u < NBATCH; ++u) p[u] = load_point(rs, i + u);
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i + u < e) f(p[u], i + u);
+      for (int u = 0; u < NBATCH; ++u) f(p[u], i + u, i + u < e);
     }
   }
 }
@@ -702,7 +735,7 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 
 template <int NB, int NL>
 constexpr int lane_words() {
-  return 3 * NL > NB / 2 ? 3 * NL : NB / 2;
+  return 3 * NL > NB ? 3 * NL : NB;
 }
 
 // nsel bit: the query's list holds original indices into the cloud's raw points
@@ -710,21 +743,25 @@ constexpr int lane_words() {
 constexpr int kNselOrig = 1 << 30;
 
 // One query of the lane path. K: block radius in cells; NB: histogram buckets;
-// NL: bucket-B capacity; buf: the lane's LDS column (lane_words words).
-// Returns true when the query is left to the next path (nothing written).
+// NL: bucket-B capacity; buf: the lane's LDS column (lane_words words).  Writes
+// the FP64 moments of the selected neighbours relative to the query (mom, nine
+// arrays of n) and nsel, and returns 0; or leaves the query to the next path
+// (nothing written) and returns the block radius that path starts from: K + 1
+// when this block held too few points or no certificate, K otherwise.
 template <int K, int NB, int NL>
-__device__ __forceinline__ bool lane_query(const GridDesc* __restrict__ grids,
-                                           const int* __restrict__ cell_start,
-                                           const float4* __restrict__ pts,
-                                           const int* __restrict__ cloud_of_sorted, int k, int64_t j,
-                                           uint32_t (*buf)[64], int lane, int* __restrict__ nbr,
-                                           int* __restrict__ nsel) {
+__device__ __forceinline__ int lane_query(const GridDesc* __restrict__ grids,
+                                          const int* __restrict__ cell_start,
+                                          const float4* __restrict__ pts,
+                                          __amdgpu_buffer_rsrc_t rs,
+                                          const int* __restrict__ cloud_of_sorted, int k, int64_t j,
+                                          uint32_t (*buf)[64], int lane, double* __restrict__ mom,
+                                          int64_t n, int* __restrict__ nsel) {
   const float4 q = pts[j];
   const GridDesc g = grids[cloud_of_sorted[j]];
   const int kk = min(k, g.npts);
   if (kk < 3) {
     nsel[j] = kk;
-    return false;
+    return 0;
   }
   const int* cs = cell_start + g.cell_off;
   const CellPos cp = cell_pos(g, q.x, q.y, q.z);
@@ -735,54 +772,47 @@ __device__ __forceinline__ bool lane_query(const GridDesc* __restrict__ grids,
   // blocks that cover the grid (no boundary to certify against) and gaps inside
   // the rounding slack go on; a finite scale keeps every bucket index of a
   // d2 < D2 inside [0, NB]
-  bool ok = Lc < INFINITY && L > 0.0f && D2 > 1e-30f && scale < INFINITY;
-  BlockRuns<K> blk;
-  if (ok) {
-    blk = block_runs<K>(g, cs, cp);
-    int m = 0;
+  if (!(Lc < INFINITY && L > 0.0f && D2 > 1e-30f && scale < INFINITY)) return K;
+  const BlockRuns<K> blk = block_runs<K>(g, cs, cp, D2);
+  int m = 0;
 #pragma unroll
-    for (int r = 0; r < BlockRuns<K>::R; ++r) m += blk.n[r];
-    ok = m >= kk && m <= kLaneMaxCand;
-  }
-  int B = -1, below = 0, nB = 0;
-  if (ok) {
+  for (int r = 0; r < BlockRuns<K>::R; ++r) m += blk.n[r];
+  if (m < kk) return K + 1;
+  if (m > kLaneMaxCand) return K;
 #pragma unroll
-    for (int c = 0; c < NB / 2; ++c) buf[c][lane] = 0u;
-    for_block<K>(blk, pts, [&](const float4& p, int) {
-      const float d = flann_d2(q.x, q.y, q.z, p);
-      if (d < D2) {
-        const int b = min((int)(d * scale), NB - 1);
-        atomicAdd(&buf[b >> 1][lane], 1u << ((b & 1) * 16));
-      }
-    });
-    int cum = 0;
-#pragma unroll
-    for (int c = 0; c < NB / 2; ++c) {
-      const uint32_t v = buf[c][lane];
-      const int lo = (int)(v & 0xffffu), hi = (int)(v >> 16);
-      if (B < 0 && cum + lo >= kk) {
-        B = 2 * c;
-        below = cum;
-        nB = lo;
-      } else if (B < 0 && cum + lo + hi >= kk) {
-        B = 2 * c + 1;
-        below = cum + lo;
-        nB = hi;
-      }
-      cum += lo + hi;
-    }
-    ok = B >= 0 && nB <= NL;
-  }
-  if (!ok) return true;
-  // the list holds sorted positions (the solve pass reads the cell-sorted copy)
-  int* out = nbr + j * k;
-  int pos = 0, nl = 0;
-  for_block<K>(blk, pts, [&](const float4& p, int i) {
+  for (int c = 0; c < NB; ++c) buf[c][lane] = 0u;
+  // pass 1, branch-free: every candidate adds 0 or 1 to its bucket
+  for_block<K, 4>(blk, rs, [&](const float4& p, int, bool valid) {
     const float d = flann_d2(q.x, q.y, q.z, p);
-    if (d < D2) {
+    const int b = (int)fminf(d * scale, (float)(NB - 1));
+    atomicAdd(&buf[b][lane], (valid && d < D2) ? 1u : 0u);
+  });
+  int B = -1, below = 0, nB = 0, cum = 0;
+  // four counts at a time: unrolled whole, the 32 reads would all be in flight
+#pragma unroll 4
+  for (int c = 0; c < NB; ++c) {
+    const int v = (int)buf[c][lane];
+    if (B < 0 && cum + v >= kk) {
+      B = c;
+      below = cum;
+      nB = v;
+    }
+    cum += v;
+  }
+  if (B < 0) return K + 1;  // not certified
+  if (nB > NL) return K;
+  // the moments of the lower buckets' points in block order, then of bucket B's
+  // picks in (d2, original index) order
+  Moments acc;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) acc.v[c] = 0.0;
+  int nl = 0;
+  for_block<K, 4>(blk, rs, [&](const float4& p, int i, bool valid) {
+    const float d = flann_d2(q.x, q.y, q.z, p);
+    if (valid && d < D2) {
       const int b = min((int)(d * scale), NB - 1);
       if (b < B) {
-        out[pos++] = i;
+        add_moment(acc, q, p);
       } else if (b == B && nl < NL) {
         buf[3 * nl][lane] = __float_as_uint(d);
         buf[3 * nl + 1][lane] = (uint32_t)__float_as_int(p.w);
@@ -802,23 +832,26 @@ __device__ __forceinline__ bool lane_query(const GridDesc* __restrict__ grids,
   const int need = kk - below;
 #pragma unroll
   for (int t = 0; t < NL; ++t)
-    if (t < need) out[pos + t] = ei[t];
+    if (t < need) add_moment(acc, q, load_point(rs, ei[t]));
+#pragma unroll
+  for (int c = 0; c < 9; ++c) mom[c * n + j] = acc.v[c];
   nsel[j] = kk;
-  return false;
+  return 0;
 }
 
-// Appends the lanes where take holds to list (list[*count ...]), one atomic per
-// wavefront; every lane of the wavefront calls it.  The order of the list
-// depends on scheduling; every consumer's result per query does not.
-__device__ __forceinline__ void wave_append(bool take, int64_t j, int* __restrict__ list,
-                                            int* __restrict__ count) {
+// Appends the lanes where take holds to list (list[*count ...]: the point and
+// the block radius to start from), one atomic per wavefront; every lane of the
+// wavefront calls it.  The order of the list depends on scheduling; every
+// consumer's result per query does not.
+__device__ __forceinline__ void wave_append(bool take, int64_t j, int start_k,
+                                            int2* __restrict__ list, int* __restrict__ count) {
   const uint64_t b = __ballot(take);
   if (b == 0) return;
   const int lead = __ffsll((unsigned long long)b) - 1;
   int base = 0;
   if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(count, __popcll(b));
   base = __builtin_amdgcn_readlane(base, lead);
-  if (take) list[base + lane_prefix(b)] = (int)j;
+  if (take) list[base + lane_prefix(b)] = make_int2((int)j, start_k);
 }
 
 // The lane pass over every point: one lane per point, blocks mapped
@@ -828,22 +861,25 @@ template <int K, int NB, int NL>
 __global__ void __launch_bounds__(kLaneThreads)
 k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
               const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
-              int64_t total, int k, int* __restrict__ nbr, int* __restrict__ nsel,
-              int* __restrict__ list, int* __restrict__ count) {
+              int64_t total, int k, double* __restrict__ mom, int* __restrict__ nsel,
+              int2* __restrict__ list, int* __restrict__ count) {
   __shared__ uint32_t s_buf[kLaneThreads / 64][lane_words<NB, NL>()][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kLaneThreads + threadIdx.x;
-  const bool left = j < total && lane_query<K, NB, NL>(grids, cell_start, pts, cloud_of_sorted, k,
-                                                       j, s_buf[w], lane, nbr, nsel);
-  wave_append(left, j, list, count);
+  const __amdgpu_buffer_rsrc_t rs = points_rsrc(pts, total);
+  const int start_k = j < total ? lane_query<K, NB, NL>(grids, cell_start, pts, rs, cloud_of_sorted,
+                                                        k, j, s_buf[w], lane, mom, total, nsel)
+                                : 0;
+  wave_append(start_k > 0, j, start_k, list, count);
 }
 
 // ---------------------------------------------------------------- wavefront pass
 //
 // The queries the lane pass leaves, one wavefront per query.  The candidates are
-// the (2K+1)^2 x-runs of the block of radius K around the query's cell, taken 64
-// runs at a time; the lanes stride through each chunk's points together
-// (consecutive lanes read consecutive points of a run).
+// the (2K+1)^2 x-runs of the block of radius K around the query's cell, each
+// trimmed to the cells that can hold a point below D2 (ball_row), taken 64 runs
+// at a time; the lanes stride through each chunk's points together (consecutive
+// lanes read consecutive points of a run).
 //
 // Pass 1 counts d2 < D2 (the block's certified range, as in the lane path) in
 // kWaveBuckets equal buckets of a histogram the wavefront shares in LDS.  Fewer
@@ -854,10 +890,12 @@ k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
 // it certifies.  A prefix over the buckets gives bucket B of the kk-th; more
 // than 64 points in B split it into 256 sub-buckets with one more pass (the
 // bucket key floor(256 * d2 * scale) at that level, still monotone in d2).
-// Pass 2 lists the points of lower buckets (ballot order: the run order of the
-// block) and keeps bucket B's (d2, original index, position) in LDS, where each
-// one's rank among them picks the first kk - (count below B).  The selected set
-// is exactly the kk smallest (d2, original index).  A block that covers the
+// Pass 2 adds the points of lower keys to the lane's FP64 moments (each lane its
+// share of the block, in block order) and keeps the kk-th's key's entries (d2,
+// original index, position) in LDS, where each one's rank among them picks the
+// first kk - (count below); a wavefront sum in a fixed order then gives the
+// query's moments.  The selected set is exactly the kk smallest (d2, original
+// index).  A block that covers the
 // grid, a radius above kWaveMaxK, or more than 64 equal keys sends the query to
 // the rescan pass.
 constexpr int kWaveBuckets = 256;
@@ -902,24 +940,26 @@ __device__ __forceinline__ float wave_kth_smallest(float v, int kk) {
 }
 
 // Calls f(valid, p, i) for this lane's share of the points of the block of radius
-// K (i: the sorted position), run chunk by run chunk; every lane takes part in
-// every call (valid = false past a chunk's end), so f may use ballots.
+// K that may lie below D2 (i: the sorted position; rows trimmed by ball_row), run
+// chunk by run chunk; every lane takes part in every call (valid = false past a
+// chunk's end), so f may use ballots.
 template <class F>
 __device__ __forceinline__ void for_wave_block(WaveLds& L, const GridDesc& g, const int* cs,
-                                               const CellPos& cp, int K, const float4* pts,
-                                               F&& f) {
+                                               const CellPos& cp, int K, float D2,
+                                               __amdgpu_buffer_rsrc_t rs, F&& f) {
   const int lane = threadIdx.x & 63;
   const int side = 2 * K + 1, R = side * side;
-  const int x0 = max(cp.cx - K, 0), x1 = min(cp.cx + K, g.nx - 1);
   for (int r0 = 0; r0 < R; r0 += 64) {
     const int r = r0 + lane;
     int st = 0, len = 0;
-    if (r < R && x0 <= x1) {
+    if (r < R) {
       const int z = cp.cz + r / side - K, y = cp.cy + r % side - K;
-      if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) {
+      int xa, xb;
+      ball_row(g, cp, K, z, y, D2, xa, xb);
+      if (xa <= xb) {
         const int row = (z * g.ny + y) * g.nx;
-        st = cs[row + x0];
-        len = cs[row + x1 + 1] - st;
+        st = cs[row + xa];
+        len = cs[row + xb + 1] - st;
       }
     }
     const int incl = wave_incl_scan(len);
@@ -930,18 +970,24 @@ __device__ __forceinline__ void for_wave_block(WaveLds& L, const GridDesc& g, co
     L.pref[lane + 1] = incl;
     if (lane == 0) L.pref[0] = 0;
     wave_lds_sync();
+    // two points per lane in flight; t past the chunk's end maps to position 0
+    // (loaded, not used)
     int rr = 0;
-    for (int base = 0; base < total; base += 64) {
-      const int t = base + lane;
-      const bool valid = t < total;
-      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-      int i = 0;
-      if (valid) {
-        while (t >= L.pref[rr + 1]) ++rr;  // runs may be empty
-        i = L.start[rr] + (t - L.pref[rr]);
-        p = pts[i];
+    for (int base = 0; base < total; base += 128) {
+      const int t0 = base + lane, t1 = t0 + 64;
+      const bool v0 = t0 < total, v1 = t1 < total;
+      int i0 = 0, i1 = 0;
+      if (v0) {
+        while (t0 >= L.pref[rr + 1]) ++rr;  // runs may be empty
+        i0 = L.start[rr] + (t0 - L.pref[rr]);
       }
-      f(valid, p, i);
+      if (v1) {
+        while (t1 >= L.pref[rr + 1]) ++rr;
+        i1 = L.start[rr] + (t1 - L.pref[rr]);
+      }
+      const float4 p0 = load_point(rs, i0), p1 = load_point(rs, i1);
+      f(v0, p0, i0);
+      f(v1, p1, i1);
     }
   }
 }
@@ -982,26 +1028,28 @@ __device__ __forceinline__ bool wave_find_bucket(const WaveLds& L, int kk, int& 
   return true;
 }
 
-__global__ void __launch_bounds__(kWaveThreads)
+__global__ void __launch_bounds__(kWaveThreads, 5)
 k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_start,
               const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted, int k,
-              const int* __restrict__ list, const int* __restrict__ count, int* __restrict__ nbr,
-              int* __restrict__ nsel, int* __restrict__ deferred) {
+              const int2* __restrict__ list, const int* __restrict__ count, int64_t total,
+              double* __restrict__ mom, int* __restrict__ nsel, int* __restrict__ deferred) {
   __shared__ WaveLds s_w[kWaveThreads / 64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   WaveLds& L = s_w[w];
   const int n = *count;
   const int nwaves = (int)gridDim.x * (kWaveThreads / 64);
+  const __amdgpu_buffer_rsrc_t rs = points_rsrc(pts, total);
   for (int qi = (int)blockIdx.x * (kWaveThreads / 64) + w; qi < n; qi += nwaves) {
-    const int64_t j = __builtin_amdgcn_readfirstlane(list[qi]);
+    const int2 e = list[qi];
+    const int64_t j = __builtin_amdgcn_readfirstlane(e.x);
     const float4 q = pts[j];
     const GridDesc g = grids[cloud_of_sorted[j]];
     const int kk = min(k, g.npts);  // at least 3: the lane pass serves the others
     const int* cs = cell_start + g.cell_off;
     const CellPos cp = cell_pos(g, q.x, q.y, q.z);
     bool done = false;
-    int K = 1;
+    int K = __builtin_amdgcn_readfirstlane(e.y);
     for (int attempt = 0; attempt < 8 && !done && K <= kWaveMaxK; ++attempt) {
       const float Lc = block_gap(g, cp, K);
       if (Lc == INFINITY) break;  // covers the grid: the rescan pass
@@ -1018,7 +1066,7 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
       // pass 1: the histogram of d2 < D2, and the lane's count, least and largest d2
       int mc = 0;
       float dmin = INFINITY, dmax = 0.0f;
-      for_wave_block(L, g, cs, cp, K, pts, [&](bool valid, const float4& p, int) {
+      for_wave_block(L, g, cs, cp, K, D2, rs, [&](bool valid, const float4& p, int) {
         if (!valid) return;
         const float d = flann_d2(q.x, q.y, q.z, p);
         ++mc;
@@ -1049,7 +1097,7 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
 #pragma unroll
         for (int c = lane; c < kWaveBuckets; c += 64) L.hist[c] = 0;
         wave_lds_sync();
-        for_wave_block(L, g, cs, cp, K, pts, [&](bool valid, const float4& p, int) {
+        for_wave_block(L, g, cs, cp, K, D2, rs, [&](bool valid, const float4& p, int) {
           if (!valid) return;
           const float d = flann_d2(q.x, q.y, q.z, p);
           if (!(d < D2)) return;
@@ -1061,25 +1109,26 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
         wave_find_bucket(L, kk, below, SB, nB);
         if (nB > 64) break;  // more than 64 equal keys: the rescan pass
       }
-      // pass 2: list the lower keys, keep the kk-th's key in LDS
-      int* out = nbr + j * k;
-      int pos = 0, nl = 0;
-      for_wave_block(L, g, cs, cp, K, pts, [&](bool valid, const float4& p, int i) {
+      // pass 2: the moments of the lower keys, the kk-th's key kept in LDS
+      Moments acc;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc.v[c] = 0.0;
+      int nl = 0;
+      for_wave_block(L, g, cs, cp, K, D2, rs, [&](bool valid, const float4& p, int i) {
         const float d = valid ? flann_d2(q.x, q.y, q.z, p) : INFINITY;
         const float x = d * scale;
         const int b = d < D2 ? min((int)x, kWaveBuckets - 1) : kWaveBuckets;
         const int sb = (split && b == B) ? min((int)((x - (float)B) * 256.0f), 255) : 0;
         const bool lt = b < B || (b == B && sb < SB);
         const bool eq = b == B && sb == SB;
-        const uint64_t blt = __ballot(lt), beq = __ballot(eq);
-        if (lt) out[pos + lane_prefix(blt)] = i;
+        const uint64_t beq = __ballot(eq);
+        if (lt) add_moment(acc, q, p);
         if (eq) {
           const int slot = nl + lane_prefix(beq);
           L.key[slot] = __float_as_uint(d);
           L.idx[slot] = (uint32_t)__float_as_int(p.w);
           L.pos[slot] = i;
         }
-        pos += __popcll(blt);
         nl += __popcll(beq);
       });
       wave_lds_sync();
@@ -1088,7 +1137,12 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
         const uint64_t mine = ((uint64_t)L.key[lane] << 32) | L.idx[lane];
         int rank = 0;
         for (int u = 0; u < nl; ++u) rank += (((uint64_t)L.key[u] << 32) | L.idx[u]) < mine ? 1 : 0;
-        if (rank < kk - below) out[pos + rank] = L.pos[lane];
+        if (rank < kk - below) add_moment(acc, q, load_point(rs, L.pos[lane]));
+      }
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const double v = wave_sum_dpp(acc.v[c]);
+        if (lane == 0) mom[c * total + j] = v;
       }
       if (lane == 0) nsel[j] = kk;
       done = true;
@@ -1184,18 +1238,17 @@ k_normal_rescan(const GridDesc* __restrict__ grids, const int* __restrict__ cell
   }
 }
 
-// Solve pass, one point per lane: FP64 moments of its neighbours relative to it
-// (in the order the search listed them), covariance, FP64 eigen-decomposition,
-// normal (eigenvector of the smallest eigenvalue) oriented towards the viewpoint.
-// The lane and wavefront passes list sorted positions (neighbouring queries read
-// neighbouring points of the cell-sorted copy); the rescan pass (kNselOrig set)
-// lists original indices into the cloud's raw points.  Both hold the same
-// coordinates.
+// Solve pass, one point per lane: the FP64 moments of its neighbours relative to
+// it (from the lane and wavefront passes, or summed here in list order from the
+// rescan pass's list of original indices, kNselOrig set), covariance, FP64
+// eigen-decomposition, normal (eigenvector of the smallest eigenvalue) oriented
+// towards the viewpoint.
 __global__ void __launch_bounds__(256)
 k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
                const float4* __restrict__ raw, const int64_t* __restrict__ cloud_off,
-               const int* __restrict__ nbr, int k, const int* __restrict__ nsel, int64_t total,
-               const float* __restrict__ vp, int flip_all, float4* __restrict__ out) {
+               const int* __restrict__ nbr, int k, const double* __restrict__ mom,
+               const int* __restrict__ nsel, int64_t total, const float* __restrict__ vp,
+               int flip_all, float4* __restrict__ out) {
   const int64_t jq = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (jq >= total) return;
   const int ns = nsel[jq];
@@ -1205,12 +1258,17 @@ k_normal_solve(const float4* __restrict__ pts, const int* __restrict__ cloud_of_
     return;
   }
   const float4 q = pts[jq];
-  const float4* R = (ns & kNselOrig) ? raw + cloud_off[cloud_of_sorted[jq]] : pts;
-  const int* list = nbr + jq * k;
   Moments acc;
+  if (ns & kNselOrig) {
+    const float4* R = raw + cloud_off[cloud_of_sorted[jq]];
+    const int* list = nbr + jq * k;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
-  for (int i = 0; i < cnt; ++i) add_moment(acc, q, R[list[i]]);
+    for (int i = 0; i < 9; ++i) acc.v[i] = 0.0;
+    for (int i = 0; i < cnt; ++i) add_moment(acc, q, R[list[i]]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc.v[i] = mom[i * total + jq];
+  }
   const double* m = acc.v;
   const double n = (double)cnt;
   const double mx = m[0] / n, my = m[1] / n, mz = m[2] / n;
@@ -1276,33 +1334,34 @@ void launch_sor_distances(const GridDesc* grids, const int* cell_start, const fl
                      cloud_of_sorted, cloud_off, sums, npts_total, mean_k, dist, finite);
 }
 
-int64_t normals_work_ints(int64_t n) { return 2 * n + 2; }
+int64_t normals_work_ints(int64_t n) { return 4 * n + 3; }
 
 void launch_normals(const GridDesc* grids, const int* cell_start, const float4* pts,
                     const int* cloud_of_sorted, int64_t npts_total, int k, const float* vp,
-                    int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, int* nsel,
-                    int* work, float4* nrm_sorted, hipStream_t s) {
+                    int flip_all, const float4* raw, const int64_t* cloud_off, int* nbr, double* mom,
+                    int* nsel, int* work, float4* nrm_sorted, hipStream_t s) {
   if (npts_total <= 0) return;
   const int64_t n = npts_total;
   // work: [0] the lane pass's list count, [1] the rescan count with its list at
-  // [2, 2 + n), the lane pass's list at [2 + n, 2 + 2n)
+  // [2, 2 + n), the lane pass's list of (point, start radius) at [2 + n, 2 + 3n)
   int* count = work;
   int* deferred = work + 1;
-  int* list = work + 2 + n;
+  int2* list = reinterpret_cast<int2*>(work + 2 + n + (n & 1));
   launch_fill_i32(work, 2, 0, s);
   hipLaunchKernelGGL((k_normal_lane<1, 32, 8>), dim3((unsigned)((n + kLaneThreads - 1) / kLaneThreads)),
-                     dim3(kLaneThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, n, k, nbr,
+                     dim3(kLaneThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, n, k, mom,
                      nsel, list, count);
   // persistent over the list (a few percent of the points): at most eight blocks
   // of four wavefronts per CU
   const int64_t wave_blocks = (n + kWaveThreads / 64 - 1) / (kWaveThreads / 64);
   hipLaunchKernelGGL(k_normal_wave, dim3((unsigned)(wave_blocks < 2048 ? wave_blocks : 2048)),
                      dim3(kWaveThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, k,
-                     (const int*)list, (const int*)count, nbr, nsel, deferred);
+                     (const int2*)list, (const int*)count, n, mom, nsel, deferred);
   hipLaunchKernelGGL(k_normal_rescan<kMoments>, dim3(1024), dim3(256), 0, s, grids, cell_start, pts,
                      cloud_of_sorted, k, deferred, (double*)nullptr, nbr);
   hipLaunchKernelGGL(k_normal_solve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts,
-                     cloud_of_sorted, raw, cloud_off, nbr, k, nsel, n, vp, flip_all, nrm_sorted);
+                     cloud_of_sorted, raw, cloud_off, nbr, k, (const double*)mom, nsel, n, vp,
+                     flip_all, nrm_sorted);
 }
 
 }  // namespace lcr

@@ -263,6 +263,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
     }
     g.cell_off = cells;
     g.near_r = 0.0f;
+    g.near_off = 0;
     cells += (int64_t)g.nx * g.ny * g.nz + 1;
   }
   d_grids.reserve((size_t)n * sizeof(GridDesc));
@@ -314,6 +315,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
     }
     g.cell_off = cells;
     g.near_r = 0.0f;
+    g.near_off = 0;
     cells += (int64_t)g.nx * g.ny * g.nz + 1;
   }
   if (cells > (int64_t)1 << 30) throw Error{-5, "grid too large"};
@@ -404,15 +406,18 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
     for (int a = 0; a < 3; ++a) v[3 * c + a] = vp_host ? vp_host[3 * c + a] : 0.0f;
   LCR_HIP(hipMemcpyAsync(d_vp.p, v, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
   LCR_HIP(hipEventRecord(ev_vp, s));
+  // the selection reads the sorted points through a raw buffer of 32-bit byte offsets
+  if (total_pts > ((int64_t)1 << 27) - 1) throw Error{-5, "more than 2^27 points in one normals batch"};
   const int span = prof.begin("normals", s, spans);
   const size_t np = (size_t)std::max<int64_t>(total_pts, 1);
   d_nbr.reserve(np * (size_t)k * sizeof(int));
   d_nsel.reserve(np * sizeof(int));
+  d_nmom.reserve(np * 9 * sizeof(double));
   d_nwork.reserve((size_t)normals_work_ints((int64_t)np) * sizeof(int));
   launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
                  d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
-                 d_raw.as<float4>(), d_cloud_off.as<int64_t>(), d_nbr.as<int>(), d_nsel.as<int>(),
-                 d_nwork.as<int>(), d_nrm.as<float4>(), s);
+                 d_raw.as<float4>(), d_cloud_off.as<int64_t>(), d_nbr.as<int>(), d_nmom.as<double>(),
+                 d_nsel.as<int>(), d_nwork.as<int>(), d_nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
 }
@@ -425,14 +430,20 @@ void CloudSet::build_near_mask(hipStream_t s, double r) {
   for (const GridDesc& g : grids)
     if (g.npts > 0) kd_max = std::max(kd_max, (int)ceil(r * 1.0001 * (double)g.inv_h) + 1);
   const bool use = std::isfinite(r) && r > 0.0 && kd_max <= kMaxNearCells && total_cells > 0;
-  for (GridDesc& g : grids) g.near_r = use ? (float)r : 0.0f;
-  if (use) {
-    d_near.reserve((size_t)total_cells);
-    d_near_tmp.reserve((size_t)total_cells);
-    launch_near_mask(d_grids.as<GridDesc>(), n, d_cell_start.as<int>(), total_cells, (float)r,
-                     d_near.as<uint8_t>(), d_near_tmp.as<uint8_t>(), s);
+  // the same word offsets launch_set_near gives the device copy
+  int64_t words = 0;
+  for (GridDesc& g : grids) {
+    g.near_r = use ? (float)r : 0.0f;
+    g.near_off = words;
+    words += (int64_t)near_row_words(g.nx) * g.ny * g.nz;
   }
-  launch_set_near_r(d_grids.as<GridDesc>(), n, use ? (float)r : 0.0f, s);
+  launch_set_near(d_grids.as<GridDesc>(), n, use ? (float)r : 0.0f, s);
+  if (use) {
+    d_near.reserve((size_t)std::max<int64_t>(words, 1) * sizeof(uint32_t));
+    d_near_tmp.reserve((size_t)std::max<int64_t>(words, 1) * sizeof(uint32_t));
+    launch_near_mask(d_grids.as<GridDesc>(), n, d_cell_start.as<int>(), words, (float)r,
+                     d_near.as<uint32_t>(), d_near_tmp.as<uint32_t>(), s);
+  }
 }
 
 void CloudSet::normals_to_raw(hipStream_t s) {
@@ -535,7 +546,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.state = d_state.as<PairState>();
   b.partials = d_partials.as<double>();
   b.prev = d_prev.as<int>();
-  b.near = cs.d_near.p ? cs.d_near.as<uint8_t>() : nullptr;
+  b.near = cs.d_near.p ? cs.d_near.as<uint32_t>() : nullptr;
   b.gate_tiles = d_gate_tiles.as<int>();
   b.ngate_tiles = ngate_tiles;
   b.it_list = d_it_list.as<int2>();

@@ -141,7 +141,8 @@ struct CloudSet {
   DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel;
   DevBuf d_tpts, d_tpos, d_tkeys, d_tkeys_sorted, d_tvals, d_ttmp;  // tree order
   DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
-  DevBuf d_nwork;  // normals: lane-pass flags, their scan and the second pass's list
+  DevBuf d_nwork;  // normals: the query lists of the wavefront and rescan passes
+  DevBuf d_nmom;   // normals: FP64 neighbour moments, nine arrays of one per point
   HostBuf h_stage, h_stage_nrm, h_small, h_build, h_vp;
 
   // Packs and uploads clouds (x, y, z at the start of each `stride` bytes record;

@@ -101,6 +101,12 @@ def test_icp_point_to_point_c1_parity(lcr, c1):
     assert abs(icp.getFitnessScore() - ref["fitness"]) <= 1e-6 * ref["fitness"]
     fit_same_T = oracle.fitness(c1.src, c1.tgt, T)
     assert abs(icp.getFitnessScore() - fit_same_T) <= 1e-12 * fit_same_T
+    # and within the bar set against PCL (1e-4 rad, 1e-5 m) of the oracle's
+    # PCL-semantics FP32 mode (float Umeyama, incremental float transforms)
+    pcl = oracle.icp(c1.src, c1.tgt, max_corr_dist=0.05, max_iterations=50, mode=oracle.PCL32)
+    dR = T[:3, :3].T @ pcl["T"][:3, :3]
+    rot = np.arccos(np.clip((np.trace(dR) - 1.0) / 2.0, -1.0, 1.0))
+    assert rot <= 1e-4 and np.linalg.norm(T[:3, 3] - pcl["T"][:3, 3]) <= 1e-5
     # K8: the registered cloud is the source under the final transform (FP64, rounded once)
     np.testing.assert_array_equal(out, oracle.transform_points(c1.src, T))
 
