@@ -106,6 +106,28 @@ __device__ __forceinline__ int cell_of(const GridDesc& g, const float4& p) {
 }
 
 // Per point: global cell index (or -1 for non-finite points) and per-cell counts.
+// counter[key] += 1 for every calling lane, one atomic per distinct key in the
+// wavefront (raw points in image order share cells, so a wavefront holds a few
+// keys); returns the counter's value before the lane's own increment.  It
+// ballots: every lane of the wavefront that is still running must call it.
+__device__ __forceinline__ int wave_counter_add(int* __restrict__ counter, int key) {
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t todo = __ballot(true);
+  int result = 0;
+  while (todo) {
+    const int lead = __ffsll((unsigned long long)todo) - 1;
+    const int k = __shfl(key, lead, 64);
+    const uint64_t same = __ballot(key == k) & todo;
+    int base = 0;
+    if (lane == lead) base = atomicAdd(&counter[k], __popcll(same));
+    base = __shfl(base, lead, 64);
+    if ((same >> lane) & 1ull)
+      result = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return result;
+}
+
 __global__ void k_cell_count(const float4* __restrict__ raw, const int* __restrict__ point_cloud,
                              const GridDesc* __restrict__ grids, int64_t n,
                              int* __restrict__ cid, int* __restrict__ counts,
@@ -121,7 +143,7 @@ __global__ void k_cell_count(const float4* __restrict__ raw, const int* __restri
   const GridDesc g = grids[c];
   const int gc = (int)g.cell_off + cell_of(g, p);
   cid[i] = gc;
-  atomicAdd(&counts[gc], 1);
+  wave_counter_add(counts, gc);
 }
 
 // occupied[cloud] = number of non-empty cells of the cloud's grid (one atomic per
@@ -225,7 +247,7 @@ __global__ void k_scatter(const int* __restrict__ cid, int64_t n, const int* __r
   if (i >= n) return;
   const int c = cid[i];
   if (c < 0) return;
-  const int pos = cell_start[c] + atomicAdd(&fill[c], 1);
+  const int pos = cell_start[c] + wave_counter_add(fill, c);
   slot_raw[pos] = (int)i;
   slot_cell[pos] = c;
 }

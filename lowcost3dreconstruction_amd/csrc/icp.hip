@@ -333,7 +333,13 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
       xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
       const bool empty = bounded && b.near && gt.near_r >= rq && !near_cell(gt, b.near, qx, qy, qz);
       const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
-      if (!empty)
+      // the final pass over refined instances: a point without a match within r in
+      // the last iteration (seed -1) is farther than about r from the target; a
+      // ring search would scan two unpruned rings before handing it to the tree,
+      // so it goes to the tree walk at once
+      if (!bounded && b.prev && seed < 0)
+        nn.tree = true;
+      else if (!empty)
         nn = nn_query(gt, b.cell_start + gt.cell_off, b.pts, b.tree(), qx, qy, qz, r, bounded, seed,
                       2, true);
     }
