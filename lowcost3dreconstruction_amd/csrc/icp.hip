@@ -299,7 +299,6 @@ template <int TILE>
 __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double max_range_arg,
                                                   int per_instance, int interleave) {
   __shared__ double lds[2 * (TILE / 64)];
-  __shared__ int s_last;
   // the gate's score (per_instance) counts a fixed subsample of the source's
   // tiles, the same source points for every hypothesis of a pair: the tiles
   // listed in gate_tiles (a pass over every tile that skipped the others spent
@@ -317,7 +316,6 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
     const PairDesc pd = b.pairs[p];
     const int stride = per_instance ? gate_stride(pd.ntiles) : 1;
     if ((tile - pd.tile_begin) % stride != 0) continue;  // uniform across the block
-    const int nparts = (pd.ntiles + stride - 1) / stride;  // tiles taking part
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
     const double max_range = per_instance ? pd.max_corr_d2 : max_range_arg;
@@ -354,26 +352,32 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
       double* out = b.partials + (int64_t)tile * kMaxVals;
       out[0] = v[0];
       out[1] = v[1];
-      __threadfence();
-      const unsigned t = atomicAdd(&st->ticket, 1u);
-      s_last = (t == (unsigned)(nparts - 1));
     }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      v[0] = v[1] = 0.0;
-      for (int t = threadIdx.x; t < nparts; t += TILE) {
-        const double* in = b.partials + (int64_t)(pd.tile_begin + t * stride) * kMaxVals;
-        v[0] += in[0];
-        v[1] += in[1];
-      }
-      block_sum<2>(v, lds);
-      if (threadIdx.x == 0) {
-        st->fit_count = (int)v[1];
-        st->fitness = v[1] > 0.0 ? v[0] / v[1] : DBL_MAX;
-        st->ticket = 0u;
-      }
-    }
+  }
+}
+
+// The fitness pass's second half, one block per instance that took part (every
+// instance the gate has not set aside): its tiles' partial sums in a fixed order,
+// then the mean.  A separate launch: folding in the pass itself took a
+// device-scope fence and an atomic ticket per tile (each fence writes back the
+// XCD's L2), which cost cycle36 a third of the pass.
+__global__ void __launch_bounds__(kSolveThreads) k_fitness_fold(const IcpBuffers b, int per_instance) {
+  __shared__ double lds[2 * (kSolveThreads / 64)];
+  PairState* st = b.state + blockIdx.x;
+  if (st->gated) return;  // uniform across the block
+  const PairDesc pd = b.pairs[blockIdx.x];
+  const int stride = per_instance ? gate_stride(pd.ntiles) : 1;
+  const int nparts = (pd.ntiles + stride - 1) / stride;
+  double v[2] = {0.0, 0.0};
+  for (int t = threadIdx.x; t < nparts; t += kSolveThreads) {
+    const double* in = b.partials + (int64_t)(pd.tile_begin + t * stride) * kMaxVals;
+    v[0] += in[0];
+    v[1] += in[1];
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) {
+    st->fit_count = (int)v[1];
+    st->fitness = v[1] > 0.0 ? v[0] / v[1] : DBL_MAX;
   }
 }
 
@@ -477,7 +481,6 @@ __global__ void k_state_init(PairState* st, const double* guesses, const PairDes
   s.similar = 0;
   s.ncorr = 0;
   s.solve_failed = 0;
-  s.ticket = 0u;
   s.fit_count = 0;
   s.gated = 0;
   st[p] = s;
@@ -654,6 +657,8 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
                          interleave);
       break;
   }
+  if (b.npairs > 0)
+    hipLaunchKernelGGL(k_fitness_fold, dim3(b.npairs), dim3(kSolveThreads), 0, s, b, pi);
 }
 
 void launch_compact_tiles(const IcpBuffers& b, int* off, int2* it_list, int* it_count,

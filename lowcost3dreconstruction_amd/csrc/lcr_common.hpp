@@ -4,7 +4,7 @@
 //     index), float4 normals in the same order (nx, ny, nz, curvature), and a
 //     dense cell-start table over the cloud's bounding box (ncells + 1 int32);
 //   * per pair / hypothesis: a PairDesc (which clouds, tile range, radius) and a
-//     PairState (FP64 transform, convergence bookkeeping, fitness ticket);
+//     PairState (FP64 transform, convergence bookkeeping, fitness);
 //   * per source tile: FP64 partial sums written by the tile's block.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -80,7 +80,6 @@ struct PairState {
   int similar;         // iterations_similar_transforms_
   int ncorr;           // correspondences of the last iteration
   int solve_failed;    // 6x6 system not solvable in the last iteration
-  unsigned int ticket; // fitness pass: blocks of the launch that finished this instance
   int fit_count;       // points counted by the fitness pass
   int gated;           // 1: set aside by the hypothesis gate (never refined or chosen)
 };
