@@ -725,13 +725,14 @@ int lcr_voxel_grid(const float* xyz, const float* nrm, const float* curv, const 
       nvox *= L.div[a];
     }
     if (nvox > ((int64_t)1 << 30)) return fail(LCR_E_LIMIT, "voxel lattice larger than 2^30");
-    DevBuf key, slot_raw, sorted_raw, counts, start, flags, outidx, scan_tmp, dcurv, drgb, ox, on,
-        oc, orgb;
-    key.reserve((size_t)n * sizeof(int64_t));
-    slot_raw.reserve((size_t)nfin * sizeof(int));
-    sorted_raw.reserve((size_t)nfin * sizeof(int));
+    DevBuf skey, skey_sorted, idx, sorted_raw, counts, start, flags, outidx, scan_tmp, sort_tmp,
+        dcurv, drgb, ox, on, oc, orgb;
+    for (DevBuf* v : {&skey, &skey_sorted, &idx, &sorted_raw}) v->reserve((size_t)n * sizeof(int));
     for (DevBuf* v : {&counts, &start, &flags, &outidx}) v->reserve((size_t)nvox * sizeof(int));
     scan_tmp.reserve((size_t)scan_tmp_ints(nvox) * sizeof(int));
+    const size_t sort_bytes = voxel_sort_tmp_bytes(n, nvox);
+    if (sort_bytes == 0) return fail(LCR_E_HIP, "voxel sort: no temporary size");
+    sort_tmp.reserve(sort_bytes);
     if (curv) {
       dcurv.reserve((size_t)n * sizeof(float));
       LCR_HIP(hipMemcpyAsync(dcurv.p, curv, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st.s));
@@ -744,12 +745,15 @@ int lcr_voxel_grid(const float* xyz, const float* nrm, const float* curv, const 
     if (nrm && out_nrm) on.reserve((size_t)nfin * 3 * sizeof(float));
     if (curv && out_curv) oc.reserve((size_t)nfin * sizeof(float));
     if (rgb && out_rgb) orgb.reserve((size_t)nfin * 3);
-    VoxScratch w{key.as<int64_t>(), slot_raw.as<int>(), sorted_raw.as<int>(), counts.as<int>(),
-                 start.as<int>(), flags.as<int>(), outidx.as<int>(), scan_tmp.as<int>()};
-    launch_voxel_grid(cs.d_raw.as<float4>(), (nrm && cs.has_nrm) ? cs.d_raw_nrm.as<float4>() : nullptr,
-                      curv ? dcurv.as<float>() : nullptr, rgb ? drgb.as<uint8_t>() : nullptr, n, L,
-                      nvox, nfin, w, ox.as<float>(), on.p ? on.as<float>() : nullptr,
-                      oc.p ? oc.as<float>() : nullptr, orgb.p ? orgb.as<uint8_t>() : nullptr, st.s);
+    VoxScratch w{skey.as<uint32_t>(), skey_sorted.as<uint32_t>(), idx.as<int>(), sorted_raw.as<int>(),
+                 counts.as<int>(), start.as<int>(), flags.as<int>(), outidx.as<int>(),
+                 scan_tmp.as<int>(), sort_tmp.p, sort_bytes};
+    LCR_HIP(launch_voxel_grid(cs.d_raw.as<float4>(),
+                              (nrm && cs.has_nrm) ? cs.d_raw_nrm.as<float4>() : nullptr,
+                              curv ? dcurv.as<float>() : nullptr, rgb ? drgb.as<uint8_t>() : nullptr,
+                              n, L, nvox, w, ox.as<float>(), on.p ? on.as<float>() : nullptr,
+                              oc.p ? oc.as<float>() : nullptr, orgb.p ? orgb.as<uint8_t>() : nullptr,
+                              st.s));
     int last[2];
     LCR_HIP(hipMemcpyAsync(&last[0], outidx.as<int>() + nvox - 1, sizeof(int), hipMemcpyDeviceToHost,
                            st.s));

@@ -4,10 +4,13 @@
 // K11 replaces pcl::VoxelGrid<PointXYZRGBNormal>::filter with
 // setLeafSize(l, l, l) (pcl_tools/cloud_downsampling.cpp:73-75): voxel
 // coordinates floor(p * (1/l)) relative to the minimum's, one centroid per
-// occupied voxel in increasing voxel index (x fastest).  Counting, prefix sums
-// and a rank placement give every voxel its points in input order; one thread
-// per voxel then sums them in that order (FP32, as PCL's CentroidPoint), so the
-// output does not depend on the order in which atomics arrived.
+// occupied voxel in increasing voxel index (x fastest).  Counting and prefix
+// sums give every voxel its range; a stable radix sort of (voxel, input index)
+// gives every voxel its points in input order (linear in the points, whatever
+// the voxel sizes: with the CLI's default 1 m leaf a whole capture falls into a
+// few voxels); one thread per voxel then sums them in that order (FP32, as PCL's
+// CentroidPoint), so the output does not depend on the order in which atomics
+// arrived.
 //
 // K12 finishes pcl::StatisticalOutlierRemoval (pcl_tools/outlier_removal.cpp:80-84):
 // given each point's mean distance to its k nearest neighbours (normals.hip), the
@@ -20,6 +23,8 @@
 // points, normals and colours.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include "lcr_kernels.hpp"
 
@@ -35,50 +40,30 @@ __device__ __forceinline__ bool finite3(const float4& p) {
   return isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
 }
 
-__global__ void k_vox_key(const float4* __restrict__ raw, int64_t n, VoxLattice L,
-                          int64_t* __restrict__ key, int* __restrict__ counts) {
+// Per point its voxel (skey: nvox for a non-finite point, so that those sort
+// last) and its input index (the sort's payload).
+__global__ void k_vox_key(const float4* __restrict__ raw, int64_t n, VoxLattice L, int64_t nvox,
+                          uint32_t* __restrict__ skey, int* __restrict__ idx,
+                          int* __restrict__ counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  idx[i] = (int)i;
   const float4 p = raw[i];
   if (!finite3(p)) {
-    key[i] = -1;
+    skey[i] = (uint32_t)nvox;
     return;
   }
   const int ix = (int)(floorf(p.x * L.inv) - (float)L.minb[0]);
   const int iy = (int)(floorf(p.y * L.inv) - (float)L.minb[1]);
   const int iz = (int)(floorf(p.z * L.inv) - (float)L.minb[2]);
   const int64_t k = (int64_t)ix + (int64_t)iy * L.div[0] + (int64_t)iz * L.div[0] * L.div[1];
-  key[i] = k;
+  skey[i] = (uint32_t)k;
   atomicAdd(&counts[k], 1);
 }
 
 __global__ void k_vox_flags(const int* __restrict__ counts, int64_t nvox, int* __restrict__ flags) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v < nvox) flags[v] = counts[v] > 0 ? 1 : 0;
-}
-
-__global__ void k_vox_slots(const int64_t* __restrict__ key, int64_t n, const int* __restrict__ start,
-                            int* __restrict__ fill, int* __restrict__ slot_raw) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t k = key[i];
-  if (k < 0) return;
-  slot_raw[start[k] + atomicAdd(&fill[k], 1)] = (int)i;
-}
-
-// position of a slot's point inside its voxel = number of the voxel's points with
-// a smaller input index
-__global__ void k_vox_rank(const int64_t* __restrict__ key, const int* __restrict__ slot_raw,
-                           int64_t nfin, const int* __restrict__ start,
-                           const int* __restrict__ counts, int* __restrict__ sorted_raw) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nfin) return;
-  const int i = slot_raw[j];
-  const int64_t k = key[i];
-  const int b = start[k], e = b + counts[k];
-  int rank = 0;
-  for (int t = b; t < e; ++t) rank += slot_raw[t] < i ? 1 : 0;
-  sorted_raw[b + rank] = i;
 }
 
 __global__ void k_vox_centroid(const int* __restrict__ counts, const int* __restrict__ start,
@@ -256,27 +241,45 @@ __global__ void k_depth_count(const int* __restrict__ pos, const int* __restrict
 
 }  // namespace
 
-void launch_voxel_grid(const float4* raw, const float4* raw_nrm, const float* curv,
-                       const uint8_t* rgb, int64_t n, const VoxLattice& L, int64_t nvox,
-                       int64_t nfin, VoxScratch w, float* out_xyz, float* out_nrm, float* out_curv,
-                       uint8_t* out_rgb, hipStream_t s) {
-  if (n <= 0 || nvox <= 0) return;
+// Bits of the voxel keys the sort looks at: up to nvox (the non-finite marker).
+static int vox_key_bits(int64_t nvox) {
+  int b = 1;
+  while (((int64_t)1 << b) <= nvox) ++b;
+  return b;
+}
+
+size_t voxel_sort_tmp_bytes(int64_t n, int64_t nvox) {
+  size_t bytes = 0;
+  if (n <= 0) return 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                         (const int*)nullptr, (int*)nullptr, (int)n, 0,
+                                         vox_key_bits(nvox)) != hipSuccess)
+    return 0;
+  return bytes;
+}
+
+hipError_t launch_voxel_grid(const float4* raw, const float4* raw_nrm, const float* curv,
+                             const uint8_t* rgb, int64_t n, const VoxLattice& L, int64_t nvox,
+                             VoxScratch w, float* out_xyz, float* out_nrm, float* out_curv,
+                             uint8_t* out_rgb, hipStream_t s) {
+  if (n <= 0 || nvox <= 0) return hipSuccess;
   launch_fill_i32(w.counts, nvox, 0, s);
-  hipLaunchKernelGGL(k_vox_key, dim3(blocks_of(n, 256)), dim3(256), 0, s, raw, n, L, w.key,
-                     w.counts);
+  hipLaunchKernelGGL(k_vox_key, dim3(blocks_of(n, 256)), dim3(256), 0, s, raw, n, L, nvox, w.skey,
+                     w.idx, w.counts);
   launch_exclusive_scan(w.counts, w.start, nvox, w.scan_tmp, s);
   hipLaunchKernelGGL(k_vox_flags, dim3(blocks_of(nvox, 256)), dim3(256), 0, s, w.counts, nvox,
                      w.flags);
   launch_exclusive_scan(w.flags, w.outidx, nvox, w.scan_tmp, s);
-  launch_fill_i32(w.flags, nvox, 0, s);  // reused as the per-voxel fill counter
-  hipLaunchKernelGGL(k_vox_slots, dim3(blocks_of(n, 256)), dim3(256), 0, s, w.key, n, w.start,
-                     w.flags, w.slot_raw);
-  if (nfin > 0)
-    hipLaunchKernelGGL(k_vox_rank, dim3(blocks_of(nfin, 256)), dim3(256), 0, s, w.key, w.slot_raw,
-                       nfin, w.start, w.counts, w.sorted_raw);
+  // stable: within a voxel the points keep increasing input index
+  size_t bytes = w.sort_tmp_bytes;
+  const hipError_t e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, bytes, w.skey, w.skey_sorted,
+                                                          w.idx, w.sorted_raw, (int)n, 0,
+                                                          vox_key_bits(nvox), s);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_vox_centroid, dim3(blocks_of(nvox, 256)), dim3(256), 0, s, w.counts,
                      w.start, w.outidx, nvox, w.sorted_raw, raw, raw_nrm, curv, rgb, out_xyz,
                      out_nrm, out_curv, out_rgb);
+  return hipSuccess;
 }
 
 int sor_stat_blocks(int64_t n) {

@@ -135,23 +135,25 @@ struct VoxLattice {
   int minb[3];
   int64_t div[3];
 };
-// device scratch of one VoxelGrid pass: key (n int64), slot_raw / sorted_raw
-// (finite count ints), counts / start / flags / outidx (nvox ints), scan_tmp
-// (scan_tmp_ints(nvox) ints)
+// device scratch of one VoxelGrid pass (sizes per field)
 struct VoxScratch {
-  int64_t* key;
-  int* slot_raw;
-  int* sorted_raw;
-  int* counts;
-  int* start;
-  int* flags;
-  int* outidx;
-  int* scan_tmp;
+  uint32_t* skey;          // n: voxel per point (nvox: non-finite)
+  uint32_t* skey_sorted;   // n
+  int* idx;                // n: input index per point
+  int* sorted_raw;         // n: input indices in (voxel, input index) order
+  int* counts;             // nvox
+  int* start;              // nvox
+  int* flags;              // nvox
+  int* outidx;             // nvox
+  int* scan_tmp;           // scan_tmp_ints(nvox)
+  void* sort_tmp;          // voxel_sort_tmp_bytes(n, nvox)
+  size_t sort_tmp_bytes;
 };
-void launch_voxel_grid(const float4* raw, const float4* raw_nrm, const float* curv,
-                       const uint8_t* rgb, int64_t n, const VoxLattice& L, int64_t nvox,
-                       int64_t nfin, VoxScratch w, float* out_xyz, float* out_nrm, float* out_curv,
-                       uint8_t* out_rgb, hipStream_t s);
+size_t voxel_sort_tmp_bytes(int64_t n, int64_t nvox);
+hipError_t launch_voxel_grid(const float4* raw, const float4* raw_nrm, const float* curv,
+                             const uint8_t* rgb, int64_t n, const VoxLattice& L, int64_t nvox,
+                             VoxScratch w, float* out_xyz, float* out_nrm, float* out_curv,
+                             uint8_t* out_rgb, hipStream_t s);
 // StatisticalOutlierRemoval statistics and keep flags from per-point mean
 // distances (finite[i] = 1 for points that count); part holds 3 doubles per block
 // of sor_stat_blocks(n), stats 3 doubles (mean, stddev, threshold).

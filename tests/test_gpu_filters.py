@@ -36,6 +36,23 @@ def test_voxel_grid_matches_oracle(lcr, depth_cloud, leaf):
         np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize("leaf", [1.0, 0.25])
+def test_voxel_grid_large_voxels(lcr, depth_cloud, leaf):
+    """The CLI's default 1 m leaf on a whole capture (a few voxels of tens of
+    thousands of points each): bitwise equal to the oracle, in time linear in the
+    points."""
+    rng = np.random.default_rng(17)
+    pts = np.concatenate([depth_cloud, depth_cloud + rng.normal(scale=0.01, size=depth_cloud.shape)])
+    pts = pts.astype(np.float32)
+    nrm = rng.normal(size=pts.shape).astype(np.float32)
+    print_case(test="voxel_grid_large", leaf=leaf, n=len(pts))
+    got = lcr.voxel_grid(pts, leaf, nrm=nrm)
+    ref = oracle.voxel_grid(pts, leaf, nrm=nrm)
+    assert 0 < len(got["xyz"]) == len(ref["xyz"]) <= 64
+    for k in ("xyz", "nrm"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
 def test_voxel_grid_edges(lcr):
     print_case(test="voxel_grid_edges")
     assert len(lcr.voxel_grid(np.full((4, 3), np.nan, np.float32), 0.1)["xyz"]) == 0
