@@ -128,8 +128,9 @@ def test_learned_proposal_gains_over_plain_icp(lcr):
     """Held-out pairs (new shapes and motions, 20-45 deg / up to 10 cm initial
     error), point-to-plane ICP with a 1 cm radius: from the identity alone versus
     with the trained network's hypothesis added, refined beside the identity's when
-    the gate finds it has more inliers, the lower final fitness winning.  The learned hypothesis must align clearly more pairs in fewer
-    iterations, and the fallback must keep almost every pair plain ICP aligned.
+    the gate finds it has more inliers, the lower final fitness winning.  The
+    learned hypothesis must align clearly more pairs in fewer iterations, and the
+    fallback must keep almost every pair plain ICP aligned.
     (Measured on this set: 0.71 -> 0.98 aligned, 13.6 -> 7.4 iterations.)"""
     from lowcost3dreconstruction_amd import proposal_train as pt
     prop = lcr.Proposal(np.load(WEIGHTS))
