@@ -259,12 +259,66 @@ __global__ void k_outer_part(const float* __restrict__ A, int lda, const float* 
   const int i = idx / N, j = idx % N;
   const int r0 = k * rows, r1 = min(R, r0 + rows);
   float s = 0.f;
+  // unrolled so that the loads of several rows are in flight; one accumulator,
+  // the rows in order
   if (i < M) {
+#pragma unroll 8
     for (int r = r0; r < r1; ++r) s = fmaf(A[(int64_t)r * lda + i], B[(int64_t)r * ldb + j], s);
   } else {
+#pragma unroll 8
     for (int r = r0; r < r1; ++r) s += B[(int64_t)r * ldb + j];
   }
   part[(int64_t)k * (M + 1) * N + idx] = s;
+}
+
+// k_outer_part with a 4 x 4 block of outputs per thread (M, N, lda, ldb multiples
+// of 4, A and B 16-byte aligned): per row one float4 of A and one of B feed 16
+// multiply-adds, where one output per thread read two values per multiply-add and
+// was bound by those loads.  Every output is the same fmaf chain over the rows in
+// order, so the sums are bitwise those of k_outer_part.  Threads past the 4 x 4
+// blocks sum the bias row, four columns each.
+__global__ void k_outer_part4(const float* __restrict__ A, int lda, const float* __restrict__ B,
+                              int ldb, int R, int M, int N, int rows, float* __restrict__ part) {
+  const int k = blockIdx.y;
+  const int nq = N >> 2, tiles = (M >> 2) * nq;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles + nq) return;
+  const int r0 = k * rows, r1 = min(R, r0 + rows);
+  float* out = part + (int64_t)k * (M + 1) * N;
+  if (t < tiles) {
+    const int i0 = (t / nq) * 4, j0 = (t % nq) * 4;
+    float acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = 0.f;
+#pragma unroll 4
+    for (int r = r0; r < r1; ++r) {
+      const float4 a = *reinterpret_cast<const float4*>(A + (int64_t)r * lda + i0);
+      const float4 b = *reinterpret_cast<const float4*>(B + (int64_t)r * ldb + j0);
+      const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(av[u], bv[v], acc[u][v]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[(i0 + u) * N + j0 + v] = acc[u][v];
+  } else {
+    const int j0 = (t - tiles) * 4;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int r = r0; r < r1; ++r) {
+      const float4 b = *reinterpret_cast<const float4*>(B + (int64_t)r * ldb + j0);
+      acc[0] += b.x;
+      acc[1] += b.y;
+      acc[2] += b.z;
+      acc[3] += b.w;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[M * N + j0 + v] = acc[v];
+  }
 }
 
 __global__ void k_outer_fold(const float* __restrict__ part, int nparts, int M, int N,
@@ -438,7 +492,7 @@ void launch_enc_backward(const EncBuffers& e, int npairs, int npts, hipStream_t 
   const int R = nclouds * 256;
   const int rows = 1024;
   const int parts = (R + rows - 1) / rows;
-  hipLaunchKernelGGL(k_outer_part, dim3(nb(65 * 128, 256), parts), dim3(256), 0, s, e.H1A, 64,
+  hipLaunchKernelGGL(k_outer_part4, dim3(nb(16 * 32 + 32, 256), parts), dim3(256), 0, s, e.H1A, 64,
                      e.DA2, 128, R, 64, 128, rows, e.part_grad);
   hipLaunchKernelGGL(k_outer_fold, dim3(nb(65 * 128, 256)), dim3(256), 0, s, e.part_grad, parts, 64,
                      128, G + kEncW2, G + kEncB2);
