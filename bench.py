@@ -300,9 +300,13 @@ def main():
     lcr.prof_enable(False)
     prof = {k: lcr.prof_read(k) for k in ("icp_iteration", "icp_solve", "fitness", "grid_build",
                                           "normals", "proposal", "gate")}
+    # the ICP search launches that ran four lanes per query (a part of icp_iteration)
+    split_n, split_ms = lcr.prof_read("icp_iteration_split")
     log(f"batched step {step_s * 1e3:.3f} ms (profiled replay {prof_step_s * 1e3:.3f} ms); "
         + ", ".join(f"{k} {v[1] / args.steps:.3f} ms ({v[0] // args.steps} launches)"
-                    for k, v in prof.items()))
+                    for k, v in prof.items())
+        + f"; of the icp_iteration launches, {split_n // args.steps} split "
+          f"({split_ms / args.steps:.3f} ms)")
 
     # ---------------- the same cycle without the learned hypothesis, and every
     # precision mode (their transforms must agree bitwise with the measured run)
@@ -542,6 +546,7 @@ def main():
         "precision_modes": modes,
         "kernel_ms_per_step": {**{k: v[1] / steps for k, v in prof.items()},
                                "profiled_step": prof_step_s * 1e3},
+        "icp_split_launches_per_step": {"launches": split_n / steps, "ms": split_ms / steps},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -97,9 +97,14 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles, int interleave) {
 // the d2 sum.  Each value is formed and summed over the wavefront in turn (fixed
 // order), so only the per-lane inputs stay live; the waves' sums are added in
 // wave order.
-template <bool P2PL, int TILE>
-__device__ __forceinline__ void tile_moments(bool has, float qx, float qy, float qz, const float4& t,
-                                             const float4& n, float d2, double* lds, double* out) {
+//
+// tile_moments_emit writes each wavefront's sums to lds[wid * NV + v] (every lane
+// of the wavefront calls it); tile_moments_store adds the NW wavefronts' sums in
+// wave order into out.
+template <bool P2PL>
+__device__ __forceinline__ void tile_moments_emit(bool has, float qx, float qy, float qz,
+                                                  const float4& t, const float4& n, float d2,
+                                                  double* lds) {
   constexpr int NV = P2PL ? kP2LVals : kP2PVals;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int k = 0;
@@ -140,13 +145,25 @@ __device__ __forceinline__ void tile_moments(bool has, float qx, float qy, float
     for (int i = 0; i < 6; ++i) emit(row[i] * d);
   }
   emit(has ? (double)d2 : 0.0);
-  __syncthreads();
+}
+
+template <int NV, int NW>
+__device__ __forceinline__ void tile_moments_store(const double* lds, double* out) {
   if (threadIdx.x < NV) {
     double sum = lds[threadIdx.x];
 #pragma unroll
-    for (int w = 1; w < TILE / 64; ++w) sum += lds[w * NV + threadIdx.x];
+    for (int w = 1; w < NW; ++w) sum += lds[w * NV + threadIdx.x];
     out[threadIdx.x] = sum;
   }
+}
+
+template <bool P2PL, int TILE>
+__device__ __forceinline__ void tile_moments(bool has, float qx, float qy, float qz, const float4& t,
+                                             const float4& n, float d2, double* lds, double* out) {
+  constexpr int NV = P2PL ? kP2LVals : kP2PVals;
+  tile_moments_emit<P2PL>(has, qx, qy, qz, t, n, d2, lds);
+  __syncthreads();
+  tile_moments_store<NV, TILE / 64>(lds, out);
   __syncthreads();  // lds is reused by the next tile
 }
 
@@ -236,6 +253,125 @@ __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const 
       }
     }
     tile_moments<P2PL, TILE>(has, qx, qy, qz, t, n, nn.d2, lds, b.partials + (int64_t)tile * kMaxVals);
+  }
+}
+
+// The ring-only search for launches with few live tiles.  Each query gets
+// kSplitLanes lanes; each lane scans every kSplitLanes-th (z, y) row of a ring
+// (ring_rows), and the lanes take the minimum by (d2, id) after every ring, so the
+// result is the one-lane search's.  A block of kSplitLanes wavefronts takes one
+// tile, 64 queries per pass (64 / kSplitLanes per wavefront); the TILE results go
+// through LDS to the first TILE threads, which form the tile's sums exactly as
+// k_icp_iter<TILE> does (query i in lane i % 64 of wavefront i / 64).  With a few
+// hundred tiles live the one-lane kernel leaves most SIMDs idle, and each
+// wavefront waits on its longest chain of row loads; this one runs kSplitLanes
+// times as many wavefronts, each with about 1 / kSplitLanes of the rows.
+constexpr int kSplitLanes = 4;
+
+template <int S>
+__device__ __forceinline__ void lanes_min(float& best, int& bid, int& bj) {
+#pragma unroll
+  for (int m = 1; m < S; m <<= 1) {
+    const float ob = __shfl_xor(best, m, 64);
+    const int oid = __shfl_xor(bid, m, 64);
+    const int oj = __shfl_xor(bj, m, 64);
+    const bool take = ob < best || (ob == best && oid < bid);
+    best = take ? ob : best;
+    bid = take ? oid : bid;
+    bj = take ? oj : bj;
+  }
+}
+
+// Bounded ring-only 1-NN over S lanes (sub = this lane's index among them).  The S
+// lanes of a query are consecutive and reach every lanes_min together: their
+// early returns and ring loop depend only on the query and the shared best.
+template <int S>
+__device__ __forceinline__ NNResult nn_ring_split(const GridDesc& g, const int* __restrict__ cs,
+                                                  const float4* __restrict__ P, float qx, float qy,
+                                                  float qz, float r, int seed_j, int sub) {
+  NNResult res{INFINITY, 0x7fffffff, -1, false};
+  if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
+  const CellPos c = cell_pos(g, qx, qy, qz);
+  const float rlim = r * 1.00001f;
+  const int k0 = ring_start(g, c, rlim);
+  if (k0 < 0) return res;
+  const int kmax = k0 + max(g.nx, max(g.ny, g.nz));
+  float best = INFINITY;
+  int bid = 0x7fffffff, bj = -1;
+  if (seed_j >= 0) scan_points(P, seed_j, seed_j + 1, qx, qy, qz, best, bid, bj);
+  const float h2 = g.h * g.h, lim2 = rlim * rlim;
+  for (int k = k0; k <= kmax; ++k) {
+    ring_rows(g, cs, P, c, k, sub, S, h2, lim2, qx, qy, qz, best, bid, bj);
+    lanes_min<S>(best, bid, bj);
+    if (ring_stop(g, c, k, best, rlim)) break;
+  }
+  res.d2 = best;
+  res.id = bid;
+  res.j = bj;
+  return res;
+}
+
+template <bool P2PL, int TILE>
+__global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBuffers b,
+                                                                     const IcpConst c) {
+  constexpr int S = kSplitLanes, NV = P2PL ? kP2LVals : kP2PVals;
+  static_assert(64 % S == 0 && (S & (S - 1)) == 0, "lanes per query: a power of two dividing 64");
+  static_assert(TILE % 64 == 0 && TILE <= 64 * S, "one thread per query forms the sums");
+  __shared__ double lds[NV * (TILE / 64)];
+  __shared__ float s_d2[TILE];
+  __shared__ int s_j[TILE];
+  const int nt = b.it_count ? *b.it_count : b.ntiles;
+  const int q = (int)threadIdx.x / S, sub = (int)threadIdx.x % S;
+  for (int it = blockIdx.x; it < nt; it += gridDim.x) {
+    const int2 e = b.it_list ? b.it_list[it] : make_int2(it, b.tile_pair[it]);
+    const int tile = e.x, p = e.y;
+    const PairState* st = b.state + p;
+    if (st->active == 0) continue;  // uniform across the block
+    const PairDesc pd = b.pairs[p];
+    const GridDesc gs = b.grids[pd.src];
+    const GridDesc gt = b.grids[pd.tgt];
+    const int base = (tile - pd.tile_begin) * TILE;
+    // every radius is finite here (IcpConst::ring_only)
+    const float r = (float)sqrt(pd.max_corr_d2);
+    for (int pass = 0; pass < TILE / 64; ++pass) {
+      const int qi = pass * 64 + q;
+      const bool own = base + qi < gs.npts;
+      float qx = 0.f, qy = 0.f, qz = 0.f;
+      NNResult nn{INFINITY, 0x7fffffff, -1, false};
+      if (own) xform(st->T, b.pts[gs.pt_off + base + qi], qx, qy, qz);
+      const bool empty = own && b.near && gt.near_r >= r && !near_cell(gt, b.near, qx, qy, qz);
+      if (own && !empty) {
+        const int seed = b.prev ? b.prev[(int64_t)tile * TILE + qi] : -1;
+        nn = nn_ring_split<S>(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r, seed, sub);
+      }
+      if (sub == 0) {
+        s_d2[qi] = nn.d2;
+        s_j[qi] = nn.j;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < TILE) {
+      const int local = base + (int)threadIdx.x;
+      const bool own = local < gs.npts;
+      float qx = 0.f, qy = 0.f, qz = 0.f;
+      if (own) xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+      const float d2 = s_d2[threadIdx.x];
+      const int j = s_j[threadIdx.x];
+      bool has = false;
+      float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
+      if (own) {
+        if (b.prev) b.prev[(int64_t)tile * TILE + threadIdx.x] = j;
+        if (j >= 0 && (double)d2 <= pd.max_corr_d2) {
+          has = true;
+          t = b.pts[j];
+          if (P2PL) n = b.nrm[j];
+        }
+      }
+      tile_moments_emit<P2PL>(has, qx, qy, qz, t, n, d2, lds);
+    }
+    __syncthreads();
+    tile_moments_store<NV, TILE / 64>(lds, b.partials + (int64_t)tile * kMaxVals);
+    __syncthreads();  // lds, s_d2 and s_j are reused by the next tile
   }
 }
 
@@ -630,6 +766,46 @@ void launch_icp_search(const IcpBuffers& b, const IcpConst& c, int nblocks, hipS
     launch_iter_tiles<3>(b, c, nblocks, s);
   else
     launch_iter_scalar(b, c, nblocks, s);
+}
+
+template <bool P2PL, int TILE>
+void launch_split_t(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((k_icp_iter_split<P2PL, TILE>), dim3(nblocks), dim3(64 * kSplitLanes), 0, s, b,
+                     c);
+}
+
+void launch_icp_search_split(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s) {
+  const bool pl = c.point_to_plane != 0;
+  switch (b.tile) {
+    case 64:
+      pl ? launch_split_t<true, 64>(b, c, nblocks, s) : launch_split_t<false, 64>(b, c, nblocks, s);
+      break;
+    case 128:
+      pl ? launch_split_t<true, 128>(b, c, nblocks, s) : launch_split_t<false, 128>(b, c, nblocks, s);
+      break;
+    default:
+      pl ? launch_split_t<true, 256>(b, c, nblocks, s) : launch_split_t<false, 256>(b, c, nblocks, s);
+      break;
+  }
+}
+
+template <bool P2PL, int TILE>
+int split_blocks_per_cu_t() {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, k_icp_iter_split<P2PL, TILE>, 64 * kSplitLanes, 0);
+  return (e != hipSuccess || nb < 1) ? 1 : nb;
+}
+
+int icp_split_blocks_per_cu(int tile, bool p2pl) {
+  switch (tile) {
+    case 64:
+      return p2pl ? split_blocks_per_cu_t<true, 64>() : split_blocks_per_cu_t<false, 64>();
+    case 128:
+      return p2pl ? split_blocks_per_cu_t<true, 128>() : split_blocks_per_cu_t<false, 128>();
+    default:
+      return p2pl ? split_blocks_per_cu_t<true, 256>() : split_blocks_per_cu_t<false, 256>();
+  }
 }
 
 void launch_icp_solve(const IcpBuffers& b, const IcpConst& c, hipStream_t s) {

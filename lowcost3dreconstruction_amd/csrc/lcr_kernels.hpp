@@ -84,9 +84,11 @@ struct IcpBuffers {
 // One ICP iteration for every active instance: the correspondence search with
 // per-tile moment partials, then the per-instance fold, solve and update.
 void launch_icp_search(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s);
+// The same search with four lanes per query, for ring-only instances
+// (c.ring_only): the launches with few live tiles.
+void launch_icp_search_split(const IcpBuffers& b, const IcpConst& c, int nblocks, hipStream_t s);
+int icp_split_blocks_per_cu(int tile, bool p2pl);
 void launch_icp_solve(const IcpBuffers& b, const IcpConst& c, hipStream_t s);
-// getFitnessScore(max_range) for every instance with a nonzero iteration count or
-// all instances when `all` is set.
 // getFitnessScore(max_range) of every instance that is not gated; with
 // per_instance, each instance's own max_corr_d2 replaces max_range.
 void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,

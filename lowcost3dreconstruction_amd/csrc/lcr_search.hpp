@@ -228,6 +228,98 @@ LCR_SHD void bvh_nn(const GridDesc& g, const Tree& tr, float qx, float qy, float
   }
 }
 
+// Row (z, y) of ring k around the query's cell c: the whole x range of the ring on
+// its z and y faces, else the ring's two x ends.  The row is skipped when its
+// lower-bound distance exceeds the bound min(best, lim2) (with a relative margin,
+// so equal distances are still visited), and its x range is trimmed to the cells
+// whose x gap keeps the bound under it.  gz, zface: z's gap and whether z is on a
+// z face of the ring.
+LCR_SHD void scan_ring_row(const GridDesc& g, const int* __restrict__ cs,
+                           const float4* __restrict__ P, const CellPos& c, int k, int z, int y,
+                           float gz, bool zface, int x0, int x1, float h2, float lim2, float qx,
+                           float qy, float qz, float& best, int& bid, int& bj) {
+  const float gy = axis_gap(c.fy, y, c.cy, c.slack);
+  // bound on d2 for any point of this row (cell units^2 -> m^2)
+  const float thr = fminf(best, lim2) * 1.00002f;
+  const float lyz = (gz * gz + gy * gy) * h2;
+  if (lyz > thr) return;
+  int xa, xb;
+  bool two = false;
+  if (zface || y == c.cy - k || y == c.cy + k) {
+    xa = x0;
+    xb = x1;
+  } else {
+    xa = c.cx - k;
+    xb = c.cx + k;
+    two = true;
+  }
+  int xlo = xa, xhi = xb;
+  if (thr < INFINITY) {
+    const float rem = fmaxf(thr - lyz, 0.0f);
+    const float dxc = sqrtf(rem) * g.inv_h * 1.00001f + c.slack;
+    xlo = max(xa, (int)ceilf(c.fx - dxc - 1.0f));
+    xhi = min(xb, (int)floorf(c.fx + dxc));
+  }
+  const int row = (z * g.ny + y) * g.nx;
+  if (!two) {
+    if (xlo <= xhi) scan_points(P, cs[row + xlo], cs[row + xhi + 1], qx, qy, qz, best, bid, bj);
+  } else {
+    if (xa >= 0 && xa >= xlo && xa <= xhi)
+      scan_points(P, cs[row + xa], cs[row + xa + 1], qx, qy, qz, best, bid, bj);
+    if (k > 0 && xb <= g.nx - 1 && xb >= xlo && xb <= xhi)
+      scan_points(P, cs[row + xb], cs[row + xb + 1], qx, qy, qz, best, bid, bj);
+  }
+}
+
+// The rows of ring k dealt to `nsub` lanes: this one (`sub`) scans rows sub,
+// sub + nsub, ... of the ring's (z, y) rows in row-major order, each pruned by
+// its own best.  Taking the minimum (by d2, then id) over the lanes afterwards
+// gives what one lane scanning every row finds: a lane's best is never below the
+// shared one, so it prunes no more than a single lane would.
+LCR_SHD void ring_rows(const GridDesc& g, const int* __restrict__ cs, const float4* __restrict__ P,
+                       const CellPos& c, int k, int sub, int nsub, float h2, float lim2, float qx,
+                       float qy, float qz, float& best, int& bid, int& bj) {
+  const int z0 = max(c.cz - k, 0), z1 = min(c.cz + k, g.nz - 1);
+  const int y0 = max(c.cy - k, 0), y1 = min(c.cy + k, g.ny - 1);
+  const int x0 = max(c.cx - k, 0), x1 = min(c.cx + k, g.nx - 1);
+  if (!(z0 <= z1 && y0 <= y1 && x0 <= x1)) return;
+  const int ny = y1 - y0 + 1;
+  int z = z0, y = y0 + sub;
+  while (y > y1) {
+    y -= ny;
+    ++z;
+  }
+  while (z <= z1) {
+    const float gz = axis_gap(c.fz, z, c.cz, c.slack);
+    const bool zface = (z == c.cz - k) || (z == c.cz + k);
+    scan_ring_row(g, cs, P, c, k, z, y, gz, zface, x0, x1, h2, lim2, qx, qy, qz, best, bid, bj);
+    y += nsub;
+    while (y > y1) {
+      y -= ny;
+      ++z;
+    }
+  }
+}
+
+// Whether a bounded search (limit rlim) may stop after ring k with `best`: the
+// grid is exhausted, no unvisited cell can hold a point closer than best, or
+// none is within rlim.
+LCR_SHD bool ring_stop(const GridDesc& g, const CellPos& c, int k, float best, float rlim) {
+  const float Lc = block_gap(g, c, k);
+  if (Lc == INFINITY) return true;
+  const float L = (Lc - c.slack) * g.h;
+  if (L > 0.0f && best < L * L * 0.99998f) return true;
+  return L > rlim;
+}
+
+// Where a bounded ring-only search starts: k0 = the first ring that can hold a
+// point, -1 if no cell within rlim of q can (the result is then "none").
+LCR_SHD int ring_start(const GridDesc& g, const CellPos& c, float rlim) {
+  const int k0 = first_ring(g, c);
+  if (k0 > 0 && (block_gap(g, c, k0 - 1) - c.slack) * g.h > rlim) return -1;
+  return k0;
+}
+
 // Exact 1-NN of q among the grid's points.  With `bounded`, points farther than
 // `r` may be skipped (the result is then either the true nearest or a point the
 // caller rejects as d2 > r^2).  `seed_j` (or -1) is a known candidate, e.g. the
@@ -286,40 +378,8 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
       for (int z = z0; z <= z1; ++z) {
         const float gz = axis_gap(c.fz, z, c.cz, c.slack);
         const bool zface = (z == c.cz - k) || (z == c.cz + k);
-        for (int y = y0; y <= y1; ++y) {
-          const float gy = axis_gap(c.fy, y, c.cy, c.slack);
-          // bound on d2 for any point of this row (cell units^2 -> m^2)
-          const float thr = fminf(best, lim2) * 1.00002f;
-          const float lyz = (gz * gz + gy * gy) * h2;
-          if (lyz > thr) continue;
-          int xa, xb;
-          bool two = false;
-          if (zface || y == c.cy - k || y == c.cy + k) {
-            xa = x0;
-            xb = x1;
-          } else {
-            xa = c.cx - k;
-            xb = c.cx + k;
-            two = true;
-          }
-          // trim the x range to cells whose x gap keeps the bound under thr
-          int xlo = xa, xhi = xb;
-          if (thr < INFINITY) {
-            const float rem = fmaxf(thr - lyz, 0.0f);
-            const float dxc = sqrtf(rem) * g.inv_h * 1.00001f + c.slack;
-            xlo = max(xa, (int)ceilf(c.fx - dxc - 1.0f));
-            xhi = min(xb, (int)floorf(c.fx + dxc));
-          }
-          const int row = (z * g.ny + y) * g.nx;
-          if (!two) {
-            if (xlo <= xhi) scan_points(P, cs[row + xlo], cs[row + xhi + 1], qx, qy, qz, best, bid, bj);
-          } else {
-            if (xa >= 0 && xa >= xlo && xa <= xhi)
-              scan_points(P, cs[row + xa], cs[row + xa + 1], qx, qy, qz, best, bid, bj);
-            if (k > 0 && xb <= g.nx - 1 && xb >= xlo && xb <= xhi)
-              scan_points(P, cs[row + xb], cs[row + xb + 1], qx, qy, qz, best, bid, bj);
-          }
-        }
+        for (int y = y0; y <= y1; ++y)
+          scan_ring_row(g, cs, P, c, k, z, y, gz, zface, x0, x1, h2, lim2, qx, qy, qz, best, bid, bj);
       }
     }
     const float Lc = block_gap(g, c, k);

@@ -66,6 +66,46 @@ int lcrm_conv_step(int* io, double* dio, int max_iterations, const double* thr,
   return done ? 1 : 0;
 }
 
+// The split ring search with its lanes run in turn: each lane scans its share of
+// a ring's rows (ring_rows) from the shared best, then every lane takes the
+// minimum by (d2, id).
+static NNResult ring_split_host(const GridDesc& g, const int* cs, const float4* P, float qx, float qy,
+                                float qz, float r, int seed, int nsub) {
+  NNResult res{INFINITY, 0x7fffffff, -1, false};
+  if (g.npts == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz))) return res;
+  const CellPos c = cell_pos(g, qx, qy, qz);
+  const float rlim = r * 1.00001f;
+  const int k0 = ring_start(g, c, rlim);
+  if (k0 < 0) return res;
+  const int kmax = k0 + std::max(g.nx, std::max(g.ny, g.nz));
+  float best = INFINITY;
+  int bid = 0x7fffffff, bj = -1;
+  if (seed >= 0) scan_points(P, seed, seed + 1, qx, qy, qz, best, bid, bj);
+  const float h2 = g.h * g.h, lim2 = rlim * rlim;
+  for (int k = k0; k <= kmax; ++k) {
+    float lb[8];
+    int lid[8], lj[8];
+    for (int l = 0; l < nsub; ++l) {
+      lb[l] = best;
+      lid[l] = bid;
+      lj[l] = bj;
+      ring_rows(g, cs, P, c, k, l, nsub, h2, lim2, qx, qy, qz, lb[l], lid[l], lj[l]);
+    }
+    for (int l = 0; l < nsub; ++l) {
+      if (lb[l] < best || (lb[l] == best && lid[l] < bid)) {
+        best = lb[l];
+        bid = lid[l];
+        bj = lj[l];
+      }
+    }
+    if (ring_stop(g, c, k, best, rlim)) break;
+  }
+  res.d2 = best;
+  res.id = bid;
+  res.j = bj;
+  return res;
+}
+
 // Host build of one cloud's grid in the device layout (cell edge h, origin at the
 // bounding-box minimum, points sorted by cell then by index), then nn_query for
 // every query: exercises the ring search and its certification bound on the CPU.
@@ -74,8 +114,13 @@ int lcrm_conv_step(int* io, double* dio, int max_iterations, const double* thr,
 // seed candidate (-1 = none).
 // ring_limit < 0: no tree (ring search only); otherwise the tree takes over after
 // ring_limit rings, as on the device.
+// nsub > 0: the bounded ring-only search with each ring's rows dealt to nsub lanes
+// (run here one after another) and the minimum over the lanes taken after every
+// ring, as in the ICP search for launches with few live tiles.
 int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float h, float r,
-                 int bounded, const int32_t* seed_ids, int ring_limit, int32_t* idx, float* d2) {
+                 int bounded, const int32_t* seed_ids, int ring_limit, int32_t* idx, float* d2,
+                 int nsub) {
+  if (nsub < 0 || nsub > 8 || (nsub > 0 && !bounded)) return -2;
   GridDesc g{};
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   int npts = 0;
@@ -162,8 +207,11 @@ int lcrm_nn_grid(const float* tgt, int64_t nt, const float* q, int64_t nq, float
   for (int64_t i = 0; i < nq; ++i) {
     const int seed = (seed_ids && seed_ids[i] >= 0) ? pos[seed_ids[i]] : -1;
     const Tree tr{ring_limit >= 0 ? nodes.data() : nullptr, TP.data(), tpos.data()};
-    const NNResult res = nn_query(g, cs.data(), P.data(), tr, q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
-                                  bounded != 0, seed, ring_limit >= 0 ? ring_limit : 0);
+    const NNResult res =
+        nsub > 0 ? ring_split_host(g, cs.data(), P.data(), q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
+                                   seed, nsub)
+                 : nn_query(g, cs.data(), P.data(), tr, q[3 * i], q[3 * i + 1], q[3 * i + 2], r,
+                            bounded != 0, seed, ring_limit >= 0 ? ring_limit : 0);
     const bool keep = res.j >= 0 && (!bounded || (double)res.d2 <= (double)r * (double)r);
     idx[i] = keep ? res.id : -1;
     d2[i] = res.d2;

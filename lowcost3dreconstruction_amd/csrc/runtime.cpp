@@ -463,10 +463,22 @@ void Engine::reset_graph() {
   if (graph) (void)hipGraphDestroy(graph);
   if (tail_exec) (void)hipGraphExecDestroy(tail_exec);
   if (tail_graph) (void)hipGraphDestroy(tail_graph);
+  if (split_exec) (void)hipGraphExecDestroy(split_exec);
+  if (split_graph) (void)hipGraphDestroy(split_graph);
   exec = nullptr;
   graph = nullptr;
   tail_exec = nullptr;
   tail_graph = nullptr;
+  split_exec = nullptr;
+  split_graph = nullptr;
+}
+
+// Live 64-point tiles at or below which a chunk runs the four-lanes-per-query
+// search; LCR_SPLIT_TILES (read at every setup) overrides it, 0 turns the split off.
+static int icp_split_units() {
+  const char* e = getenv("LCR_SPLIT_TILES");
+  if (e && *e) return std::max(0, atoi(e));
+  return 2048;
 }
 
 void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::vector<int>& tgt,
@@ -529,6 +541,12 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   }
   int nf = std::min(std::max(ntiles, 1), cus * fitness_max_blocks_per_cu(tile));
   nblocks_fit = std::max(8, (nf + 7) / 8 * 8);
+  split_units = icp_split_units();
+  if (split_units > 0) {
+    int ns = std::min(std::max(std::min(ntiles, split_units / (tile / 64)), 1),
+                      cus * icp_split_blocks_per_cu(tile, c.point_to_plane != 0));
+    nblocks_split = std::max(8, (ns + 7) / 8 * 8);
+  }
 }
 
 IcpBuffers Engine::buffers(const CloudSet& cs) const {
@@ -560,7 +578,9 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
 }
 
 namespace {
-__global__ void k_count_active(const PairState* __restrict__ st, int n, int* __restrict__ out) {
+// out[0]: instances still active; out[1]: live tiles (the last compaction's count).
+__global__ void k_count_active(const PairState* __restrict__ st, int n,
+                               const int* __restrict__ live_tiles, int* __restrict__ out) {
   __shared__ int s[256];
   int a = 0;
   for (int i = threadIdx.x; i < n; i += 256) a += st[i].active ? 1 : 0;
@@ -570,7 +590,10 @@ __global__ void k_count_active(const PairState* __restrict__ st, int n, int* __r
     if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = s[0];
+  if (threadIdx.x == 0) {
+    out[0] = s[0];
+    out[1] = *live_tiles;
+  }
 }
 }  // namespace
 
@@ -614,8 +637,9 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   launch_fill_i32(d_prev.as<int>(), (int64_t)ntiles * tile, -1, s);  // no previous matches
   launch_compact_tiles(b, d_list_off.as<int>(), d_it_list.as<int2>(), d_list_count.as<int>(),
                        d_fit_list.as<int2>(), d_list_count.as<int>() + 1, true, s);
-  d_active.reserve(sizeof(int));
-  h_active.reserve(sizeof(int));
+  d_active.reserve(2 * sizeof(int));
+  h_active.reserve(2 * sizeof(int));
+  const bool split_ok = use_split();
   const bool eager = !use_graph || prof.on();
   std::vector<Profiler::Span> spans;
   if (gate_h > 1 && ntiles > 0) {
@@ -629,10 +653,16 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
                          d_fit_list.as<int2>(), d_list_count.as<int>() + 1, true, s);
     prof.end(k, s, &spans);
   }
-  auto chunk = [&](std::vector<Profiler::Span>* sp) {
+  auto chunk = [&](std::vector<Profiler::Span>* sp, bool split) {
     for (int it = 0; it < kChunk; ++it) {
+      // the split launches are also counted on their own
       const int k = prof.begin("icp_iteration", s, sp);
-      launch_icp_search(b, c, nb_search, s);
+      const int ks = split ? prof.begin("icp_iteration_split", s, sp) : -1;
+      if (split)
+        launch_icp_search_split(b, c, nblocks_split, s);
+      else
+        launch_icp_search(b, c, nb_search, s);
+      prof.end(ks, s, sp);
       prof.end(k, s, sp);
       const int k2 = prof.begin("icp_solve", s, sp);
       launch_icp_solve(b, c, s);
@@ -641,8 +671,8 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
     launch_compact_tiles(b, d_list_off.as<int>(), d_it_list.as<int2>(), d_list_count.as<int>(),
                          nullptr, nullptr, false, s);
     hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, d_state.as<PairState>(), npairs,
-                       d_active.as<int>());
-    LCR_HIP(hipMemcpyAsync(h_active.p, d_active.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                       d_list_count.as<int>(), d_active.as<int>());
+    LCR_HIP(hipMemcpyAsync(h_active.p, d_active.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   };
   auto tail = [&](std::vector<Profiler::Span>* sp) {
     const int k = prof.begin("fitness", s, sp);
@@ -655,21 +685,26 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   };
   if (!eager) {
     const GraphKey key = graph_key(b);
-    if (!exec || !tail_exec || !(key == graph_key_)) {
+    if (!exec || !tail_exec || (split_ok && !split_exec) || !(key == graph_key_)) {
       reset_graph();
-      capture(s, [&] { chunk(nullptr); }, &graph, &exec);
+      capture(s, [&] { chunk(nullptr, false); }, &graph, &exec);
+      if (split_ok) capture(s, [&] { chunk(nullptr, true); }, &split_graph, &split_exec);
       capture(s, [&] { tail(nullptr); }, &tail_graph, &tail_exec);
       graph_key_ = key;
     }
   }
+  // the first chunk splits when all of the tiles fit under split_units, later ones
+  // when the previous chunk left that few live
+  bool split = split_fits(ntiles);
   const int max_chunks = ntiles > 0 ? (c.max_iterations + kChunk - 1) / kChunk : 0;
   for (int k = 0; k < max_chunks; ++k) {
     if (eager)
-      chunk(&spans);
+      chunk(&spans, split);
     else
-      LCR_HIP(hipGraphLaunch(exec, s));
+      LCR_HIP(hipGraphLaunch(split ? split_exec : exec, s));
     LCR_HIP(hipStreamSynchronize(s));
-    if (*h_active.as<int>() == 0) break;
+    if (h_active.as<int>()[0] == 0) break;
+    split = split_fits(h_active.as<int>()[1]);
   }
   if (ntiles > 0) {
     if (eager)
@@ -707,6 +742,7 @@ Engine::GraphKey Engine::graph_key(const IcpBuffers& b) const {
   k.c = c;
   k.nblocks = c.ring_only ? nblocks_ring : nblocks;
   k.nblocks_fit = nblocks_fit;
+  k.nblocks_split = use_split() ? nblocks_split : 0;
   k.guess = d_guess.p;
   k.hstate = h_state.p;
   k.active = d_active.p;

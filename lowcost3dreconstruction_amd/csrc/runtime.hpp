@@ -171,6 +171,14 @@ struct CloudSet {
 struct Engine {
   int npairs = 0, ntiles = 0, tile = 256, nblocks = 8;
   int nblocks_ring = 8;  // persistent blocks of the ring-only search variant
+  // The four-lanes-per-query search (launch_icp_search_split) runs the chunks of
+  // ring-only instances that start with at most split_units live tiles, counted
+  // in 64-point units, on nblocks_split blocks.
+  int split_units = 0, nblocks_split = 8;
+  bool use_split() const { return c.ring_only && split_units > 0; }
+  bool split_fits(int64_t live_tiles) const {
+    return use_split() && live_tiles * (tile / 64) <= split_units;
+  }
   // IcpConst::ring_only for the instances over these grids: the FP32 scan, and
   // every radius finite and spanning at most kRingOnlyCells target cells
   bool ring_only(const CloudSet& cs) const;
@@ -184,12 +192,12 @@ struct Engine {
   static constexpr int kChunk = 4;  // iterations per captured chunk
   DevBuf d_active;
   HostBuf h_active;
-  hipGraph_t graph = nullptr, tail_graph = nullptr;
-  hipGraphExec_t exec = nullptr, tail_exec = nullptr;
+  hipGraph_t graph = nullptr, tail_graph = nullptr, split_graph = nullptr;
+  hipGraphExec_t exec = nullptr, tail_exec = nullptr, split_exec = nullptr;
   struct GraphKey {
     IcpBuffers b{};
     IcpConst c{};
-    int nblocks = 0, nblocks_fit = 0;
+    int nblocks = 0, nblocks_fit = 0, nblocks_split = 0;
     const void* guess = nullptr;
     const void* hstate = nullptr;
     const void* active = nullptr;

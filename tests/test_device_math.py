@@ -155,10 +155,10 @@ def test_mat4_mul(m):
     np.testing.assert_allclose(Cm.reshape(4, 4), A @ B, atol=1e-14)
 
 
-def _nn_grid(m, tgt, q, h, r, bounded, seeds=None, ring_limit=2):
+def _nn_grid(m, tgt, q, h, r, bounded, seeds=None, ring_limit=2, nsub=0):
     m.lcrm_nn_grid.argtypes = [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), C.c_int64,
                                C.c_float, C.c_float, C.c_int, C.POINTER(C.c_int32), C.c_int,
-                               C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+                               C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int]
     tgt = np.ascontiguousarray(tgt, np.float32)
     q = np.ascontiguousarray(q, np.float32)
     idx = np.empty(len(q), np.int32)
@@ -167,7 +167,7 @@ def _nn_grid(m, tgt, q, h, r, bounded, seeds=None, ring_limit=2):
     sp = None if seeds is None else np.ascontiguousarray(seeds, np.int32).ctypes.data_as(
         C.POINTER(C.c_int32))
     rc = m.lcrm_nn_grid(fp(tgt), len(tgt), fp(q), len(q), h, r, int(bounded), sp, ring_limit,
-                        idx.ctypes.data_as(C.POINTER(C.c_int32)), fp(d2))
+                        idx.ctypes.data_as(C.POINTER(C.c_int32)), fp(d2), nsub)
     assert rc == 0
     return idx, d2
 
@@ -200,6 +200,13 @@ def test_grid_ring_search_equals_bruteforce(m, h, r):
             idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded, seeds, ring_limit)
             np.testing.assert_array_equal(idx, ridx)
             np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
+        # each ring's rows dealt to 1-4 lanes, the minimum taken after every ring
+        # (bounded radii of at most a few cells: the few-tile ICP search)
+        if bounded and r <= 3 * h:
+            for nsub in (1, 2, 3, 4):
+                idx, d2 = _nn_grid(m, tgt, q, h, r, True, seeds, -1, nsub)
+                np.testing.assert_array_equal(idx, ridx)
+                np.testing.assert_array_equal(d2[idx >= 0], rd2[ridx >= 0])
 
 
 def test_tune_cache_key_and_lookup(m, tmp_path):

@@ -257,6 +257,45 @@ def test_distance_tiles_fallback_and_small_radius(lcr, prec):
         assert out[0][1:] == out[1][1:]
 
 
+@pytest.mark.parametrize("p2pl,tile", [(False, 64), (True, 64), (True, 128), (False, 256)])
+def test_split_search_matches_one_lane(lcr, monkeypatch, p2pl, tile):
+    """The four-lanes-per-query search (launches with few live tiles) finds the
+    same correspondences as the one-lane search and forms the same tile sums, so a
+    cycle run is bitwise identical with the split off, at the default threshold and
+    in every launch.  The profiler's count shows that the split kernel ran."""
+    cyc = synth.turntable_cycle(5, n_captures=5, yaw_step_deg=10.0, scale=1.5,
+                                centre=(0.0, 0.0, -0.8))
+    res = []
+    for split in ("0", None, "100000000"):
+        print_case(test="split_search", p2pl=p2pl, tile=tile, split=split)
+        if split is None:
+            monkeypatch.delenv("LCR_SPLIT_TILES", raising=False)
+        else:
+            monkeypatch.setenv("LCR_SPLIT_TILES", split)
+        c = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=cyc.meta["prior"], point_to_plane=p2pl,
+                      max_corr_dist=0.01, max_iterations=60, tile=tile)
+        c.run()
+        res.append(c.results())
+        lcr.prof_reset()
+        lcr.prof_enable(True)
+        try:
+            c.run()
+        finally:
+            lcr.prof_enable(False)
+        n_split, _ = lcr.prof_read("icp_iteration_split")
+        n_all, _ = lcr.prof_read("icp_iteration")
+        if split == "0":
+            assert n_split == 0
+        elif split is not None:
+            assert n_split == n_all > 0
+        np.testing.assert_array_equal(c.results()["T"], res[-1]["T"])
+        del c
+    for r in res[1:]:
+        np.testing.assert_array_equal(r["T"], res[0]["T"])
+        np.testing.assert_array_equal(r["iterations"], res[0]["iterations"])
+        np.testing.assert_array_equal(r["fitness"], res[0]["fitness"])
+
+
 def test_distance_tiles_cycle_matches_exact(lcr):
     cyc = synth.turntable_cycle(4, n_captures=4, yaw_step_deg=10.0, scale=1.5,
                                 centre=(0.0, 0.0, -0.8))
