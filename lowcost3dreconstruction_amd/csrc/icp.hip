@@ -376,55 +376,87 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
 }
 
 // Second half of an iteration, one block per instance: folds the instance's tile
-// partials in a fixed order (thread t takes tiles t, t + kSolveThreads, ..., then
-// the block sum), then the FP64 solve, the composition final = Tinc * final and the
-// convergence step.  Kept out of k_icp_iter so that the search kernel carries none
-// of the solver's registers.
-constexpr int kSolveThreads = 256;
+// partials in a fixed order, then the FP64 solve, the composition final = Tinc *
+// final and the convergence step.  Kept out of k_icp_iter so that the search
+// kernel carries none of the solver's registers.
+//
+// The fold reads the partials row by row: lane i of each 32-lane row takes value
+// i, row r tiles r, r + kSolveRows, ... (kSolveUnroll loads in flight), so one
+// load instruction covers two tiles' contiguous values.  Each value is then the
+// sum of the rows' sums in row order.  (One thread per tile reading all of its
+// values touched 64 cache lines per instruction: 12 of the 15.5 us of a
+// 1611-tile instance went to that fold and a block sum.)
+constexpr int kSolveThreads = 1024;
+constexpr int kSolveRows = kSolveThreads / 32;
+constexpr int kSolveUnroll = 8;
+constexpr int kFoldThreads = 256;
 
 template <bool P2PL>
 __global__ void __launch_bounds__(kSolveThreads) k_icp_solve(const IcpBuffers b, const IcpConst c) {
   constexpr int NV = P2PL ? kP2LVals : kP2PVals;
-  __shared__ double lds[NV * (kSolveThreads / 64)];
+  static_assert(NV <= 32, "one value per lane of a row");
+  __shared__ double part[kSolveRows][32];
+  __shared__ double tot[32];
   PairState* st = b.state + blockIdx.x;
   if (st->active == 0) return;  // uniform across the block
   const PairDesc pd = b.pairs[blockIdx.x];
+  // the state, read now so that its latency hides behind the fold
+  PairState ls = *st;
+  const int i = threadIdx.x & 31, row = threadIdx.x >> 5;
+  double acc = 0.0;
+  if (i < NV) {
+    const double* in = b.partials + (int64_t)pd.tile_begin * kMaxVals + i;
+    int t = row;
+    for (; t + (kSolveUnroll - 1) * kSolveRows < pd.ntiles; t += kSolveUnroll * kSolveRows) {
+      double x[kSolveUnroll];
+#pragma unroll
+      for (int u = 0; u < kSolveUnroll; ++u) x[u] = in[(int64_t)(t + u * kSolveRows) * kMaxVals];
+#pragma unroll
+      for (int u = 0; u < kSolveUnroll; ++u) acc += x[u];
+    }
+    for (; t < pd.ntiles; t += kSolveRows) acc += in[(int64_t)t * kMaxVals];
+  }
+  part[row][i] = acc;
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double sum = part[0][threadIdx.x];
+#pragma unroll 8
+    for (int r = 1; r < kSolveRows; ++r) sum += part[r][threadIdx.x];
+    tot[threadIdx.x] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double v[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = 0.0;
-  for (int t = threadIdx.x; t < pd.ntiles; t += kSolveThreads) {
-    const double* in = b.partials + (int64_t)(pd.tile_begin + t) * kMaxVals;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] += in[i];
-  }
-  block_sum<NV>(v, lds);
-  if (threadIdx.x != 0) return;
+  for (int k = 0; k < NV; ++k) v[k] = tot[k];
   const double ncorr = v[0];
-  st->ncorr = (int)ncorr;
+  ls.ncorr = (int)ncorr;
   if (ncorr < 3.0) {
     // icp.hpp: fewer than min_number_correspondences_ (3) -> not converged
-    st->state = kNoCorrespondences;
-    st->converged = 0;
-    st->active = 0;
+    ls.state = kNoCorrespondences;
+    ls.converged = 0;
+    ls.active = 0;
+    *st = ls;
     return;
   }
   double Tinc[16];
   if constexpr (P2PL) {
-    st->solve_failed = p2pl_increment(v, Tinc) ? 0 : 1;
+    ls.solve_failed = p2pl_increment(v, Tinc) ? 0 : 1;
   } else {
     p2p_increment(v, Tinc);
   }
   double Tn[16];
-  mat4_mul(Tinc, st->T, Tn);
+  mat4_mul(Tinc, ls.T, Tn);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) st->T[i] = Tn[i];
-  st->iterations += 1;
+  for (int k = 0; k < 16; ++k) ls.T[k] = Tn[k];
+  ls.iterations += 1;
   const double mse = v[NV - 1] / ncorr;
-  st->last_mse = mse;
-  if (conv_step(st, c, Tinc, mse)) {
-    st->converged = 1;
-    st->active = 0;
+  ls.last_mse = mse;
+  if (conv_step(&ls, c, Tinc, mse)) {
+    ls.converged = 1;
+    ls.active = 0;
   }
+  *st = ls;
 }
 
 // per_instance: each instance's own max_corr_d2 is the range.  A finite range
@@ -497,15 +529,15 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
 // then the mean.  A separate launch: folding in the pass itself took a
 // device-scope fence and an atomic ticket per tile (each fence writes back the
 // XCD's L2), which cost cycle36 a third of the pass.
-__global__ void __launch_bounds__(kSolveThreads) k_fitness_fold(const IcpBuffers b, int per_instance) {
-  __shared__ double lds[2 * (kSolveThreads / 64)];
+__global__ void __launch_bounds__(kFoldThreads) k_fitness_fold(const IcpBuffers b, int per_instance) {
+  __shared__ double lds[2 * (kFoldThreads / 64)];
   PairState* st = b.state + blockIdx.x;
   if (st->gated) return;  // uniform across the block
   const PairDesc pd = b.pairs[blockIdx.x];
   const int stride = per_instance ? gate_stride(pd.ntiles) : 1;
   const int nparts = (pd.ntiles + stride - 1) / stride;
   double v[2] = {0.0, 0.0};
-  for (int t = threadIdx.x; t < nparts; t += kSolveThreads) {
+  for (int t = threadIdx.x; t < nparts; t += kFoldThreads) {
     const double* in = b.partials + (int64_t)(pd.tile_begin + t * stride) * kMaxVals;
     v[0] += in[0];
     v[1] += in[1];
@@ -834,7 +866,7 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
       break;
   }
   if (b.npairs > 0)
-    hipLaunchKernelGGL(k_fitness_fold, dim3(b.npairs), dim3(kSolveThreads), 0, s, b, pi);
+    hipLaunchKernelGGL(k_fitness_fold, dim3(b.npairs), dim3(kFoldThreads), 0, s, b, pi);
 }
 
 void launch_compact_tiles(const IcpBuffers& b, int* off, int2* it_list, int* it_count,
