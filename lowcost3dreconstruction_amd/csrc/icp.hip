@@ -385,8 +385,26 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
 // load instruction covers two tiles' contiguous values.  Each value is then the
 // sum of the rows' sums in row order.  (One thread per tile reading all of its
 // values touched 64 cache lines per instruction: 12 of the 15.5 us of a
-// 1611-tile instance went to that fold and a block sum.)
+// 1611-tile instance went to that fold and a block sum.)  Rows and row sums add
+// with compensation (CompSum): a plain sum along ~50 tiles carried several times
+// the rounding of the former tree, enough to move a run's final pose by a few
+// nanometres between tile sizes.
 constexpr int kSolveThreads = 1024;
+
+// Compensated (Kahan) summation: the running sum's rounding error is carried
+// and subtracted from the next term, so a row of ~50 tiles and the 32 rows add
+// with an error of a few ulps whatever their count and order.  (Every operation
+// is rounded as written: the library is built without contraction.)
+struct CompSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    const double y = x - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s - c; }
+};
 constexpr int kSolveRows = kSolveThreads / 32;
 constexpr int kSolveUnroll = 8;
 constexpr int kFoldThreads = 256;
@@ -403,7 +421,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_icp_solve(const IcpBuffers b,
   // the state, read now so that its latency hides behind the fold
   PairState ls = *st;
   const int i = threadIdx.x & 31, row = threadIdx.x >> 5;
-  double acc = 0.0;
+  CompSum acc;
   if (i < NV) {
     const double* in = b.partials + (int64_t)pd.tile_begin * kMaxVals + i;
     int t = row;
@@ -412,17 +430,17 @@ __global__ void __launch_bounds__(kSolveThreads) k_icp_solve(const IcpBuffers b,
 #pragma unroll
       for (int u = 0; u < kSolveUnroll; ++u) x[u] = in[(int64_t)(t + u * kSolveRows) * kMaxVals];
 #pragma unroll
-      for (int u = 0; u < kSolveUnroll; ++u) acc += x[u];
+      for (int u = 0; u < kSolveUnroll; ++u) acc.add(x[u]);
     }
-    for (; t < pd.ntiles; t += kSolveRows) acc += in[(int64_t)t * kMaxVals];
+    for (; t < pd.ntiles; t += kSolveRows) acc.add(in[(int64_t)t * kMaxVals]);
   }
-  part[row][i] = acc;
+  part[row][i] = acc.value();
   __syncthreads();
   if (threadIdx.x < NV) {
-    double sum = part[0][threadIdx.x];
+    CompSum sum;
 #pragma unroll 8
-    for (int r = 1; r < kSolveRows; ++r) sum += part[r][threadIdx.x];
-    tot[threadIdx.x] = sum;
+    for (int r = 0; r < kSolveRows; ++r) sum.add(part[r][threadIdx.x]);
+    tot[threadIdx.x] = sum.value();
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -463,6 +481,18 @@ __global__ void __launch_bounds__(kSolveThreads) k_icp_solve(const IcpBuffers b,
 // bounds the search at sqrt(range): only points within it are counted.
 // Six waves per SIMD (80 VGPRs, a few bytes spilled) measured 3 % faster than
 // the unconstrained five; eight (64 VGPRs) spilled in the walk and was slower.
+//
+// With b.fetch and 64-point tiles (one wavefront per block), the tiles are dealt
+// to kFetchGroups groups of blocks (group g: blocks g, g + kFetchGroups, ...; its
+// tiles g, g + kFetchGroups, ..., a sample of every part of the list), and the
+// blocks of a group take its tiles from the group's counter as they finish the
+// last, instead of fixed shares.  A tile whose points go to the tree walk costs
+// tens of times a seeded one, and with about six tiles per wavefront the fixed
+// shares left the mean wavefront idle for half of the pass.  One counter per
+// group on its own cache line: atomics on one address serialize, and eight
+// counters shared by every block cost more than the balance gained.  Each tile
+// writes its own partials, so the order does not change the result.  Larger
+// tiles keep fixed shares.
 template <int TILE>
 __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double max_range_arg,
                                                   int per_instance, int interleave) {
@@ -475,15 +505,14 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
   const bool listed = per_instance && b.gate_tiles;
   const bool kept = !per_instance && b.fit_count;
   const int nt = listed ? b.ngate_tiles : (kept ? *b.fit_count : b.ntiles);
-  const TileRange tr = xcd_tiles(nt, interleave);
-  for (int it = tr.first; it < tr.end; it += tr.step) {
+  auto tile_pass = [&](int it) {
     const int tile = listed ? b.gate_tiles[it] : (kept ? b.fit_list[it].x : it);
     const int p = b.tile_pair[tile];
     PairState* st = b.state + p;
-    if (st->gated) continue;  // uniform across the block
+    if (st->gated) return;  // uniform across the block
     const PairDesc pd = b.pairs[p];
     const int stride = per_instance ? gate_stride(pd.ntiles) : 1;
-    if ((tile - pd.tile_begin) % stride != 0) continue;  // uniform across the block
+    if ((tile - pd.tile_begin) % stride != 0) return;  // uniform across the block
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
     const double max_range = per_instance ? pd.max_corr_d2 : max_range_arg;
@@ -521,6 +550,25 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
       out[0] = v[0];
       out[1] = v[1];
     }
+  };
+  if (TILE == 64 && b.fetch) {
+    const int ng = min((int)gridDim.x, kFetchGroups);
+    const int g = (int)blockIdx.x % ng;
+    int* ctr = b.fetch + g * kFetchStride;
+    for (;;) {
+      // one ticket per tile: lane 0 adds 1 and every other lane 0, and each lane
+      // takes lane 0's old value into a scalar register.  No lane-0-only branch
+      // and no LDS broadcast: with those the compiler turned this loop's exits
+      // into per-lane ones, and lanes 1-63 repeated a tile while lane 0 waited
+      // for them to fetch the next.
+      const int ticket = atomicAdd(ctr, threadIdx.x == 0 ? 1 : 0);
+      const int it = g + ng * __builtin_amdgcn_readfirstlane(ticket);
+      if (it >= nt) break;
+      tile_pass(it);
+    }
+  } else {
+    const TileRange tr = xcd_tiles(nt, interleave);
+    for (int it = tr.first; it < tr.end; it += tr.step) tile_pass(it);
   }
 }
 
@@ -531,6 +579,11 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
 // XCD's L2), which cost cycle36 a third of the pass.
 __global__ void __launch_bounds__(kFoldThreads) k_fitness_fold(const IcpBuffers b, int per_instance) {
   __shared__ double lds[2 * (kFoldThreads / 64)];
+  // the pass's tile counters, ready for the next pass (the pass has ended: this
+  // launch follows it on the stream)
+  static_assert(kFoldThreads >= kFetchGroups, "one thread per counter");
+  if (b.fetch && blockIdx.x == 0 && threadIdx.x < kFetchGroups)
+    b.fetch[threadIdx.x * kFetchStride] = 0;
   PairState* st = b.state + blockIdx.x;
   if (st->gated) return;  // uniform across the block
   const PairDesc pd = b.pairs[blockIdx.x];
@@ -850,6 +903,8 @@ void launch_icp_solve(const IcpBuffers& b, const IcpConst& c, hipStream_t s) {
 
 void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStream_t s,
                     bool per_instance, int interleave) {
+  // (the fold launch below also clears the tile counters, so both run or neither)
+  if (b.npairs <= 0) return;
   const int pi = per_instance ? 1 : 0;
   switch (b.tile) {
     case 64:
@@ -865,8 +920,7 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
                          interleave);
       break;
   }
-  if (b.npairs > 0)
-    hipLaunchKernelGGL(k_fitness_fold, dim3(b.npairs), dim3(kFoldThreads), 0, s, b, pi);
+  hipLaunchKernelGGL(k_fitness_fold, dim3(b.npairs), dim3(kFoldThreads), 0, s, b, pi);
 }
 
 void launch_compact_tiles(const IcpBuffers& b, int* off, int2* it_list, int* it_count,

@@ -49,6 +49,12 @@ void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const f
                       float4* nodes, hipStream_t s);
 
 // ---- K2-K6 ICP iteration, fitness, output (icp.hip)
+// Tile counters of the fitness passes: kFetchGroups groups of blocks, each
+// counter on its own 128-byte line (kFetchStride ints apart).
+constexpr int kFetchGroups = 256;
+constexpr int kFetchStride = 32;
+constexpr int kFetchCounters = kFetchGroups * kFetchStride;
+
 struct IcpBuffers {
   const GridDesc* grids;
   const int* cell_start;
@@ -75,6 +81,10 @@ struct IcpBuffers {
   const int* it_count;
   const int2* fit_list;
   const int* fit_count;
+  // (kFetchCounters ints) the fitness passes' tile counters (nullable: fixed
+  // shares); zero before each pass, and zeroed again by the fold launch that
+  // follows it
+  int* fetch;
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)

@@ -517,6 +517,10 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   d_fit_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
   d_list_count.reserve(2 * sizeof(int));
   d_list_off.reserve((size_t)std::max(npairs, 1) * sizeof(int));
+  if (!d_fetch.p) {
+    d_fetch.reserve(kFetchCounters * sizeof(int));
+    LCR_HIP(hipMemset(d_fetch.p, 0, kFetchCounters * sizeof(int)));
+  }
   {
     std::vector<int2> all((size_t)ntiles);
     for (int t = 0; t < ntiles; ++t) all[t] = make_int2(t, tp[t]);
@@ -549,6 +553,16 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   }
 }
 
+// LCR_FIT_ORDER=fixed: the fitness passes in fixed per-block shares of the tiles
+// instead of tiles handed out by counters (read once).
+static bool fitness_fixed_shares() {
+  static const bool v = [] {
+    const char* e = getenv("LCR_FIT_ORDER");
+    return e && strcmp(e, "fixed") == 0;
+  }();
+  return v;
+}
+
 IcpBuffers Engine::buffers(const CloudSet& cs) const {
   IcpBuffers b;
   memset(&b, 0, sizeof b);  // padding takes part in the graph key comparison
@@ -571,6 +585,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.it_count = d_list_count.as<int>();
   b.fit_list = d_fit_list.as<int2>();
   b.fit_count = d_list_count.as<int>() + 1;
+  b.fetch = fitness_fixed_shares() ? nullptr : d_fetch.as<int>();
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;
