@@ -572,6 +572,62 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
   }
 }
 
+// The final pass's list of tiles (fit_list) reordered into out: tiles with a
+// point that had no match in its instance's last iteration first (such a point
+// goes straight to the tree walk, tens of times the cost of a seeded one), the
+// others after, so that the longest tiles start at once.  Each block takes one
+// range of each part with one atomic on each of the two fill counters (ctr,
+// ctr + kFetchStride): heavy tiles fill out from the front, the rest from the
+// back.  The order within each part follows the blocks' atomics; it changes no
+// result (each tile writes its own partials).
+constexpr int kOrderThreads = 256;
+
+__global__ void __launch_bounds__(kOrderThreads) k_fit_order(const IcpBuffers b, int2* __restrict__ out,
+                                                             int* __restrict__ ctr) {
+  constexpr int kWaves = kOrderThreads / 64;
+  __shared__ int s_cnt[2][kWaves];
+  __shared__ int s_base[2];
+  const int nt = *b.fit_count;
+  const int i = (int)(blockIdx.x * kOrderThreads + threadIdx.x);
+  const bool valid = i < nt;
+  bool heavy = false;
+  int2 e = make_int2(0, 0);
+  if (valid) {
+    e = b.fit_list[i];
+    const PairDesc pd = b.pairs[e.y];
+    const int n = min(64, b.grids[pd.src].npts - (e.x - pd.tile_begin) * 64);
+    const int* pv = b.prev + (int64_t)e.x * 64;
+    for (int j = 0; j < n && !heavy; ++j) heavy = pv[j] < 0;
+  }
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  const uint64_t hm = __ballot(valid && heavy), lm = __ballot(valid && !heavy);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (lane == 0) {
+    s_cnt[0][w] = __popcll(hm);
+    s_cnt[1][w] = __popcll(lm);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int h = 0, l = 0;
+    for (int k = 0; k < kWaves; ++k) {
+      const int ch = s_cnt[0][k], cl = s_cnt[1][k];
+      s_cnt[0][k] = h;
+      s_cnt[1][k] = l;
+      h += ch;
+      l += cl;
+    }
+    s_base[0] = h ? atomicAdd(ctr, h) : 0;
+    s_base[1] = l ? atomicAdd(ctr + kFetchStride, l) : 0;
+  }
+  __syncthreads();
+  if (valid) {
+    if (heavy)
+      out[s_base[0] + s_cnt[0][w] + __popcll(hm & below)] = e;
+    else
+      out[nt - 1 - (s_base[1] + s_cnt[1][w] + __popcll(lm & below))] = e;
+  }
+}
+
 // The fitness pass's second half, one block per instance that took part (every
 // instance the gate has not set aside): its tiles' partial sums in a fixed order,
 // then the mean.  A separate launch: folding in the pass itself took a
@@ -581,9 +637,8 @@ __global__ void __launch_bounds__(kFoldThreads) k_fitness_fold(const IcpBuffers 
   __shared__ double lds[2 * (kFoldThreads / 64)];
   // the pass's tile counters, ready for the next pass (the pass has ended: this
   // launch follows it on the stream)
-  static_assert(kFoldThreads >= kFetchGroups, "one thread per counter");
-  if (b.fetch && blockIdx.x == 0 && threadIdx.x < kFetchGroups)
-    b.fetch[threadIdx.x * kFetchStride] = 0;
+  if (b.fetch && blockIdx.x == 0)
+    for (int k = (int)threadIdx.x; k < kFetchLines; k += kFoldThreads) b.fetch[k * kFetchStride] = 0;
   PairState* st = b.state + blockIdx.x;
   if (st->gated) return;  // uniform across the block
   const PairDesc pd = b.pairs[blockIdx.x];
@@ -908,8 +963,19 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
   const int pi = per_instance ? 1 : 0;
   switch (b.tile) {
     case 64:
-      hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range, pi,
-                         interleave);
+      if (!per_instance && b.fetch && b.fit_order && b.fit_count && b.prev && b.ntiles > 0) {
+        // the final pass over fit_list, tree-walk tiles first
+        hipLaunchKernelGGL(k_fit_order, dim3((b.ntiles + kOrderThreads - 1) / kOrderThreads),
+                           dim3(kOrderThreads), 0, s, b, b.fit_order,
+                           b.fetch + kFetchGroups * kFetchStride);
+        IcpBuffers bo = b;
+        bo.fit_list = b.fit_order;
+        hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, bo, max_range, pi,
+                           interleave);
+      } else {
+        hipLaunchKernelGGL(k_fitness<64>, dim3(nblocks), dim3(64), 0, s, b, max_range, pi,
+                           interleave);
+      }
       break;
     case 128:
       hipLaunchKernelGGL(k_fitness<128>, dim3(nblocks), dim3(128), 0, s, b, max_range, pi,

@@ -49,11 +49,13 @@ void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const f
                       float4* nodes, hipStream_t s);
 
 // ---- K2-K6 ICP iteration, fitness, output (icp.hip)
-// Tile counters of the fitness passes: kFetchGroups groups of blocks, each
-// counter on its own 128-byte line (kFetchStride ints apart).
+// Tile counters of the fitness passes: kFetchGroups groups of blocks, then the
+// two fill counters of the final pass's reordered list, each counter on its own
+// 128-byte line (kFetchStride ints apart).
 constexpr int kFetchGroups = 256;
 constexpr int kFetchStride = 32;
-constexpr int kFetchCounters = kFetchGroups * kFetchStride;
+constexpr int kFetchLines = kFetchGroups + 2;
+constexpr int kFetchCounters = kFetchLines * kFetchStride;
 
 struct IcpBuffers {
   const GridDesc* grids;
@@ -85,6 +87,9 @@ struct IcpBuffers {
   // shares); zero before each pass, and zeroed again by the fold launch that
   // follows it
   int* fetch;
+  // room for one list like fit_list: the final pass's tiles reordered with the
+  // tree-walk tiles first (nullable: fit_list's order)
+  int2* fit_order;
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)

@@ -515,6 +515,7 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   // live-tile lists: every tile until a run compacts them
   d_it_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
   d_fit_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
+  d_fit_order.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
   d_list_count.reserve(2 * sizeof(int));
   d_list_off.reserve((size_t)std::max(npairs, 1) * sizeof(int));
   if (!d_fetch.p) {
@@ -586,6 +587,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.fit_list = d_fit_list.as<int2>();
   b.fit_count = d_list_count.as<int>() + 1;
   b.fetch = fitness_fixed_shares() ? nullptr : d_fetch.as<int>();
+  b.fit_order = b.fetch ? d_fit_order.as<int2>() : nullptr;
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;
@@ -691,8 +693,8 @@ void Engine::run(hipStream_t s, const CloudSet& cs, const double* guesses, bool 
   };
   auto tail = [&](std::vector<Profiler::Span>* sp) {
     const int k = prof.begin("fitness", s, sp);
-    // one launch over every instance: XCD-contiguous tiles keep each target's
-    // tree in one L2 (measured faster here; the iterations keep their order)
+    // one launch over every instance (with fixed shares, XCD-contiguous tiles keep
+    // each target's tree in one L2)
     launch_fitness(b, DBL_MAX, nblocks_fit, s, false, 0);
     prof.end(k, s, sp);
     LCR_HIP(hipMemcpyAsync(h_state.p, d_state.p, (size_t)npairs * sizeof(PairState),

@@ -187,7 +187,8 @@ struct Engine {
   std::vector<PairDesc> hp;
   DevBuf d_pairs, d_tile_pair, d_state, d_partials, d_guess, d_prev, d_gate_tiles;
   DevBuf d_it_list, d_fit_list, d_list_count, d_list_off;  // live-tile lists (IcpBuffers::it_list)
-  DevBuf d_fetch;  // IcpBuffers::fetch
+  DevBuf d_fetch;      // IcpBuffers::fetch
+  DevBuf d_fit_order;  // IcpBuffers::fit_order
   int ngate_tiles = 0;
   HostBuf h_state, h_guess;
   static constexpr int kChunk = 4;  // iterations per captured chunk
