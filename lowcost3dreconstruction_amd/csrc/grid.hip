@@ -197,8 +197,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int* lds, int& total) {
   return before + x - v;
 }
 
+// CONSUME: the input is set to zero as it is read.
+template <bool CONSUME>
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int64_t n, int* __restrict__ sums) {
+k_scan_tiles(int* __restrict__ in, int* __restrict__ out, int64_t n, int* __restrict__ sums) {
   __shared__ int lds[kScanThreads / 64];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
   int v[kScanPerThread];
@@ -207,6 +209,11 @@ k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int64_t n, int* 
   for (int k = 0; k < kScanPerThread; ++k) {
     v[k] = (base + k < n) ? in[base + k] : 0;
     s += v[k];
+  }
+  if constexpr (CONSUME) {
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+      if (base + k < n) in[base + k] = 0;
   }
   int total;
   int run = block_excl_scan(s, lds, total);
@@ -255,16 +262,19 @@ __global__ void k_scatter(const int* __restrict__ cid, int64_t n, const int* __r
 // Step 2, one thread per slot: the point's final position is its cell's start plus
 // the number of the cell's points with a smaller raw index (raw order within a
 // cloud is original-index order), so points of a cell end up sorted by original
-// index whatever order step 1 produced.
+// index whatever order step 1 produced.  The thread of a cell's first slot sets
+// the cell's step-1 counter back to zero (empty cells were never counted), so the
+// counters are all zero again for the next build.
 __global__ void k_place(const float4* __restrict__ raw, const float4* __restrict__ raw_nrm,
                         const int* __restrict__ slot_raw, const int* __restrict__ slot_cell,
                         int64_t n_sorted, const int* __restrict__ cell_start,
                         const int64_t* __restrict__ cloud_off, const int* __restrict__ point_cloud,
-                        float4* __restrict__ pts, float4* __restrict__ nrm) {
+                        float4* __restrict__ pts, float4* __restrict__ nrm, int* __restrict__ fill) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_sorted) return;
   const int c = slot_cell[j], key = slot_raw[j];
   const int b = cell_start[c], e = cell_start[c + 1];
+  if (j == b) fill[c] = 0;
   int rank = 0;
   for (int k = b; k < e; ++k) rank += slot_raw[k] < key ? 1 : 0;
   float4 p = raw[key];
@@ -274,42 +284,78 @@ __global__ void k_place(const float4* __restrict__ raw, const float4* __restrict
 }
 
 // Bounding-volume tree over each cloud's cell-sorted points: leaf i of cloud c
-// covers sorted points [16 i, 16 i + 16); padding leaves are empty boxes.
-__global__ void k_bvh_leaves(const GridDesc* __restrict__ grids, const float4* __restrict__ pts,
-                             float4* __restrict__ nodes) {
+// covers sorted points [16 i, 16 i + 16); padding leaves are empty boxes.  Level
+// l holds nodes 2^l .. 2^(l+1) - 1 of a 1-based heap, the leaves at l = depth.
+//
+// One launch builds kBvhLevels levels: block x of cloud y takes 256 nodes of the
+// input level depth - kBvhLevels * pass (the leaves, boxed from the points, when
+// pass is 0) and combines them in LDS level by level up to its subtree's root,
+// writing every node on the way.  A tree of depth d takes ceil(d / kBvhLevels)
+// launches (at least one), where one launch per level took d + 1.
+constexpr int kBvhThreads = 256;
+constexpr int kBvhLevels = 8;
+static_assert(kBvhThreads == 1 << kBvhLevels, "a block's subtree spans kBvhLevels levels");
+
+__global__ void __launch_bounds__(kBvhThreads)
+k_bvh_levels(const GridDesc* __restrict__ grids, int pass, const float4* __restrict__ pts,
+             float4* __restrict__ nodes) {
+  __shared__ float4 slo[kBvhThreads], shi[kBvhThreads];
   const GridDesc g = grids[blockIdx.y];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nslots = (int64_t)1 << g.depth;
-  if (i >= nslots) return;
+  const int in_level = g.depth - kBvhLevels * pass;
+  if (in_level < 0) return;  // uniform across the block
+  const int64_t first = (int64_t)blockIdx.x * kBvhThreads;
+  const int64_t width = (int64_t)1 << in_level;
+  if (first >= width) return;  // uniform across the block
+  const int count = (int)min<int64_t>(kBvhThreads, width);
+  float4* N = nodes + 2 * g.node_off;
+  const int t = (int)threadIdx.x;
   float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
   float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-  const int64_t s = i * kLeaf, e = min<int64_t>(s + kLeaf, g.npts);
-  for (int64_t j = s; j < e; ++j) {
-    const float4 p = pts[g.pt_off + j];
-    lo.x = fminf(lo.x, p.x);
-    lo.y = fminf(lo.y, p.y);
-    lo.z = fminf(lo.z, p.z);
-    hi.x = fmaxf(hi.x, p.x);
-    hi.y = fmaxf(hi.y, p.y);
-    hi.z = fmaxf(hi.z, p.z);
+  if (t < count) {
+    const int64_t node = width + first + t;
+    if (pass == 0) {
+      const int64_t s = (first + t) * kLeaf, e = min<int64_t>(s + kLeaf, g.npts);
+      for (int64_t j = s; j < e; ++j) {
+        const float4 p = pts[g.pt_off + j];
+        lo.x = fminf(lo.x, p.x);
+        lo.y = fminf(lo.y, p.y);
+        lo.z = fminf(lo.z, p.z);
+        hi.x = fmaxf(hi.x, p.x);
+        hi.y = fmaxf(hi.y, p.y);
+        hi.z = fmaxf(hi.z, p.z);
+      }
+      N[2 * node] = lo;
+      N[2 * node + 1] = hi;
+    } else {
+      lo = N[2 * node];
+      hi = N[2 * node + 1];
+    }
   }
-  float4* n = nodes + 2 * (g.node_off + nslots + i);
-  n[0] = lo;
-  n[1] = hi;
-}
-
-// one tree level, `step` levels above each cloud's leaves
-__global__ void k_bvh_up(const GridDesc* __restrict__ grids, int step, float4* __restrict__ nodes) {
-  const GridDesc g = grids[blockIdx.y];
-  const int level = g.depth - step;
-  if (level < 0) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ((int64_t)1 << level)) return;
-  const int64_t node = ((int64_t)1 << level) + i;
-  float4* N = nodes + 2 * g.node_off;
-  const float4 l0 = N[4 * node], l1 = N[4 * node + 1], r0 = N[4 * node + 2], r1 = N[4 * node + 3];
-  N[2 * node] = make_float4(fminf(l0.x, r0.x), fminf(l0.y, r0.y), fminf(l0.z, r0.z), 0.f);
-  N[2 * node + 1] = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
+  slo[t] = lo;
+  shi[t] = hi;
+  __syncthreads();
+  int level = in_level, n = count;
+  int64_t base = first;  // index of the block's first node within its level
+  while (n > 1 && level > 0) {
+    n >>= 1;
+    --level;
+    base >>= 1;
+    float4 a, b;
+    if (t < n) {
+      const float4 l0 = slo[2 * t], l1 = shi[2 * t], r0 = slo[2 * t + 1], r1 = shi[2 * t + 1];
+      a = make_float4(fminf(l0.x, r0.x), fminf(l0.y, r0.y), fminf(l0.z, r0.z), 0.f);
+      b = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
+      const int64_t node = ((int64_t)1 << level) + base + t;
+      N[2 * node] = a;
+      N[2 * node + 1] = b;
+    }
+    __syncthreads();
+    if (t < n) {
+      slo[t] = a;
+      shi[t] = b;
+    }
+    __syncthreads();
+  }
 }
 
 // grid owning a global cell_start entry (grids in cell_off order)
@@ -450,13 +496,10 @@ void launch_bbox(const float4* raw, const int64_t* cloud_off, int nclouds, float
 void launch_bvh_build(const GridDesc* grids, int nclouds, int max_depth, const float4* pts,
                       float4* nodes, hipStream_t s) {
   if (nclouds <= 0) return;
-  const int64_t slots = (int64_t)1 << max_depth;
-  hipLaunchKernelGGL(k_bvh_leaves, dim3(blocks_for(slots, 256), nclouds), dim3(256), 0, s, grids,
-                     pts, nodes);
-  for (int step = 1; step <= max_depth; ++step) {
-    const int64_t n = (int64_t)1 << (max_depth - step);
-    hipLaunchKernelGGL(k_bvh_up, dim3(blocks_for(n, 256), nclouds), dim3(256), 0, s, grids, step,
-                       nodes);
+  for (int pass = 0; pass == 0 || kBvhLevels * pass < max_depth; ++pass) {
+    const int64_t width = (int64_t)1 << (max_depth - kBvhLevels * pass);
+    hipLaunchKernelGGL(k_bvh_levels, dim3(blocks_for(width, kBvhThreads), nclouds),
+                       dim3(kBvhThreads), 0, s, grids, pass, pts, nodes);
   }
 }
 
@@ -511,8 +554,17 @@ int64_t scan_tmp_ints(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s) {
   if (n <= 0) return;
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, out, n,
-                     tmp);
+  hipLaunchKernelGGL(k_scan_tiles<false>, dim3((unsigned)tiles), dim3(kScanThreads), 0, s,
+                     const_cast<int*>(in), out, n, tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, tmp, (int)tiles);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, s, out, n, tmp);
+}
+
+void launch_exclusive_scan_consume(int* in, int* out, int64_t n, int* tmp, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tiles<true>, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, out,
+                     n, tmp);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, tmp, (int)tiles);
   hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, s, out, n, tmp);
 }
@@ -526,7 +578,8 @@ void launch_place_points(const float4* raw, const float4* raw_nrm, const int* ci
                      cell_start, fill, slot_raw, slot_cell);
   if (n_sorted <= 0) return;
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n_sorted, 256)), dim3(256), 0, s, raw, raw_nrm,
-                     slot_raw, slot_cell, n_sorted, cell_start, cloud_off, point_cloud, pts, nrm);
+                     slot_raw, slot_cell, n_sorted, cell_start, cloud_off, point_cloud, pts, nrm,
+                     fill);
 }
 
 }  // namespace lcr

@@ -19,8 +19,11 @@ void launch_occupancy(const GridDesc* grids, int nclouds, const int* counts, int
                       hipStream_t s);
 int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
+// The same, setting the input to zero as it is read.
+void launch_exclusive_scan_consume(int* in, int* out, int64_t n, int* tmp, hipStream_t s);
 // Places every finite raw point at its cell-sorted position (cells in order,
-// points of a cell by original index); fill must be zero, slot_* hold n_sorted ints.
+// points of a cell by original index); fill must be zero on entry and is zero
+// again on exit, slot_* hold n_sorted ints.
 void launch_place_points(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n_raw,
                          int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
                          int* slot_cell, const int64_t* cloud_off, const int* point_cloud,

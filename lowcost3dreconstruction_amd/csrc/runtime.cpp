@@ -270,6 +270,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   d_cid.reserve((size_t)std::max<int64_t>(total_raw, 1) * sizeof(int));
   if (!geom_valid) {
     d_counts.reserve((size_t)cells * sizeof(int));
+    counts_zero = false;  // the probe counts below stay in the array
     d_occ.reserve((size_t)n * sizeof(int));
     memcpy(hg, grids.data(), (size_t)n * sizeof(GridDesc));
     LCR_HIP(hipMemcpyAsync(d_grids.p, hg, (size_t)n * sizeof(GridDesc), hipMemcpyHostToDevice, s));
@@ -337,19 +338,25 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   memcpy(hg, grids.data(), (size_t)n * sizeof(GridDesc));
   LCR_HIP(hipMemcpyAsync(d_grids.p, hg, (size_t)n * sizeof(GridDesc), hipMemcpyHostToDevice, s));
 
-  // 4. count, scan, scatter, per-cell sort
+  // 4. count, scan, scatter, per-cell sort.  The counters are zero over the whole
+  // allocation between builds (the scan consumes the counts, the placement
+  // clears its slot counters), so they are filled only after a reallocation or
+  // the geometry probe.
+  const void* counts_before = d_counts.p;
   d_counts.reserve((size_t)cells * sizeof(int));
+  if (d_counts.p != counts_before) counts_zero = false;
   d_cell_start.reserve((size_t)cells * sizeof(int));
   d_scan_tmp.reserve((size_t)scan_tmp_ints(cells) * sizeof(int));
   const size_t npts = (size_t)std::max<int64_t>(total_pts, 1);
   d_pts.reserve(npts * sizeof(float4));
   d_nrm.reserve(npts * sizeof(float4));
   d_cloud_of_sorted.reserve(npts * sizeof(int));
-  launch_fill_i32(d_counts.as<int>(), cells, 0, s);
+  if (!counts_zero) launch_fill_i32(d_counts.as<int>(), (int64_t)(d_counts.bytes / sizeof(int)), 0, s);
+  counts_zero = false;
   launch_cell_count(d_raw.as<float4>(), d_point_cloud.as<int>(), d_grids.as<GridDesc>(),
                     total_raw, d_cid.as<int>(), d_counts.as<int>(), nullptr, s);
-  launch_exclusive_scan(d_counts.as<int>(), d_cell_start.as<int>(), cells, d_scan_tmp.as<int>(), s);
-  launch_fill_i32(d_counts.as<int>(), cells, 0, s);
+  launch_exclusive_scan_consume(d_counts.as<int>(), d_cell_start.as<int>(), cells,
+                                d_scan_tmp.as<int>(), s);
   d_slot_raw.reserve(npts * sizeof(int));
   d_slot_cell.reserve(npts * sizeof(int));
   launch_place_points(d_raw.as<float4>(), has_nrm ? d_raw_nrm.as<float4>() : nullptr,
@@ -357,6 +364,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
                       d_counts.as<int>(), d_slot_raw.as<int>(), d_slot_cell.as<int>(),
                       d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), d_pts.as<float4>(),
                       d_nrm.as<float4>(), s);
+  counts_zero = true;
   // sorted point -> cloud
   for (int c = 0; c < n; ++c) hpoff[c] = grids[c].pt_off;
   hpoff[n] = total_pts;

@@ -138,6 +138,9 @@ struct CloudSet {
 
   DevBuf d_raw, d_raw_nrm, d_point_cloud, d_cloud_off;
   DevBuf d_grids, d_counts, d_cell_start, d_cid, d_scan_tmp, d_bbox_part, d_bbox, d_occ;
+  // d_counts is zero over its whole allocation once the stream has run the work
+  // issued so far (each build leaves it so; the geometry probe does not)
+  bool counts_zero = false;
   DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel;
   DevBuf d_tpts, d_tpos, d_tkeys, d_tkeys_sorted, d_tvals, d_ttmp;  // tree order
   DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
