@@ -519,15 +519,22 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
     const bool bounded = max_range < DBL_MAX && max_range >= 0.0;
     const float rq = bounded ? (float)sqrt(max_range) : INFINITY;
     const float r = rq * 1.00001f;
-    const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
+    // the point's rank in its instance's source: in cell order for the gate, in
+    // the cloud's tree (Morton) order for the final pass, whose tiles are then
+    // compact patches (a tile of cell order can run along a whole row of cells,
+    // 20 cm on cycle36, and its far points' tree walks then share little)
+    const int rank = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
+    const bool morton = !per_instance && b.tpts;
     double v[2] = {0.0, 0.0};
-    const bool own = local < gs.npts;
+    const bool own = rank < gs.npts;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
     if (own) {
-      xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+      // local: the point's cell-order position, which indexes its seed
+      const int local = morton ? b.tpos[gs.pt_off + rank] - (int)gs.pt_off : rank;
+      xform(st->T, morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank], qx, qy, qz);
       const bool empty = bounded && b.near && gt.near_r >= rq && !near_cell(gt, b.near, qx, qy, qz);
-      const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
+      const int seed = b.prev ? b.prev[(int64_t)pd.tile_begin * TILE + local] : -1;
       // the final pass over refined instances: a point without a match within r in
       // the last iteration (seed -1) is farther than about r from the target; a
       // ring search would scan two unpruned rings before handing it to the tree,
@@ -595,9 +602,15 @@ __global__ void __launch_bounds__(kOrderThreads) k_fit_order(const IcpBuffers b,
   if (valid) {
     e = b.fit_list[i];
     const PairDesc pd = b.pairs[e.y];
-    const int n = min(64, b.grids[pd.src].npts - (e.x - pd.tile_begin) * 64);
-    const int* pv = b.prev + (int64_t)e.x * 64;
-    for (int j = 0; j < n && !heavy; ++j) heavy = pv[j] < 0;
+    const GridDesc gs = b.grids[pd.src];
+    const int first = (e.x - pd.tile_begin) * 64;
+    const int n = min(64, gs.npts - first);
+    const int* pv = b.prev + (int64_t)pd.tile_begin * 64;
+    // the tile's points in the cloud's tree order, as the final pass takes them
+    for (int j = 0; j < n && !heavy; ++j) {
+      const int local = b.tpts ? b.tpos[gs.pt_off + first + j] - (int)gs.pt_off : first + j;
+      heavy = pv[local] < 0;
+    }
   }
   const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
   const uint64_t hm = __ballot(valid && heavy), lm = __ballot(valid && !heavy);
