@@ -100,9 +100,20 @@ int tile_order_interleave() {
   return v;
 }
 
+// The ICP iterations take each tile's points in the source's tree (Morton)
+// order (read once); LCR_ICP_ORDER=cell takes them in cell order.
+bool icp_point_order_morton() {
+  static const bool v = [] {
+    const char* e = getenv("LCR_ICP_ORDER");
+    return !(e && strcmp(e, "cell") == 0);
+  }();
+  return v;
+}
+
 IcpConst make_const(const lcr_icp_params& p) {
   IcpConst c{};
   c.interleave = tile_order_interleave();
+  c.morton = icp_point_order_morton() ? 1 : 0;
   c.max_iterations = p.max_iterations;
   c.point_to_plane = p.point_to_plane ? 1 : 0;
   c.rotation_threshold = 1.0 - p.transformation_epsilon;

@@ -199,14 +199,17 @@ __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const 
     const PairDesc pd = b.pairs[p];
     const GridDesc gs = b.grids[pd.src];
     const GridDesc gt = b.grids[pd.tgt];
-    const int local = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
-    const bool own = local < gs.npts;
+    const int rank = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
+    const bool own = rank < gs.npts;
     const float r = (float)sqrt(pd.max_corr_d2);
     const bool bounded = isfinite(r);
-    const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
+    const bool morton = c.morton && b.tpts;
+    // the point's cell-order position indexes its previous match
+    const int local = (own && morton) ? b.tpos[gs.pt_off + rank] - (int)gs.pt_off : rank;
+    const int64_t slot = (int64_t)pd.tile_begin * TILE + local;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
-    if (own) xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+    if (own) xform(st->T, morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank], qx, qy, qz);
     // no target point within r of the query's cell: no correspondence, no search
     const bool empty = own && bounded && b.near && gt.near_r >= r &&
                        !near_cell(gt, b.near, qx, qy, qz);
@@ -333,15 +336,23 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
     const int base = (tile - pd.tile_begin) * TILE;
     // every radius is finite here (IcpConst::ring_only)
     const float r = (float)sqrt(pd.max_corr_d2);
+    const bool morton = c.morton && b.tpts;
+    // a query's point (in tree or cell order) and its cell-order position, which
+    // indexes its previous match
+    auto point_of = [&](int rank, int& local) {
+      local = morton ? b.tpos[gs.pt_off + rank] - (int)gs.pt_off : rank;
+      return morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank];
+    };
     for (int pass = 0; pass < TILE / 64; ++pass) {
       const int qi = pass * 64 + q;
       const bool own = base + qi < gs.npts;
       float qx = 0.f, qy = 0.f, qz = 0.f;
+      int local = 0;
       NNResult nn{INFINITY, 0x7fffffff, -1, false};
-      if (own) xform(st->T, b.pts[gs.pt_off + base + qi], qx, qy, qz);
+      if (own) xform(st->T, point_of(base + qi, local), qx, qy, qz);
       const bool empty = own && b.near && gt.near_r >= r && !near_cell(gt, b.near, qx, qy, qz);
       if (own && !empty) {
-        const int seed = b.prev ? b.prev[(int64_t)tile * TILE + qi] : -1;
+        const int seed = b.prev ? b.prev[(int64_t)pd.tile_begin * TILE + local] : -1;
         nn = nn_ring_split<S>(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r, seed, sub);
       }
       if (sub == 0) {
@@ -351,16 +362,16 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
     }
     __syncthreads();
     if (threadIdx.x < TILE) {
-      const int local = base + (int)threadIdx.x;
-      const bool own = local < gs.npts;
+      const bool own = base + (int)threadIdx.x < gs.npts;
       float qx = 0.f, qy = 0.f, qz = 0.f;
-      if (own) xform(st->T, b.pts[gs.pt_off + local], qx, qy, qz);
+      int local = 0;
+      if (own) xform(st->T, point_of(base + (int)threadIdx.x, local), qx, qy, qz);
       const float d2 = s_d2[threadIdx.x];
       const int j = s_j[threadIdx.x];
       bool has = false;
       float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
       if (own) {
-        if (b.prev) b.prev[(int64_t)tile * TILE + threadIdx.x] = j;
+        if (b.prev) b.prev[(int64_t)pd.tile_begin * TILE + local] = j;
         if (j >= 0 && (double)d2 <= pd.max_corr_d2) {
           has = true;
           t = b.pts[j];

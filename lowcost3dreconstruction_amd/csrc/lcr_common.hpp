@@ -95,6 +95,8 @@ struct IcpConst {
   int interleave;                // tile order: 0 XCD-contiguous, 1 grid stride, 2 by live tiles
   int ring_only;                 // 1: every instance's radius spans at most kRingOnlyCells
                                  // target cells, so the search never needs the tree
+  int morton;                    // 1: a tile's points in the source's tree (Morton) order
+                                 // (compact patches), 0: in cell order
 };
 
 // The hypothesis gate scores each pose on every s-th work tile of the source,
