@@ -7,12 +7,14 @@
 // on BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, from BF16 copies
 // of W2/W3 and BF16 activations h1/h2.
 //
-// Forward (k_enc_points): one block per 256 points of one cloud.  Layer 1 writes
-// h1 (bf16) to LDS; each of the 4 waves computes a 32-column slab of layer 2 for
-// all 256 rows (its B fragments stay in registers) and writes h2 (bf16) to LDS;
+// Forward (k_enc_points): one block per kRows (128) points of one cloud.  Layer 1
+// writes h1 (bf16) to LDS; each of the 4 waves computes a 32-column slab of layer
+// 2 for all rows (its B fragments stay in registers) and writes h2 (bf16) to LDS;
 // each wave then computes a 64-column slab of layer 3 and folds bias, ReLU and
 // the max-pool (with the lowest point index on ties) into registers, so only 256
-// (max, argmax) pairs per block leave the chip.
+// (max, argmax) pairs per block leave the chip.  At 128 rows a block holds 53 KB
+// of LDS and two or three blocks share a CU, so one block's MFMA work overlaps
+// another's epilogues; at 256 rows (106 KB) a CU held one.
 //
 // Backward: the max-pool sends each channel's gradient to one point, so the
 // per-point layers only need the <= 256 argmax rows per cloud (k_enc_rows
@@ -30,7 +32,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRows = 256;      // points per forward block
+constexpr int kRows = 128;      // points per forward block (two blocks' LDS fit a CU)
 constexpr int kH1S = 64 + 8;    // LDS row strides (bf16), padded
 constexpr int kH2S = 128 + 8;
 
@@ -51,19 +53,22 @@ k_enc_points(const float4* __restrict__ X, int npts, const float* __restrict__ W
   const float* b1 = W + kEncB1;
   const float* b2 = W + kEncB2;
   const float* b3 = W + kEncB3;
-  // layer 1 (FP32), one point per thread
+  // layer 1 (FP32), kL1Threads threads per point, each a slice of the 64 outputs
   {
-    const int p = p0 + t;
+    constexpr int kL1Threads = 256 / kRows, kL1Cols = 64 / kL1Threads;
+    const int row = t % kRows, j0 = (t / kRows) * kL1Cols;
+    const int p = p0 + row;
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
     const bool ok = p < npts;
     if (ok) x = X[(int64_t)cloud * npts + p];
 #pragma unroll 8
-    for (int j = 0; j < 64; ++j) {
+    for (int jj = 0; jj < kL1Cols; ++jj) {
+      const int j = j0 + jj;
       float a = b1[j];
       a = fmaf(x.x, W1[j], a);
       a = fmaf(x.y, W1[64 + j], a);
       a = fmaf(x.z, W1[128 + j], a);
-      H1[t * kH1S + j] = (__bf16)(ok ? relu(a) : 0.f);
+      H1[row * kH1S + j] = (__bf16)(ok ? relu(a) : 0.f);
     }
   }
   __syncthreads();
@@ -460,6 +465,8 @@ inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
+int enc_forward_chunks(int npts) { return (npts + kRows - 1) / kRows; }
+
 void launch_enc_bf16_weights(const float* W, __bf16* W2t, __bf16* W3t, hipStream_t s) {
   hipLaunchKernelGGL(k_enc_bf16_weights, dim3(nb(128 * 256, 256)), dim3(256), 0, s, W, W2t, W3t);
 }
@@ -467,7 +474,7 @@ void launch_enc_bf16_weights(const float* W, __bf16* W2t, __bf16* W3t, hipStream
 void launch_enc_forward(const EncBuffers& e, int npairs, int npts, const float* labels,
                         hipStream_t s) {
   const int nclouds = 2 * npairs;
-  const int nchunks = (npts + kRows - 1) / kRows;
+  const int nchunks = enc_forward_chunks(npts);
   hipLaunchKernelGGL(k_enc_points, dim3(nchunks, nclouds), dim3(256), 0, s, e.X, npts, e.W, e.W2t,
                      e.W3t, e.part_max, e.part_arg);
   hipLaunchKernelGGL(k_enc_pool, dim3(nclouds), dim3(256), 0, s, e.part_max, e.part_arg, nchunks,

@@ -227,6 +227,8 @@ struct EncBuffers {
   float* G;              // gradient (kEncNParam)
 };
 void launch_enc_bf16_weights(const float* W, __bf16* W2t, __bf16* W3t, hipStream_t s);
+// Forward blocks (and max-pool partials per channel) per cloud of npts points.
+int enc_forward_chunks(int npts);
 void launch_enc_forward(const EncBuffers& e, int npairs, int npts, const float* labels,
                         hipStream_t s);
 void launch_enc_backward(const EncBuffers& e, int npairs, int npts, hipStream_t s);

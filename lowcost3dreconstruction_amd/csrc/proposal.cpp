@@ -220,7 +220,7 @@ struct ProposalImpl {
     cap_pairs = std::max(cap_pairs, npairs);
     cap_pts = std::max(cap_pts, npts);
     const int64_t P = cap_pairs, C = 2 * P, N = cap_pts;
-    const int64_t chunks = (N + 255) / 256;
+    const int64_t chunks = enc_forward_chunks((int)N);
     X.reserve(C * N * sizeof(float4));
     part_max.reserve(C * chunks * 256 * sizeof(float));
     part_arg.reserve(C * chunks * 256 * sizeof(int));
