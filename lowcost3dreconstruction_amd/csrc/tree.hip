@@ -6,7 +6,7 @@
 // cell-sorted order its upper nodes would be thin slabs of whole grid planes;
 // built over the Morton order of the points every node bounds a compact region,
 // so a far query prunes most of the tree at once.  The order is a radix sort of
-// (cloud, 30-bit Morton code) keys carrying each point's cell-sorted position;
+// (cloud, 24-bit Morton code) keys carrying each point's cell-sorted position;
 // the points are then gathered into that order.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -19,14 +19,17 @@ namespace lcr {
 
 namespace {
 
+// key = (cloud, Morton code): 32-bit keys while the cloud index fits in the
+// bits above the code, 64-bit otherwise.
+template <class Key>
 __global__ void k_tree_keys(const GridDesc* __restrict__ grids, const float4* __restrict__ pts,
                             const int* __restrict__ cloud_of_sorted, int64_t n,
-                            uint64_t* __restrict__ keys, int* __restrict__ vals) {
+                            Key* __restrict__ keys, int* __restrict__ vals) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int c = cloud_of_sorted[j];
   const float4 p = pts[j];
-  keys[j] = ((uint64_t)c << 30) | tree_key(grids[c], p.x, p.y, p.z);
+  keys[j] = ((Key)c << kTreeKeyBits) | (Key)tree_key(grids[c], p.x, p.y, p.z);
   vals[j] = (int)j;
 }
 
@@ -39,19 +42,25 @@ __global__ void k_tree_gather(const float4* __restrict__ pts, const int* __restr
 int key_bits(int nclouds) {
   int b = 0;
   while ((1 << b) < nclouds) ++b;
-  return 30 + b;
+  return kTreeKeyBits + b;
 }
+
+bool narrow_keys(int nclouds) { return key_bits(nclouds) <= 32; }
 
 }  // namespace
 
 size_t tree_order_tmp_bytes(int64_t npts, int nclouds) {
   size_t bytes = 0;
   if (npts <= 0) return 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                         (const int*)nullptr, (int*)nullptr, (int)npts, 0,
-                                         key_bits(nclouds)) != hipSuccess)
-    return 0;
-  return bytes;
+  const hipError_t e =
+      narrow_keys(nclouds)
+          ? hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr,
+                                               (uint32_t*)nullptr, (const int*)nullptr, (int*)nullptr,
+                                               (int)npts, 0, key_bits(nclouds))
+          : hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t*)nullptr,
+                                               (uint64_t*)nullptr, (const int*)nullptr, (int*)nullptr,
+                                               (int)npts, 0, key_bits(nclouds));
+  return e == hipSuccess ? bytes : 0;
 }
 
 hipError_t launch_tree_order(const GridDesc* grids, int nclouds, const float4* pts,
@@ -60,11 +69,21 @@ hipError_t launch_tree_order(const GridDesc* grids, int nclouds, const float4* p
                              float4* tpts, hipStream_t s) {
   if (npts <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((npts + 255) / 256);
-  hipLaunchKernelGGL(k_tree_keys, dim3(blocks), dim3(256), 0, s, grids, pts, cloud_of_sorted, npts,
-                     keys, vals);
   size_t bytes = tmp_bytes;
-  const hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys_sorted, vals, tpos,
-                                                          (int)npts, 0, key_bits(nclouds), s);
+  hipError_t e;
+  if (narrow_keys(nclouds)) {
+    uint32_t* k = reinterpret_cast<uint32_t*>(keys);
+    uint32_t* ks = reinterpret_cast<uint32_t*>(keys_sorted);
+    hipLaunchKernelGGL(k_tree_keys<uint32_t>, dim3(blocks), dim3(256), 0, s, grids, pts,
+                       cloud_of_sorted, npts, k, vals);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, ks, vals, tpos, (int)npts, 0,
+                                           key_bits(nclouds), s);
+  } else {
+    hipLaunchKernelGGL(k_tree_keys<uint64_t>, dim3(blocks), dim3(256), 0, s, grids, pts,
+                       cloud_of_sorted, npts, keys, vals);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys_sorted, vals, tpos, (int)npts, 0,
+                                           key_bits(nclouds), s);
+  }
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tree_gather, dim3(blocks), dim3(256), 0, s, pts, tpos, npts, tpts);
   return hipSuccess;
