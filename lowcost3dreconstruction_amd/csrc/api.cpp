@@ -567,12 +567,16 @@ int lcr_cycle_run(lcr_cycle* c) {
     if (c->need_normals) {
       // normals on a grid with cells sized for a k-neighbourhood (~sqrt(k) point
       // spacings), then the ICP grid with cells sized for 1-NN
+      // the ICP grid is sorted from the normals grid's order (spatially coherent
+      // input; the normals stay in sorted order)
       c->cs.has_nrm = false;
-      c->cs.build(c->stream.s, std::max(2.5, 0.70 * sqrt((double)k)), &spans, false);
-      c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans);
-      c->cs.normals_to_raw(c->stream.s);
+      c->cs.build(c->stream.s, std::max(2.5, 0.70 * sqrt((double)k)), &spans, false,
+                  CloudSet::kBuildNormals);
+      c->cs.compute_normals(c->stream.s, k, c->vps.data(), 0, &spans, true);
+      c->cs.build(c->stream.s, d.params.cell_factor, &spans, true, CloudSet::kBuildFromNormals);
+    } else {
+      c->cs.build(c->stream.s, d.params.cell_factor, &spans);
     }
-    c->cs.build(c->stream.s, d.params.cell_factor, &spans);
     c->cs.build_near_mask(c->stream.s, d.params.max_corr_dist);
     if (d.params.point_to_plane && !c->cs.has_nrm)
       return fail(LCR_E_NORMALS, "point-to-plane ICP needs normals");

@@ -264,12 +264,14 @@ __global__ void k_scatter(const int* __restrict__ cid, int64_t n, const int* __r
 // cloud is original-index order), so points of a cell end up sorted by original
 // index whatever order step 1 produced.  The thread of a cell's first slot sets
 // the cell's step-1 counter back to zero (empty cells were never counted), so the
-// counters are all zero again for the next build.
+// counters are all zero again for the next build.  keep_w: the input points
+// already carry their original index in w (an earlier build's sorted points).
 __global__ void k_place(const float4* __restrict__ raw, const float4* __restrict__ raw_nrm,
                         const int* __restrict__ slot_raw, const int* __restrict__ slot_cell,
                         int64_t n_sorted, const int* __restrict__ cell_start,
                         const int64_t* __restrict__ cloud_off, const int* __restrict__ point_cloud,
-                        float4* __restrict__ pts, float4* __restrict__ nrm, int* __restrict__ fill) {
+                        float4* __restrict__ pts, float4* __restrict__ nrm, int* __restrict__ fill,
+                        int keep_w) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_sorted) return;
   const int c = slot_cell[j], key = slot_raw[j];
@@ -278,7 +280,7 @@ __global__ void k_place(const float4* __restrict__ raw, const float4* __restrict
   int rank = 0;
   for (int k = b; k < e; ++k) rank += slot_raw[k] < key ? 1 : 0;
   float4 p = raw[key];
-  p.w = __int_as_float((int)(key - cloud_off[point_cloud[key]]));
+  if (!keep_w) p.w = __int_as_float((int)(key - cloud_off[point_cloud[key]]));
   pts[b + rank] = p;
   if (raw_nrm) nrm[b + rank] = raw_nrm[key];
 }
@@ -572,14 +574,14 @@ void launch_exclusive_scan_consume(int* in, int* out, int64_t n, int* tmp, hipSt
 void launch_place_points(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n_raw,
                          int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
                          int* slot_cell, const int64_t* cloud_off, const int* point_cloud,
-                         float4* pts, float4* nrm, hipStream_t s) {
+                         float4* pts, float4* nrm, bool keep_w, hipStream_t s) {
   if (n_raw <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n_raw, 256)), dim3(256), 0, s, cid, n_raw,
                      cell_start, fill, slot_raw, slot_cell);
   if (n_sorted <= 0) return;
   hipLaunchKernelGGL(k_place, dim3(blocks_for(n_sorted, 256)), dim3(256), 0, s, raw, raw_nrm,
                      slot_raw, slot_cell, n_sorted, cell_start, cloud_off, point_cloud, pts, nrm,
-                     fill);
+                     fill, keep_w ? 1 : 0);
 }
 
 }  // namespace lcr

@@ -21,13 +21,15 @@ int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
 // The same, setting the input to zero as it is read.
 void launch_exclusive_scan_consume(int* in, int* out, int64_t n, int* tmp, hipStream_t s);
-// Places every finite raw point at its cell-sorted position (cells in order,
-// points of a cell by original index); fill must be zero on entry and is zero
-// again on exit, slot_* hold n_sorted ints.
+// Places every finite input point at its cell-sorted position (cells in order,
+// points of a cell by input index); fill must be zero on entry and is zero again
+// on exit, slot_* hold n_sorted ints.  The input is the raw upload (w set to the
+// point's index in its cloud) or, with keep_w, an earlier build's sorted points
+// (w already holds it).
 void launch_place_points(const float4* raw, const float4* raw_nrm, const int* cid, int64_t n_raw,
                          int64_t n_sorted, const int* cell_start, int* fill, int* slot_raw,
                          int* slot_cell, const int64_t* cloud_off, const int* point_cloud,
-                         float4* pts, float4* nrm, hipStream_t s);
+                         float4* pts, float4* nrm, bool keep_w, hipStream_t s);
 // Dilated occupancy per cell, one bit (set: some point within r of the cell):
 // occupancy, then three separable passes; tmp and out hold total_words words
 // (the grids' near_off must be set first, launch_set_near).

@@ -207,7 +207,10 @@ CloudSet::~CloudSet() {
 }
 
 void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans,
-                     bool with_tree) {
+                     bool with_tree, int stage) {
+  const bool from_normals = stage == kBuildFromNormals;
+  // the sorted input's layout (finite points per cloud) comes from the same upload
+  if (from_normals && !geom_valid) throw Error{-5, "no normals-stage build to start from"};
   // the previous build's copies out of the pinned region must be done
   if (ev_build)
     LCR_HIP(hipEventSynchronize(ev_build));
@@ -348,22 +351,31 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
   d_cell_start.reserve((size_t)cells * sizeof(int));
   d_scan_tmp.reserve((size_t)scan_tmp_ints(cells) * sizeof(int));
   const size_t npts = (size_t)std::max<int64_t>(total_pts, 1);
-  d_pts.reserve(npts * sizeof(float4));
-  d_nrm.reserve(npts * sizeof(float4));
+  DevBuf& out_pts = stage == kBuildNormals ? d_npts : d_pts;
+  DevBuf& out_nrm = stage == kBuildNormals ? d_nnrm : d_nrm;
+  out_pts.reserve(npts * sizeof(float4));
+  out_nrm.reserve(npts * sizeof(float4));
   d_cloud_of_sorted.reserve(npts * sizeof(int));
+  // input: the upload, or the normals stage's sorted points (finite only, the
+  // clouds in the same order) with the normals computed on them
+  const float4* in_pts = from_normals ? d_npts.as<float4>() : d_raw.as<float4>();
+  const float4* in_nrm = from_normals ? d_nnrm.as<float4>()
+                                      : (has_nrm ? d_raw_nrm.as<float4>() : nullptr);
+  const int* in_cloud = from_normals ? d_cloud_of_sorted.as<int>() : d_point_cloud.as<int>();
+  const int64_t n_in = from_normals ? total_pts : total_raw;
   if (!counts_zero) launch_fill_i32(d_counts.as<int>(), (int64_t)(d_counts.bytes / sizeof(int)), 0, s);
   counts_zero = false;
-  launch_cell_count(d_raw.as<float4>(), d_point_cloud.as<int>(), d_grids.as<GridDesc>(),
-                    total_raw, d_cid.as<int>(), d_counts.as<int>(), nullptr, s);
+  launch_cell_count(in_pts, in_cloud, d_grids.as<GridDesc>(), n_in, d_cid.as<int>(),
+                    d_counts.as<int>(), nullptr, s);
   launch_exclusive_scan_consume(d_counts.as<int>(), d_cell_start.as<int>(), cells,
                                 d_scan_tmp.as<int>(), s);
   d_slot_raw.reserve(npts * sizeof(int));
   d_slot_cell.reserve(npts * sizeof(int));
-  launch_place_points(d_raw.as<float4>(), has_nrm ? d_raw_nrm.as<float4>() : nullptr,
-                      d_cid.as<int>(), total_raw, total_pts, d_cell_start.as<int>(),
+  launch_place_points(in_pts, in_nrm, d_cid.as<int>(), n_in, total_pts, d_cell_start.as<int>(),
                       d_counts.as<int>(), d_slot_raw.as<int>(), d_slot_cell.as<int>(),
-                      d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), d_pts.as<float4>(),
-                      d_nrm.as<float4>(), s);
+                      d_cloud_off.as<int64_t>(), d_point_cloud.as<int>(), out_pts.as<float4>(),
+                      out_nrm.as<float4>(), from_normals, s);
+  if (from_normals) has_nrm = true;
   counts_zero = true;
   // sorted point -> cloud
   for (int c = 0; c < n; ++c) hpoff[c] = grids[c].pt_off;
@@ -400,7 +412,7 @@ void CloudSet::build(hipStream_t s, double factor, std::vector<Profiler::Span>* 
 }
 
 void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
-                               std::vector<Profiler::Span>* spans) {
+                               std::vector<Profiler::Span>* spans, bool normals_stage) {
   Profiler& prof = Profiler::get();
   // the previous pass's copy out of h_vp must be done before it is rewritten
   if (ev_vp)
@@ -422,10 +434,13 @@ void CloudSet::compute_normals(hipStream_t s, int k, const float* vp_host, int f
   d_nsel.reserve(np * sizeof(int));
   d_nmom.reserve(np * 9 * sizeof(double));
   d_nwork.reserve((size_t)normals_work_ints((int64_t)np) * sizeof(int));
-  launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), d_pts.as<float4>(),
+  DevBuf& pts = normals_stage ? d_npts : d_pts;
+  DevBuf& nrm = normals_stage ? d_nnrm : d_nrm;
+  nrm.reserve(np * sizeof(float4));
+  launch_normals(d_grids.as<GridDesc>(), d_cell_start.as<int>(), pts.as<float4>(),
                  d_cloud_of_sorted.as<int>(), total_pts, k, d_vp.as<float>(), flip_all,
                  d_raw.as<float4>(), d_cloud_off.as<int64_t>(), d_nbr.as<int>(), d_nmom.as<double>(),
-                 d_nsel.as<int>(), d_nwork.as<int>(), d_nrm.as<float4>(), s);
+                 d_nsel.as<int>(), d_nwork.as<int>(), nrm.as<float4>(), s);
   prof.end(span, s, spans);
   has_nrm = true;
 }

@@ -142,6 +142,7 @@ struct CloudSet {
   // issued so far (each build leaves it so; the geometry probe does not)
   bool counts_zero = false;
   DevBuf d_pts, d_nrm, d_cloud_of_sorted, d_vp, d_poff, d_nodes, d_nbr, d_nsel;
+  DevBuf d_npts, d_nnrm;  // the normals stage's sorted points and normals
   DevBuf d_tpts, d_tpos, d_tkeys, d_tkeys_sorted, d_tvals, d_ttmp;  // tree order
   DevBuf d_slot_raw, d_slot_cell, d_near, d_near_tmp;
   DevBuf d_nwork;  // normals: the query lists of the wavefront and rescan passes
@@ -156,10 +157,18 @@ struct CloudSet {
   // Builds every cloud's grid; cell edge = factor * estimated point spacing.
   // with_tree: also the tree order and the bounding-volume tree (the 1-NN searches
   // need them; the k-NN selections of normals and outlier removal do not)
-  void build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans, bool with_tree = true);
-  // K7 normals for every cloud into d_nrm (cell-sorted order).
+  // stage: kBuildRaw sorts the uploaded points into d_pts (d_nrm); kBuildNormals
+  // sorts them into d_npts for compute_normals(..., true); kBuildFromNormals sorts
+  // d_npts and the normals computed there (d_nnrm) into d_pts / d_nrm, so the
+  // second grid of a run starts from spatially ordered points and the normals
+  // need no trip back to input order.
+  static constexpr int kBuildRaw = 0, kBuildNormals = 1, kBuildFromNormals = 2;
+  void build(hipStream_t s, double factor, std::vector<Profiler::Span>* spans, bool with_tree = true,
+             int stage = kBuildRaw);
+  // K7 normals for every cloud into d_nrm (cell-sorted order); normals_stage: over
+  // d_npts into d_nnrm (after a kBuildNormals build).
   void compute_normals(hipStream_t s, int k, const float* vp_host, int flip_all,
-                       std::vector<Profiler::Span>* spans);
+                       std::vector<Profiler::Span>* spans, bool normals_stage = false);
   // Copies the cell-sorted normals back to input order (d_raw_nrm), so the next
   // build() carries them into its own order.
   void normals_to_raw(hipStream_t s);
