@@ -247,6 +247,11 @@ __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const 
     }
     bool has = false;
     float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
+    if constexpr (TILE == 64) {
+      // the tile has a point without a match: the final fitness pass orders by it
+      const uint64_t unmatched = __ballot(own && nn.j < 0);
+      if (b.fit_flags && threadIdx.x == 0) b.fit_flags[tile] = unmatched != 0 ? 1 : 0;
+    }
     if (own) {
       if (b.prev) b.prev[slot] = nn.j;
       if (nn.j >= 0 && (double)nn.d2 <= pd.max_corr_d2) {
@@ -370,6 +375,11 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
       const int j = s_j[threadIdx.x];
       bool has = false;
       float4 t = make_float4(0.f, 0.f, 0.f, 0.f), n = t;
+      if constexpr (TILE == 64) {
+        // as in k_icp_iter (the first wavefront holds the tile's 64 queries)
+        const uint64_t unmatched = __ballot(own && j < 0);
+        if (b.fit_flags && threadIdx.x == 0) b.fit_flags[tile] = unmatched != 0 ? 1 : 0;
+      }
       if (own) {
         if (b.prev) b.prev[(int64_t)pd.tile_begin * TILE + local] = j;
         if (j >= 0 && (double)d2 <= pd.max_corr_d2) {
@@ -593,11 +603,13 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
 // The final pass's list of tiles (fit_list) reordered into out: tiles with a
 // point that had no match in its instance's last iteration first (such a point
 // goes straight to the tree walk, tens of times the cost of a seeded one), the
-// others after, so that the longest tiles start at once.  Each block takes one
-// range of each part with one atomic on each of the two fill counters (ctr,
-// ctr + kFetchStride): heavy tiles fill out from the front, the rest from the
-// back.  The order within each part follows the blocks' atomics; it changes no
-// result (each tile writes its own partials).
+// others after, so that the longest tiles start at once.  The search kernels
+// flag each tile (fit_flags[tile]) in every iteration: the last one leaves the
+// flag the final pass's tile needs (both passes take a tile's 64 points in the
+// same order).  Each block takes one range of each part with one atomic on each
+// of the two fill counters (ctr, ctr + kFetchStride): heavy tiles fill out from
+// the front, the rest from the back.  The order within each part follows the blocks' atomics;
+// it changes no result (each tile writes its own partials).
 constexpr int kOrderThreads = 256;
 
 __global__ void __launch_bounds__(kOrderThreads) k_fit_order(const IcpBuffers b, int2* __restrict__ out,
@@ -612,16 +624,7 @@ __global__ void __launch_bounds__(kOrderThreads) k_fit_order(const IcpBuffers b,
   int2 e = make_int2(0, 0);
   if (valid) {
     e = b.fit_list[i];
-    const PairDesc pd = b.pairs[e.y];
-    const GridDesc gs = b.grids[pd.src];
-    const int first = (e.x - pd.tile_begin) * 64;
-    const int n = min(64, gs.npts - first);
-    const int* pv = b.prev + (int64_t)pd.tile_begin * 64;
-    // the tile's points in the cloud's tree order, as the final pass takes them
-    for (int j = 0; j < n && !heavy; ++j) {
-      const int local = b.tpts ? b.tpos[gs.pt_off + first + j] - (int)gs.pt_off : first + j;
-      heavy = pv[local] < 0;
-    }
+    heavy = b.fit_flags[e.x] != 0;
   }
   const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
   const uint64_t hm = __ballot(valid && heavy), lm = __ballot(valid && !heavy);
@@ -987,7 +990,8 @@ void launch_fitness(const IcpBuffers& b, double max_range, int nblocks, hipStrea
   const int pi = per_instance ? 1 : 0;
   switch (b.tile) {
     case 64:
-      if (!per_instance && b.fetch && b.fit_order && b.fit_count && b.prev && b.ntiles > 0) {
+      if (!per_instance && b.fetch && b.fit_order && b.fit_flags && b.fit_count && b.prev &&
+          b.ntiles > 0) {
         // the final pass over fit_list, tree-walk tiles first
         hipLaunchKernelGGL(k_fit_order, dim3((b.ntiles + kOrderThreads - 1) / kOrderThreads),
                            dim3(kOrderThreads), 0, s, b, b.fit_order,

@@ -93,8 +93,9 @@ struct IcpBuffers {
   // follows it
   int* fetch;
   // room for one list like fit_list: the final pass's tiles reordered with the
-  // tree-walk tiles first (nullable: fit_list's order)
+  // tree-walk tiles first (nullable: fit_list's order), and one flag per entry
   int2* fit_order;
+  int* fit_flags;
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)

@@ -539,6 +539,7 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
   d_it_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
   d_fit_list.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
   d_fit_order.reserve((size_t)std::max(ntiles, 1) * sizeof(int2));
+  d_fit_flags.reserve((size_t)std::max(ntiles, 1) * sizeof(int));
   d_list_count.reserve(2 * sizeof(int));
   d_list_off.reserve((size_t)std::max(npairs, 1) * sizeof(int));
   if (!d_fetch.p) {
@@ -611,6 +612,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.fit_count = d_list_count.as<int>() + 1;
   b.fetch = fitness_fixed_shares() ? nullptr : d_fetch.as<int>();
   b.fit_order = b.fetch ? d_fit_order.as<int2>() : nullptr;
+  b.fit_flags = d_fit_flags.as<int>();
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;

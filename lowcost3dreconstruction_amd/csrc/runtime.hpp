@@ -201,6 +201,7 @@ struct Engine {
   DevBuf d_it_list, d_fit_list, d_list_count, d_list_off;  // live-tile lists (IcpBuffers::it_list)
   DevBuf d_fetch;      // IcpBuffers::fetch
   DevBuf d_fit_order;  // IcpBuffers::fit_order
+  DevBuf d_fit_flags;  // IcpBuffers::fit_flags
   int ngate_tiles = 0;
   HostBuf h_state, h_guess;
   static constexpr int kChunk = 4;  // iterations per captured chunk
