@@ -358,10 +358,10 @@ def main():
     kernels = [
         # per instance and executed iteration: 12 B per source point + 24 B per target
         # point (xyz + normal)
-        roofline_entry("icp_search", "k_icp_iter", float(np.sum(it * (12 * ns + 24 * nt)))
+        roofline_entry("icp_search", "k_icp_iter + k_icp_iter_split", float(np.sum(it * (12 * ns + 24 * nt)))
                        / max(n_it / steps, 1), n_it, it_ms, steps, traffic),
         # per refined instance: 12 B per source point + 12 B per target point
-        roofline_entry("fitness", "k_fitness + k_fitness_fold", float(np.sum((12 * ns + 12 * nt)[refined]))
+        roofline_entry("fitness", "k_fit_order + k_fitness + k_fitness_fold", float(np.sum((12 * ns + 12 * nt)[refined]))
                        / max(n_fit / steps, 1), n_fit, fit_ms, steps, traffic),
         # per cloud: 12 B read + 24 B grid + 16 B normal and curvature written
         roofline_entry("normals", "k_normal_lane + k_normal_wave + k_normal_rescan + k_normal_solve",
