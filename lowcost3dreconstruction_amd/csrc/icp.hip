@@ -203,13 +203,12 @@ __global__ void __launch_bounds__(TILE, 7) k_icp_iter(const IcpBuffers b, const 
     const bool own = rank < gs.npts;
     const float r = (float)sqrt(pd.max_corr_d2);
     const bool bounded = isfinite(r);
-    const bool morton = c.morton && b.tpts;
-    // the point's cell-order position indexes its previous match
-    const int local = (own && morton) ? b.tpos[gs.pt_off + rank] - (int)gs.pt_off : rank;
-    const int64_t slot = (int64_t)pd.tile_begin * TILE + local;
+    // the previous match of the tile's lane (the final fitness pass takes the
+    // same points in the same lanes)
+    const int64_t slot = (int64_t)tile * TILE + threadIdx.x;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
-    if (own) xform(st->T, morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank], qx, qy, qz);
+    if (own) xform(st->T, b.morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank], qx, qy, qz);
     // no target point within r of the query's cell: no correspondence, no search
     const bool empty = own && bounded && b.near && gt.near_r >= r &&
                        !near_cell(gt, b.near, qx, qy, qz);
@@ -341,23 +340,17 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
     const int base = (tile - pd.tile_begin) * TILE;
     // every radius is finite here (IcpConst::ring_only)
     const float r = (float)sqrt(pd.max_corr_d2);
-    const bool morton = c.morton && b.tpts;
-    // a query's point (in tree or cell order) and its cell-order position, which
-    // indexes its previous match
-    auto point_of = [&](int rank, int& local) {
-      local = morton ? b.tpos[gs.pt_off + rank] - (int)gs.pt_off : rank;
-      return morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank];
-    };
+    // a query's point, in tree or cell order
+    auto point_of = [&](int rank) { return b.morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank]; };
     for (int pass = 0; pass < TILE / 64; ++pass) {
       const int qi = pass * 64 + q;
       const bool own = base + qi < gs.npts;
       float qx = 0.f, qy = 0.f, qz = 0.f;
-      int local = 0;
       NNResult nn{INFINITY, 0x7fffffff, -1, false};
-      if (own) xform(st->T, point_of(base + qi, local), qx, qy, qz);
+      if (own) xform(st->T, point_of(base + qi), qx, qy, qz);
       const bool empty = own && b.near && gt.near_r >= r && !near_cell(gt, b.near, qx, qy, qz);
       if (own && !empty) {
-        const int seed = b.prev ? b.prev[(int64_t)pd.tile_begin * TILE + local] : -1;
+        const int seed = b.prev ? b.prev[(int64_t)tile * TILE + qi] : -1;
         nn = nn_ring_split<S>(gt, b.cell_start + gt.cell_off, b.pts, qx, qy, qz, r, seed, sub);
       }
       if (sub == 0) {
@@ -369,8 +362,7 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
     if (threadIdx.x < TILE) {
       const bool own = base + (int)threadIdx.x < gs.npts;
       float qx = 0.f, qy = 0.f, qz = 0.f;
-      int local = 0;
-      if (own) xform(st->T, point_of(base + (int)threadIdx.x, local), qx, qy, qz);
+      if (own) xform(st->T, point_of(base + (int)threadIdx.x), qx, qy, qz);
       const float d2 = s_d2[threadIdx.x];
       const int j = s_j[threadIdx.x];
       bool has = false;
@@ -381,7 +373,7 @@ __global__ void __launch_bounds__(64 * kSplitLanes) k_icp_iter_split(const IcpBu
         if (b.fit_flags && threadIdx.x == 0) b.fit_flags[tile] = unmatched != 0 ? 1 : 0;
       }
       if (own) {
-        if (b.prev) b.prev[(int64_t)pd.tile_begin * TILE + local] = j;
+        if (b.prev) b.prev[(int64_t)tile * TILE + threadIdx.x] = j;
         if (j >= 0 && (double)d2 <= pd.max_corr_d2) {
           has = true;
           t = b.pts[j];
@@ -541,21 +533,22 @@ __global__ void __launch_bounds__(TILE, 6) k_fitness(const IcpBuffers b, double 
     const float rq = bounded ? (float)sqrt(max_range) : INFINITY;
     const float r = rq * 1.00001f;
     // the point's rank in its instance's source: in cell order for the gate, in
-    // the cloud's tree (Morton) order for the final pass, whose tiles are then
-    // compact patches (a tile of cell order can run along a whole row of cells,
-    // 20 cm on cycle36, and its far points' tree walks then share little)
+    // the cloud's tree (Morton) order for the final pass (as in the iterations),
+    // whose tiles are then compact patches (a tile of cell order can run along a
+    // whole row of cells, 20 cm on cycle36, and its far points' tree walks then
+    // share little)
     const int rank = (tile - pd.tile_begin) * TILE + (int)threadIdx.x;
-    const bool morton = !per_instance && b.tpts;
+    const bool morton = !per_instance && b.morton;
     double v[2] = {0.0, 0.0};
     const bool own = rank < gs.npts;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     NNResult nn{INFINITY, 0x7fffffff, -1, false};
     if (own) {
-      // local: the point's cell-order position, which indexes its seed
-      const int local = morton ? b.tpos[gs.pt_off + rank] - (int)gs.pt_off : rank;
       xform(st->T, morton ? b.tpts[gs.pt_off + rank] : b.pts[gs.pt_off + rank], qx, qy, qz);
       const bool empty = bounded && b.near && gt.near_r >= rq && !near_cell(gt, b.near, qx, qy, qz);
-      const int seed = b.prev ? b.prev[(int64_t)pd.tile_begin * TILE + local] : -1;
+      // the lane's previous match: the iterations take the same points in the same
+      // lanes as the final pass (the gate's pass runs before any match exists)
+      const int seed = b.prev ? b.prev[(int64_t)tile * TILE + threadIdx.x] : -1;
       // the final pass over refined instances: a point without a match within r in
       // the last iteration (seed -1) is farther than about r from the target; a
       // ring search would scan two unpruned rings before handing it to the tree,

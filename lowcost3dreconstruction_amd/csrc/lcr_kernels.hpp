@@ -99,6 +99,8 @@ struct IcpBuffers {
   int ntiles;
   int npairs;
   int tile;               // source points per tile (= threads per block)
+  int morton;             // 1: the iterations and the final fitness pass take a tile's
+                          // points in the source's tree order (tpts), 0: in cell order
   __host__ __device__ Tree tree() const { return Tree{nodes, tpts, tpos}; }
 };
 

@@ -616,6 +616,7 @@ IcpBuffers Engine::buffers(const CloudSet& cs) const {
   b.ntiles = ntiles;
   b.npairs = npairs;
   b.tile = tile;
+  b.morton = (c.morton && cs.d_tpts.p) ? 1 : 0;
   return b;
 }
 
