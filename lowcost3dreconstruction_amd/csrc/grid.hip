@@ -522,7 +522,11 @@ void launch_near_mask(const GridDesc* grids, int ngrids, const int* cell_start, 
   // per grid a row of blocks, grid-stride over its words
   const unsigned bx = (unsigned)std::min<int64_t>(blocks_for(total_words / ngrids + 1, 256), 256);
   const dim3 grid(bx, (unsigned)ngrids);
-  hipLaunchKernelGGL(k_near_occ, grid, dim3(256), 0, s, grids, cell_start, tmp);
+  // occupancy: one wavefront item per 64 cells (two words), each a round trip to
+  // the cell table; enough blocks that a wavefront takes a few items (with the
+  // words' grid each took ~32 in turn: 74 us per cycle36 build)
+  const unsigned bo = (unsigned)std::min<int64_t>(blocks_for(total_words / ngrids / 2 + 1, 4), 1024);
+  hipLaunchKernelGGL(k_near_occ, dim3(bo, (unsigned)ngrids), dim3(256), 0, s, grids, cell_start, tmp);
   hipLaunchKernelGGL(k_near_x, grid, dim3(256), 0, s, grids, (const uint32_t*)tmp, out, r);
   hipLaunchKernelGGL(k_near_yz, grid, dim3(256), 0, s, grids, (const uint32_t*)out, tmp, 1, r);
   hipLaunchKernelGGL(k_near_yz, grid, dim3(256), 0, s, grids, (const uint32_t*)tmp, out, 2, r);
