@@ -579,13 +579,10 @@ void Engine::setup(const CloudSet& cs, const std::vector<int>& src, const std::v
 }
 
 // LCR_FIT_ORDER=fixed: the fitness passes in fixed per-block shares of the tiles
-// instead of tiles handed out by counters (read once).
+// instead of tiles handed out by counters (read at every run).
 static bool fitness_fixed_shares() {
-  static const bool v = [] {
-    const char* e = getenv("LCR_FIT_ORDER");
-    return e && strcmp(e, "fixed") == 0;
-  }();
-  return v;
+  const char* e = getenv("LCR_FIT_ORDER");
+  return e && strcmp(e, "fixed") == 0;
 }
 
 IcpBuffers Engine::buffers(const CloudSet& cs) const {

@@ -296,6 +296,33 @@ def test_split_search_matches_one_lane(lcr, monkeypatch, p2pl, tile):
         np.testing.assert_array_equal(r["fitness"], res[0]["fitness"])
 
 
+@pytest.mark.parametrize("p2pl", [False, True])
+def test_fitness_tile_order_does_not_change_results(lcr, monkeypatch, p2pl):
+    """The fitness passes' tiles handed out by counters, the tree-walk tiles first,
+    give bitwise the poses, iteration counts and fitness of fixed per-block shares:
+    every tile writes its own partial sums and the fold adds them in tile order."""
+    cyc = synth.turntable_cycle(6, n_captures=5, yaw_step_deg=10.0, scale=1.5,
+                                centre=(0.0, 0.0, -0.8))
+    res = []
+    for order in ("fixed", None):
+        print_case(test="fitness_order", p2pl=p2pl, order=order)
+        if order is None:
+            monkeypatch.delenv("LCR_FIT_ORDER", raising=False)
+        else:
+            monkeypatch.setenv("LCR_FIT_ORDER", order)
+        c = lcr.Cycle(cyc.clouds, cyc.pairs, guesses=cyc.meta["prior"], point_to_plane=p2pl,
+                      max_corr_dist=0.01, max_iterations=60, tile=64)
+        c.run()
+        res.append(c.results())
+        c.run()  # graph replay
+        np.testing.assert_array_equal(c.results()["fitness"], res[-1]["fitness"])
+        del c
+    np.testing.assert_array_equal(res[1]["T"], res[0]["T"])
+    np.testing.assert_array_equal(res[1]["iterations"], res[0]["iterations"])
+    np.testing.assert_array_equal(res[1]["fitness"], res[0]["fitness"])
+    assert np.all(np.isfinite(res[0]["fitness"]))
+
+
 def test_distance_tiles_cycle_matches_exact(lcr):
     cyc = synth.turntable_cycle(4, n_captures=4, yaw_step_deg=10.0, scale=1.5,
                                 centre=(0.0, 0.0, -0.8))
