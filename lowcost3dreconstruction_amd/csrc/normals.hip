@@ -673,21 +673,56 @@ __device__ __forceinline__ float4 load_point(__amdgpu_buffer_rsrc_t r, int i) {
   return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, i * 16, 0, 0));
 }
 
-// Calls f(p, i, valid) for the points i of the block's runs, NB loads in
+// Calls f(p, i, valid) for the points i of the block's runs, NBATCH loads in
 // flight; the loads of a run's last batch read on past its end (the next points,
-// or zeros past the buffer) with valid = false.
+// or zeros past the buffer) with valid = false.  The runs go column by column:
+// the 2K+1 runs of one y offset (z = -K .. K) back to back, each lane moving to
+// its next non-empty run as soon as it finishes one.  A loop over one run at a
+// time lasts as long as the wavefront's longest lane in every run; per column,
+// the lanes' long and short runs even out (about 76 against 101 batches per
+// wavefront on the 36-capture cycle's queries).
 template <int K, int NBATCH, class F>
 __device__ __forceinline__ void for_block(const BlockRuns<K>& b, __amdgpu_buffer_rsrc_t rs,
                                           F&& f) {
+  constexpr int S = 2 * K + 1;
 #pragma unroll
-  for (int r = 0; r < BlockRuns<K>::R; ++r) {
-    const int s = b.s[r], e = s + b.n[r];
-    for (int i = s; i < e; i += NBATCH) {
+  for (int dy = 0; dy < S; ++dy) {
+    // the column's runs with the empty ones moved to the end, order kept
+    int qs[S], qe[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      qs[t] = b.s[t * S + dy];
+      qe[t] = qs[t] + b.n[t * S + dy];
+    }
+#pragma unroll
+    for (int pass = 0; pass + 1 < S; ++pass)
+#pragma unroll
+      for (int t = 0; t + 1 < S; ++t) {
+        const bool gap = qs[t] >= qe[t];
+        qs[t] = gap ? qs[t + 1] : qs[t];
+        qe[t] = gap ? qe[t + 1] : qe[t];
+        qs[t + 1] = gap ? 0 : qs[t + 1];
+        qe[t + 1] = gap ? 0 : qe[t + 1];
+      }
+    int i = qs[0], e = qe[0];
+    while (i < e) {
       float4 p[NBATCH];
 #pragma unroll
       for (int u = 0; u < NBATCH; ++u) p[u] = load_point(rs, i + u);
 #pragma unroll
       for (int u = 0; u < NBATCH; ++u) f(p[u], i + u, i + u < e);
+      i += NBATCH;
+      if (i >= e) {
+        i = qs[1];
+        e = qe[1];
+#pragma unroll
+        for (int t = 1; t + 1 < S; ++t) {
+          qs[t] = qs[t + 1];
+          qe[t] = qe[t + 1];
+        }
+        qs[S - 1] = 0;
+        qe[S - 1] = 0;
+      }
     }
   }
 }
@@ -801,13 +836,21 @@ __device__ __forceinline__ int lane_query(const GridDesc* __restrict__ grids,
   }
   if (B < 0) return K + 1;  // not certified
   if (nB > NL) return K;
+  // Pass 2 takes points of buckets up to B only: floor(d2 * scale) <= B < NB - 1
+  // means d2 < (B + 1) D2 / NB, to within the rounding of d2 * scale (1e-4 covers
+  // it).  The rows trimmed to that bound (ball_row, monotone in the bound) keep
+  // every such point, and their runs are sub-runs of pass 1's, so the points
+  // pass 2 takes and the order it adds them in are unchanged.
+  const BlockRuns<K> sel =
+      B < NB - 1 ? block_runs<K>(g, cs, cp, fminf(D2, D2 * ((float)(B + 1) / (float)NB) * 1.0001f))
+                 : blk;
   // the moments of the lower buckets' points in block order, then of bucket B's
   // picks in (d2, original index) order
   Moments acc;
 #pragma unroll
   for (int c = 0; c < 9; ++c) acc.v[c] = 0.0;
   int nl = 0;
-  for_block<K, 4>(blk, rs, [&](const float4& p, int i, bool valid) {
+  for_block<K, 4>(sel, rs, [&](const float4& p, int i, bool valid) {
     const float d = flann_d2(q.x, q.y, q.z, p);
     if (valid && d < D2) {
       const int b = min((int)(d * scale), NB - 1);
