@@ -1132,6 +1132,14 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
         K = max(K + 1, (int)ceilf(fminf(need, (float)(kWaveMaxK + 1))));
         continue;
       }
+      // The passes below take points of keys up to bucket B only: floor(d2 *
+      // scale) <= B < kWaveBuckets - 1 means d2 < (B + 1) D2 / kWaveBuckets, to
+      // within the rounding of d2 * scale (1e-4 covers it), so they walk the rows
+      // trimmed to that bound (ball_row keeps every cell that may hold such a
+      // point).
+      const float D2b = B < kWaveBuckets - 1
+                            ? fminf(D2, D2 * ((float)(B + 1) / (float)kWaveBuckets) * 1.0001f)
+                            : D2;
       // level 1 inside bucket B when it holds more than 64 points
       int SB = 0;
       const bool split = nB > 64;
@@ -1140,7 +1148,7 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
 #pragma unroll
         for (int c = lane; c < kWaveBuckets; c += 64) L.hist[c] = 0;
         wave_lds_sync();
-        for_wave_block(L, g, cs, cp, K, D2, rs, [&](bool valid, const float4& p, int) {
+        for_wave_block(L, g, cs, cp, K, D2b, rs, [&](bool valid, const float4& p, int) {
           if (!valid) return;
           const float d = flann_d2(q.x, q.y, q.z, p);
           if (!(d < D2)) return;
@@ -1152,12 +1160,19 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
         wave_find_bucket(L, kk, below, SB, nB);
         if (nB > 64) break;  // more than 64 equal keys: the rescan pass
       }
-      // pass 2: the moments of the lower keys, the kk-th's key kept in LDS
+      // pass 2: the moments of the lower keys, the kk-th's key kept in LDS; with
+      // the split, keys up to (B, SB) lie below (B + (SB + 1) / 256) D2 /
+      // kWaveBuckets (x - B and its product by 256 are exact)
+      const float D2s =
+          split && B < kWaveBuckets - 1
+              ? fminf(D2b, D2 * (((float)B + (float)(SB + 1) * (1.0f / 256.0f)) / (float)kWaveBuckets) *
+                               1.0001f)
+              : D2b;
       Moments acc;
 #pragma unroll
       for (int c = 0; c < 9; ++c) acc.v[c] = 0.0;
       int nl = 0;
-      for_wave_block(L, g, cs, cp, K, D2, rs, [&](bool valid, const float4& p, int i) {
+      for_wave_block(L, g, cs, cp, K, D2s, rs, [&](bool valid, const float4& p, int i) {
         const float d = valid ? flann_d2(q.x, q.y, q.z, p) : INFINITY;
         const float x = d * scale;
         const int b = d < D2 ? min((int)x, kWaveBuckets - 1) : kWaveBuckets;
