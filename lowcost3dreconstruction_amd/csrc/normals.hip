@@ -615,6 +615,9 @@ __device__ __forceinline__ bool select_staged(const GridDesc& g, const int* cs, 
 // block holds too few; dense spots overflow bucket B) to a list, which the
 // wavefront pass below serves.
 constexpr int kLaneMaxCand = 4096;
+// The lane pass's histogram buckets (the wavefront pass reads its bucket bounds
+// in units of D2 / kLaneBuckets).
+constexpr int kLaneBuckets = 32;
 
 constexpr int kLaneThreads = 256;
 
@@ -782,7 +785,9 @@ constexpr int kNselOrig = 1 << 30;
 // the FP64 moments of the selected neighbours relative to the query (mom, nine
 // arrays of n) and nsel, and returns 0; or leaves the query to the next path
 // (nothing written) and returns the block radius that path starts from: K + 1
-// when this block held too few points or no certificate, K otherwise.
+// when this block held too few points or no certificate, K otherwise; when
+// bucket B overflowed, B + 1 in bits 8 and up (at least kk points lie below
+// (B + 1) D2 / NB, up to the rounding of d2 * scale).
 template <int K, int NB, int NL>
 __device__ __forceinline__ int lane_query(const GridDesc* __restrict__ grids,
                                           const int* __restrict__ cell_start,
@@ -835,7 +840,7 @@ __device__ __forceinline__ int lane_query(const GridDesc* __restrict__ grids,
     cum += v;
   }
   if (B < 0) return K + 1;  // not certified
-  if (nB > NL) return K;
+  if (nB > NL) return K | ((B + 1) << 8);
   // Pass 2 takes points of buckets up to B only: floor(d2 * scale) <= B < NB - 1
   // means d2 < (B + 1) D2 / NB, to within the rounding of d2 * scale (1e-4 covers
   // it).  The rows trimmed to that bound (ball_row, monotone in the bound) keep
@@ -906,6 +911,7 @@ k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
               const float4* __restrict__ pts, const int* __restrict__ cloud_of_sorted,
               int64_t total, int k, double* __restrict__ mom, int* __restrict__ nsel,
               int2* __restrict__ list, int* __restrict__ count) {
+  static_assert(NB == kLaneBuckets, "the wavefront pass reads bucket bounds of kLaneBuckets");
   __shared__ uint32_t s_buf[kLaneThreads / 64][lane_words<NB, NL>()][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kLaneThreads + threadIdx.x;
@@ -930,7 +936,11 @@ k_normal_lane(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
 // of the block's points bound its distance from above (the kk-th smallest of the
 // per-lane minima, distinct points, or the largest d2 when fewer lanes hold one),
 // and the next block is the smallest whose certified range holds that bound, so
-// it certifies.  A prefix over the buckets gives bucket B of the kk-th; more
+// it certifies.  That bound (or, for a query whose lane-pass bucket overflowed,
+// the bound of that bucket) then narrows the histogram's range: at least kk
+// points lie below it, so counting d2 < R2 = min(D2, bound) over the rows
+// trimmed to R2 finds the same kk nearest with fewer candidates and finer
+// buckets.  A prefix over the buckets gives bucket B of the kk-th; more
 // than 64 points in B split it into 256 sub-buckets with one more pass (the
 // bucket key floor(256 * d2 * scale) at that level, still monotone in d2).
 // Pass 2 adds the points of lower keys to the lane's FP64 moments (each lane its
@@ -1092,13 +1102,22 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
     const int* cs = cell_start + g.cell_off;
     const CellPos cp = cell_pos(g, q.x, q.y, q.z);
     bool done = false;
-    int K = __builtin_amdgcn_readfirstlane(e.y);
+    const int code = __builtin_amdgcn_readfirstlane(e.y);
+    int K = code & 0xff;
+    // a d2 that at least kk points lie below (INFINITY: none known yet)
+    float U = INFINITY;
     for (int attempt = 0; attempt < 8 && !done && K <= kWaveMaxK; ++attempt) {
       const float Lc = block_gap(g, cp, K);
       if (Lc == INFINITY) break;  // covers the grid: the rescan pass
       const float Lm = (Lc - cp.slack) * g.h;
       const float D2 = Lm * Lm * 0.99998f;
-      const float scale = (float)kWaveBuckets / D2;
+      // the lane pass's bucket bound (same block, same D2), widened for rounding
+      if (attempt == 0 && (code >> 8) > 0)
+        U = D2 * ((float)(code >> 8) / (float)kLaneBuckets) * 1.0001f;
+      // The passes count d2 < R2: the certified range, or U below it.  The kk
+      // nearest then lie below R2, and the rows trimmed to R2 hold them.
+      const float R2 = (U > 1e-30f && U < D2) ? U : D2;
+      const float scale = (float)kWaveBuckets / R2;
       if (!(Lm > 0.0f && D2 > 1e-30f && scale < INFINITY)) {
         ++K;
         continue;
@@ -1106,16 +1125,16 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
       wave_lds_sync();
 #pragma unroll
       for (int c = lane; c < kWaveBuckets; c += 64) L.hist[c] = 0;
-      // pass 1: the histogram of d2 < D2, and the lane's count, least and largest d2
+      // pass 1: the histogram of d2 < R2, and the lane's count, least and largest d2
       int mc = 0;
       float dmin = INFINITY, dmax = 0.0f;
-      for_wave_block(L, g, cs, cp, K, D2, rs, [&](bool valid, const float4& p, int) {
+      for_wave_block(L, g, cs, cp, K, R2, rs, [&](bool valid, const float4& p, int) {
         if (!valid) return;
         const float d = flann_d2(q.x, q.y, q.z, p);
         ++mc;
         dmin = fminf(dmin, d);
         dmax = fmaxf(dmax, d);
-        if (d < D2) atomicAdd(&L.hist[min((int)(d * scale), kWaveBuckets - 1)], 1);
+        if (d < R2) atomicAdd(&L.hist[min((int)(d * scale), kWaveBuckets - 1)], 1);
       });
       wave_lds_sync();
       if (wave_sum_i(mc) < kk) {
@@ -1126,20 +1145,21 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
       if (!wave_find_bucket(L, kk, below, B, nB)) {
         // not certified: a block whose certified range holds an upper bound U of
         // the kk-th d2 certifies (its gap is at least its radius in cells)
-        const float U = (kk <= 64 && popc_ballot(mc > 0) >= kk) ? wave_kth_smallest(dmin, kk)
-                                                                 : wave_max_f(dmax);
-        const float need = sqrtf(U) * g.inv_h * 1.0001f + cp.slack + 1e-3f;
+        const float Ub = (kk <= 64 && popc_ballot(mc > 0) >= kk) ? wave_kth_smallest(dmin, kk)
+                                                                  : wave_max_f(dmax);
+        const float need = sqrtf(Ub) * g.inv_h * 1.0001f + cp.slack + 1e-3f;
         K = max(K + 1, (int)ceilf(fminf(need, (float)(kWaveMaxK + 1))));
+        U = fminf(U, Ub * 1.0001f);  // kk points of this block lie at or below Ub
         continue;
       }
       // The passes below take points of keys up to bucket B only: floor(d2 *
-      // scale) <= B < kWaveBuckets - 1 means d2 < (B + 1) D2 / kWaveBuckets, to
+      // scale) <= B < kWaveBuckets - 1 means d2 < (B + 1) R2 / kWaveBuckets, to
       // within the rounding of d2 * scale (1e-4 covers it), so they walk the rows
       // trimmed to that bound (ball_row keeps every cell that may hold such a
       // point).
       const float D2b = B < kWaveBuckets - 1
-                            ? fminf(D2, D2 * ((float)(B + 1) / (float)kWaveBuckets) * 1.0001f)
-                            : D2;
+                            ? fminf(R2, R2 * ((float)(B + 1) / (float)kWaveBuckets) * 1.0001f)
+                            : R2;
       // level 1 inside bucket B when it holds more than 64 points
       int SB = 0;
       const bool split = nB > 64;
@@ -1151,7 +1171,7 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
         for_wave_block(L, g, cs, cp, K, D2b, rs, [&](bool valid, const float4& p, int) {
           if (!valid) return;
           const float d = flann_d2(q.x, q.y, q.z, p);
-          if (!(d < D2)) return;
+          if (!(d < R2)) return;
           const float x = d * scale;
           if (min((int)x, kWaveBuckets - 1) != B) return;
           atomicAdd(&L.hist[min((int)((x - (float)B) * 256.0f), 255)], 1);
@@ -1161,11 +1181,11 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
         if (nB > 64) break;  // more than 64 equal keys: the rescan pass
       }
       // pass 2: the moments of the lower keys, the kk-th's key kept in LDS; with
-      // the split, keys up to (B, SB) lie below (B + (SB + 1) / 256) D2 /
+      // the split, keys up to (B, SB) lie below (B + (SB + 1) / 256) R2 /
       // kWaveBuckets (x - B and its product by 256 are exact)
       const float D2s =
           split && B < kWaveBuckets - 1
-              ? fminf(D2b, D2 * (((float)B + (float)(SB + 1) * (1.0f / 256.0f)) / (float)kWaveBuckets) *
+              ? fminf(D2b, R2 * (((float)B + (float)(SB + 1) * (1.0f / 256.0f)) / (float)kWaveBuckets) *
                                1.0001f)
               : D2b;
       Moments acc;
@@ -1175,7 +1195,7 @@ k_normal_wave(const GridDesc* __restrict__ grids, const int* __restrict__ cell_s
       for_wave_block(L, g, cs, cp, K, D2s, rs, [&](bool valid, const float4& p, int i) {
         const float d = valid ? flann_d2(q.x, q.y, q.z, p) : INFINITY;
         const float x = d * scale;
-        const int b = d < D2 ? min((int)x, kWaveBuckets - 1) : kWaveBuckets;
+        const int b = d < R2 ? min((int)x, kWaveBuckets - 1) : kWaveBuckets;
         const int sb = (split && b == B) ? min((int)((x - (float)B) * 256.0f), 255) : 0;
         const bool lt = b < B || (b == B && sb < SB);
         const bool eq = b == B && sb == SB;
@@ -1406,7 +1426,7 @@ void launch_normals(const GridDesc* grids, const int* cell_start, const float4* 
   int* deferred = work + 1;
   int2* list = reinterpret_cast<int2*>(work + 2 + n + (n & 1));
   launch_fill_i32(work, 2, 0, s);
-  hipLaunchKernelGGL((k_normal_lane<1, 32, 8>), dim3((unsigned)((n + kLaneThreads - 1) / kLaneThreads)),
+  hipLaunchKernelGGL((k_normal_lane<1, kLaneBuckets, 8>), dim3((unsigned)((n + kLaneThreads - 1) / kLaneThreads)),
                      dim3(kLaneThreads), 0, s, grids, cell_start, pts, cloud_of_sorted, n, k, mom,
                      nsel, list, count);
   // persistent over the list (a few percent of the points): at most eight blocks
