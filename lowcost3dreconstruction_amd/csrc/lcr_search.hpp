@@ -364,6 +364,34 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   // and the rings reach it within a few steps, so the tree is never needed
   const bool ring_only = bounded && rlim * g.inv_h <= kRingOnlyCells;
   const float lim2 = bounded ? rlim * rlim : INFINITY;
+  // A seed closer than about a cell: every point at least as close (within the
+  // limit) lies in a cell that the ball of that radius around q touches, at most
+  // 2 x 2 x 2 cells, so their rows are scanned instead of the rings.  The rows
+  // outside the ball's z and y range are those a ring walk prunes (their gap
+  // exceeds the ball's radius, slack included); inside a row scan_ring_row prunes
+  // and trims by the current best as the rings do; the x range of rings 0 and 1
+  // holds the ball (radius below one cell).  The result is the same point.
+  if (seed_j >= 0) {
+    const float thr = fminf(best, lim2) * 1.00002f;
+    const float rho = sqrtf(thr) * g.inv_h * 1.00001f + c.slack;
+    if (rho < 1.0f) {
+      const int za = max((int)floorf(c.fz - rho), 0), zb = min((int)floorf(c.fz + rho), g.nz - 1);
+      const int ya = max((int)floorf(c.fy - rho), 0), yb = min((int)floorf(c.fy + rho), g.ny - 1);
+      const int x0 = max(c.cx - 1, 0), x1 = min(c.cx + 1, g.nx - 1);
+      if (x0 <= x1) {
+        for (int z = za; z <= zb; ++z) {
+          const float gz = axis_gap(c.fz, z, c.cz, c.slack);
+          for (int y = ya; y <= yb; ++y)
+            scan_ring_row(g, cs, P, c, 1, z, y, gz, true, x0, x1, h2, lim2, qx, qy, qz, best, bid,
+                          bj);
+        }
+      }
+      res.d2 = best;
+      res.id = bid;
+      res.j = bj;
+      return res;
+    }
+  }
   if (nodes && !ring_only && k0 > ring_limit) {  // far outside the grid: straight to the tree
     if (defer_tree)
       res.tree = true;

@@ -194,7 +194,14 @@ def test_grid_ring_search_equals_bruteforce(m, h, r):
     seeds_rand = rng.integers(-1, len(tgt), size=len(q))
     seeds_rand[seeds_rand == 5] = -1  # the NaN point is not in the grid
     seeds_true = np.where(ridx >= 0, ridx, -1)
-    for seeds in (None, seeds_rand, seeds_true):
+    # a near miss: one of the 2nd-4th nearest targets (a seed inside a cell or so
+    # whose ball holds the answer, so the search must still find the closer point)
+    d = np.linalg.norm(q[:, None, :].astype(np.float64) - tgt[None, :, :], axis=2)
+    d[~np.isfinite(d)] = np.inf
+    order = np.argsort(d, axis=1)[:, 1:4]
+    seeds_near = order[np.arange(len(q)), rng.integers(0, 3, size=len(q))].astype(np.int64)
+    seeds_near[~np.isfinite(q).all(axis=1)] = -1
+    for seeds in (None, seeds_rand, seeds_true, seeds_near):
         # ring search alone, tree after 2 rings (the device default), tree at once
         for ring_limit in (-1, 2, 0):
             idx, d2 = _nn_grid(m, tgt, q, h, r if bounded else np.inf, bounded, seeds, ring_limit)
