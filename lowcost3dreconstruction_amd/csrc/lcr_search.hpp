@@ -305,6 +305,10 @@ LCR_SHD void ring_rows(const GridDesc& g, const int* __restrict__ cs, const floa
   }
 }
 
+// Ball radius (in cells) up to which a seeded query scans its ball's rows instead
+// of the rings.
+constexpr float kSeedBallCells = 2.0f;
+
 // Whether a bounded search (limit rlim) may stop after ring k with `best`: the
 // grid is exhausted, no unvisited cell can hold a point closer than best, or
 // none is within rlim.
@@ -364,20 +368,23 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   // and the rings reach it within a few steps, so the tree is never needed
   const bool ring_only = bounded && rlim * g.inv_h <= kRingOnlyCells;
   const float lim2 = bounded ? rlim * rlim : INFINITY;
-  // A seed closer than about a cell: every point at least as close (within the
-  // limit) lies in a cell that the ball of that radius around q touches, at most
-  // 2 x 2 x 2 cells, so their rows are scanned instead of the rings.  The rows
-  // outside the ball's z and y range are those a ring walk prunes (their gap
-  // exceeds the ball's radius, slack included); inside a row scan_ring_row prunes
-  // and trims by the current best as the rings do; the x range of rings 0 and 1
-  // holds the ball (radius below one cell).  The result is the same point.
+  // A seed within kSeedBallCells cells, or a limit that close: every point at
+  // least as close (within the limit) lies in a cell that the ball of that radius
+  // around q touches, so the rows of the ball's cell box are scanned instead of
+  // the rings.  The rows outside the box's z and y range are those a ring walk
+  // prunes (their gap exceeds the ball's radius, slack included), and so are the
+  // cells outside its x range; inside a row scan_ring_row prunes and trims by the
+  // current best as the rings do.  The result is the same point, in fewer row
+  // iterations per wavefront (at most 2 x 2 rows for a radius under one cell,
+  // against ring 0 and the nine rows of ring 1; about 16 for a query with no
+  // target point within a limit of 1.6 cells, against rings 0-2).
   if (seed_j >= 0) {
     const float thr = fminf(best, lim2) * 1.00002f;
     const float rho = sqrtf(thr) * g.inv_h * 1.00001f + c.slack;
-    if (rho < 1.0f) {
+    if (rho < kSeedBallCells) {
       const int za = max((int)floorf(c.fz - rho), 0), zb = min((int)floorf(c.fz + rho), g.nz - 1);
       const int ya = max((int)floorf(c.fy - rho), 0), yb = min((int)floorf(c.fy + rho), g.ny - 1);
-      const int x0 = max(c.cx - 1, 0), x1 = min(c.cx + 1, g.nx - 1);
+      const int x0 = max((int)floorf(c.fx - rho), 0), x1 = min((int)floorf(c.fx + rho), g.nx - 1);
       if (x0 <= x1) {
         for (int z = za; z <= zb; ++z) {
           const float gz = axis_gap(c.fz, z, c.cz, c.slack);
