@@ -307,7 +307,7 @@ LCR_SHD void ring_rows(const GridDesc& g, const int* __restrict__ cs, const floa
 
 // Ball radius (in cells) up to which a seeded query scans its ball's rows instead
 // of the rings.
-constexpr float kSeedBallCells = 2.0f;
+constexpr float kSeedBallCells = 3.0f;
 
 // Whether a bounded search (limit rlim) may stop after ring k with `best`: the
 // grid is exhausted, no unvisited cell can hold a point closer than best, or
@@ -377,7 +377,8 @@ LCR_SHD NNResult nn_query(const GridDesc& g, const int* __restrict__ cs,
   // current best as the rings do.  The result is the same point, in fewer row
   // iterations per wavefront (at most 2 x 2 rows for a radius under one cell,
   // against ring 0 and the nine rows of ring 1; about 16 for a query with no
-  // target point within a limit of 1.6 cells, against rings 0-2).
+  // target point within a limit of 1.6 cells, against rings 0-2; the unbounded
+  // fitness pass's seeds up to three cells away as well).
   if (seed_j >= 0) {
     const float thr = fminf(best, lim2) * 1.00002f;
     const float rho = sqrtf(thr) * g.inv_h * 1.00001f + c.slack;
